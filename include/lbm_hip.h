@@ -1,0 +1,239 @@
+/*
+ * lbm_hip.h — C ABI of the MI355X (gfx950) lattice-Boltzmann hot path.
+ *
+ * Drop-in boundary for lbm-core's `Backend` trait (TakuTsuzuki/lbmflow,
+ * crates/lbm-core/src/backend.rs:126-363) plus the Solver-side hooks that
+ * the reference keeps in host code (rotating IBM, scalar ADE, reducers).
+ * Plain C: pointers, sizes and status codes only.  Every entry point
+ * returns 0 on success or a negative LBM_ERR_* code; the message of the
+ * most recent failure on a context is available from lbm_last_error().
+ * No call throws or aborts across the ABI.
+ *
+ * Array conventions (host side):
+ *   cell     = (z * ny + y) * nx + x            compact global order
+ *   f[q][cell]                                  q-major planes, deviation form f - w_q
+ *   vec3[cell][3]                               interleaved xyz
+ * This is the reference's compact-core order (fields.rs:143-148) with the
+ * padded q-major planes (fields.rs:1-16) stripped of their halo.
+ *
+ * Threading: one HIP stream per context; calls on one context are
+ * serialized by the caller (the reference Solver is single-threaded too,
+ * gpu/backend.rs:38).  Readback calls block until the stream drains.
+ */
+#ifndef LBM_HIP_H
+#define LBM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (UnsupportedReason-like, solver.rs:172-180) ---------- */
+#define LBM_OK 0
+#define LBM_ERR_INVALID (-1)     /* bad argument / shape */
+#define LBM_ERR_UNSUPPORTED (-2) /* feature not implemented on this backend */
+#define LBM_ERR_DEVICE (-3)      /* HIP runtime failure */
+#define LBM_ERR_NOMEM (-4)       /* device allocation failed */
+#define LBM_ERR_STATE (-5)       /* call out of order (e.g. no markers set) */
+
+/* ---- enums -------------------------------------------------------------- */
+/* Population storage precision (gpu/backend.rs:143-159 has f32|f16; bf16 is
+ * new).  Storage always holds the deviation f - w_q; arithmetic is fp32. */
+#define LBM_STORAGE_F32 0
+#define LBM_STORAGE_F16 1
+#define LBM_STORAGE_BF16 2
+
+/* CollisionKind, params.rs:22-37 */
+#define LBM_COLLISION_BGK 0
+#define LBM_COLLISION_TRT 1
+#define LBM_COLLISION_CM 2
+
+/* Reduction, params.rs:411-419 */
+#define LBM_REDUCE_FLUID_CELLS 0
+#define LBM_REDUCE_MASS_DEVIATION 1
+#define LBM_REDUCE_MOMENTUM 2
+
+/* scalar distribution lattice: 0 = the hydro lattice (reference,
+ * solver.rs:2113-2205), 7 = D3Q7 (WSCAL spec; cs^2 = 1/4) */
+#define LBM_SCALAR_HYDRO_LATTICE 0
+#define LBM_SCALAR_D3Q7 7
+
+/* capability bits (backend.rs:157-199 probes) */
+#define LBM_CAP_TRT 0x1u
+#define LBM_CAP_CENTRAL_MOMENT 0x2u
+#define LBM_CAP_F16_STORAGE 0x4u
+#define LBM_CAP_BF16_STORAGE 0x8u
+#define LBM_CAP_FORCE_FIELD 0x10u
+#define LBM_CAP_OMEGA_FIELD 0x20u
+#define LBM_CAP_ROTATING_IBM 0x40u
+#define LBM_CAP_SCALAR 0x80u
+#define LBM_CAP_PERIODIC 0x100u
+#define LBM_CAP_OPEN_FACES 0x200u    /* not set: open-face BCs are not on this path */
+#define LBM_CAP_TWO_PASS 0x400u      /* not set: one fused pass per step */
+#define LBM_CAP_GRAVITY 0x800u       /* not set */
+#define LBM_CAP_PROBE 0x1000u        /* not set: momentum-exchange probe */
+
+typedef struct lbm_ctx lbm_ctx;
+typedef struct lbm_fields lbm_fields;
+
+/* StepParams, params.rs:302-320 (the subset on this path).  Relaxation rates
+ * stay f64 here and are rounded to fp32 when kernel constants are built,
+ * exactly as KParams::new does (params.rs:350-374). */
+typedef struct lbm_step_params {
+  int collision;      /* LBM_COLLISION_* */
+  double omega_p;     /* 1/tau */
+  double omega_m;     /* TRT antisymmetric rate (== omega_p for BGK/CM) */
+  double omega_shear; /* CM shear rate */
+  float force[3];     /* uniform Guo body force */
+} lbm_step_params;
+
+/* DirectForcingConfig, rotating_ibm.rs:104-130 */
+typedef struct lbm_ibm_config {
+  int max_iterations;    /* default 3 */
+  double slip_tolerance; /* default 1e-2 */
+  int kernel_radius;     /* 1 (linear) or 2 (quadratic B-spline) */
+  double relaxation;     /* default 1.0 */
+} lbm_ibm_config;
+
+/* IbmDiagnostics, rotating_ibm.rs:132-153 */
+typedef struct lbm_ibm_diag {
+  int iterations;
+  double slip_max, slip_rms, slip_max_rel, slip_rms_rel;
+  double torque[3];
+  double fluid_force[3];
+  double marker_force[3];
+  double momentum_error_rel;
+} lbm_ibm_diag;
+
+/* ---- context ------------------------------------------------------------ */
+/* No reference counterpart (wgpu device creation, gpu/backend.rs:1895+). */
+int lbm_ctx_create(int device, lbm_ctx** out);
+void lbm_ctx_destroy(lbm_ctx* ctx);
+const char* lbm_last_error(const lbm_ctx* ctx);
+/* Backend capability probes, backend.rs:157-199. */
+int lbm_capabilities(const lbm_ctx* ctx, uint32_t* flags);
+/* Host-visible description of the compiled kernels (for diagnostics). */
+const char* lbm_build_info(void);
+
+/* ---- fields ------------------------------------------------------------- */
+/* Backend::alloc, backend.rs:138-139: quiescent fields (f = 0, rho = 1,
+ * u = 0, no solids).  Q is 19 or 27.  periodic[a] != 0 wraps axis a
+ * (Subdomain::monolithic, subdomain.rs:31-44). */
+int lbm_fields_alloc(lbm_ctx* ctx, int Q, int storage, const int dims[3], const int periodic[3],
+                     lbm_fields** out);
+void lbm_fields_free(lbm_fields* fl);
+/* Device bytes held by this domain. */
+size_t lbm_fields_device_bytes(const lbm_fields* fl);
+
+/* Solver::build mask distribution, solver.rs:1829-1848: solid (1 byte per
+ * cell, nonzero = solid) and the per-cell wall velocity (meaningful on
+ * solids).  Either pointer may be NULL (all fluid / resting walls).
+ * A domain with a fluid cell on a non-periodic face cannot be stepped
+ * (open faces are not on this path): lbm_run_span then fails with
+ * LBM_ERR_UNSUPPORTED until the masks close it. */
+int lbm_set_masks(lbm_fields* fl, const uint8_t* solid, const float* wall_u);
+int lbm_read_solid(lbm_fields* fl, uint8_t* solid);
+/* Half-way bounce-back link list (implicit in stream_row, kernels.rs:572-603):
+ * all (cell, q) with cell fluid and cell - c_q solid, ordered by cell then q.
+ * With cells == NULL only *count is written. */
+int lbm_boundary_links(lbm_fields* fl, int64_t* count, int64_t* cells, uint8_t* qs, int64_t capacity);
+
+/* StepParams / KParams, params.rs:302-374. */
+int lbm_set_params(lbm_fields* fl, const lbm_step_params* p);
+/* Per-cell omega_plus (docs/KERNEL_EXTENSION_POINTS.md:22-73): NULL clears. */
+int lbm_set_omega_field(lbm_fields* fl, const float* omega);
+
+/* Backend::stage_in / stage_out, backend.rs:141-155 (compact q-major).
+ * stage_in makes the cached moments those of the staged populations with
+ * the current force field (as Solver::build's update_moments does);
+ * stage_out writes zero for solid cells. */
+int lbm_stage_in(lbm_fields* fl, const float* f);
+int lbm_stage_out(lbm_fields* fl, float* f);
+/* Solver::init_with, solver.rs:2474-2623: f = feq(rho, u) + f_neq(grad u),
+ * computed on the device from compact rho[N] and u[N][3]. */
+int lbm_init_with(lbm_fields* fl, const float* rho, const float* u);
+/* Backend::update_moments, backend.rs:243-249 (refresh the cached moments
+ * from the populations with the current force field). */
+int lbm_update_moments(lbm_fields* fl);
+
+/* Solver::set_body_force_field / clear_body_force_field,
+ * solver.rs:2776-2822 (compact vec3; refreshes moments only when uniform). */
+int lbm_set_force_field(lbm_fields* fl, const float* ff);
+int lbm_clear_force_field(lbm_fields* fl);
+int lbm_read_force_field(lbm_fields* fl, float* ff, int* present);
+
+/* Backend::run_span, backend.rs:262-328: `steps` whole steps (collide,
+ * stream with half-way bounce-back and moving walls, moments), plus the
+ * enabled scalar distributions after each hydro step (Solver::step,
+ * solver.rs:2237-2280).  Asynchronous: returns after enqueueing. */
+int lbm_run_span(lbm_fields* fl, int steps);
+int lbm_synchronize(lbm_fields* fl);
+/* Kernel timing (no reference counterpart; the reference times whole runs
+ * with Instant, runner.rs:1076): with timing on, HIP events bracket every
+ * main collide-stream launch on the context stream; _read returns the
+ * summed elapsed time and the launch count since the last read. */
+int lbm_kernel_timing(lbm_fields* fl, int enable);
+int lbm_kernel_timing_read(lbm_fields* fl, double* total_ms, int64_t* launches);
+/* Cells handled by the per-cell-force kernel (IBM stencils, force fields). */
+int64_t lbm_force_list_size(const lbm_fields* fl);
+/* Completed steps since alloc. */
+int64_t lbm_time(const lbm_fields* fl);
+
+/* Backend::read_moments, backend.rs:361-362 (compact; solids keep the
+ * values of the last explicit write, as in the reference). */
+int lbm_read_moments(lbm_fields* fl, float* rho, float* ux, float* uy, float* uz);
+/* Backend::reduce, backend.rs:351-359: f64 accumulation over fluid cells. */
+int lbm_reduce(lbm_fields* fl, int kind, int axis, double* out);
+
+/* ---- device geometry (bit-exact with geometry.rs) ----------------------- */
+/* build_wall_rims, solver.rs:1357-1406; is_wall[6] and u[6][3] in Face
+ * order XNeg, XPos, YNeg, YPos, ZNeg, ZPos. Writes solid + wall_u. */
+int lbm_geom_wall_rims(lbm_fields* fl, const int is_wall[6], const double u[18]);
+/* apply_cylindrical_tank_mask + apply_baffle_mask (count 4),
+ * geometry.rs:657-723.  Lengths in metres with cell size dx; baffle_count
+ * 0 or 4.  ORs into the current solid mask, then rebuilds links. */
+int lbm_geom_stirred_tank(lbm_fields* fl, double dx, double vessel_diameter, double liquid_height,
+                          int baffle_count, double baffle_width, double baffle_thickness);
+/* build_radial_impeller, geometry.rs:815-906 (host, libm trig).  blade
+ * angle 0 = Rushton (inner radius 0.65 R), else pitched (0.35 R).  Writes
+ * up to `capacity` markers (pos[n][3] in lattice units, weight[n],
+ * velocity[n][3]) and center[3]; *count receives the full count. */
+int lbm_geom_radial_impeller(const int dims[3], double dx, double vessel_diameter, double liquid_height,
+                             double diameter, double clearance, int blades, double blade_angle_deg,
+                             double omega_z, double thickness_fraction, double* pos, double* weight,
+                             double* velocity, double* center, int64_t capacity, int64_t* count);
+
+/* ---- rotating IBM (Solver::apply_rotating_ibm, solver.rs:3016-3282) ----- */
+/* Register a rigid marker body (RotatingBody::from_markers,
+ * rotating_ibm.rs:31-52).  Marker positions are frozen, as in the
+ * reference; the stencil cell sets are built here once. */
+int lbm_ibm_set_body(lbm_fields* fl, int64_t n, const double* pos, const double* weight, const double center[3],
+                     const double omega[3], const lbm_ibm_config* cfg);
+/* One direct-forcing update: adds the spread impulses to the force field
+ * (the caller clears it first, as the runner does, runner.rs:1080-1084). */
+int lbm_ibm_apply(lbm_fields* fl, lbm_ibm_diag* diag);
+
+/* ---- passive scalar ADE (solver.rs:2113-2205, scalar.rs:6-15) ------------ */
+int lbm_scalar_enable(lbm_fields* fl, double diffusivity, int lattice, int* index);
+/* set_scalar_concentration, solver.rs:2910-2965 (h = h_eq(C, u)). */
+int lbm_scalar_set(lbm_fields* fl, int index, const float* conc);
+/* gather_scalar_concentration, solver.rs:2887-2907. */
+int lbm_scalar_gather(lbm_fields* fl, int index, float* conc);
+/* qoi::scalar_cv over fluid cells (qoi.rs:507-532), f64 on the device.
+ * *defined = 0 when the mean is zero or there are no fluid cells. */
+int lbm_scalar_cv(lbm_fields* fl, int index, double* cv, int* defined);
+
+/* ---- QOI reducers (qoi.rs:477-556) --------------------------------------- */
+/* power_qois: out = {power, n_hz, np, p_over_v}. */
+int lbm_qoi_power(double torque, double omega, double rho, double diameter, double volume, double out[4]);
+/* mixing_time_from_cv: out = {cv0, t95, t99}; NaN where not reached.
+ * Returns LBM_ERR_STATE when no finite positive CV exists. */
+int lbm_qoi_mixing_time(int64_t n, const double* t, const double* cv, double out[3]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LBM_HIP_H */

@@ -1,0 +1,851 @@
+// lbm_api.cpp — C ABI over the HIP kernels: the Backend-phase host side
+// (context, fields, masks, staging, force fields, stepping, readback,
+// reductions, geometry, IBM, scalar ADE, QOI helpers).
+//
+// Reference interface: lbm-core's Backend trait (backend.rs:126-363) and the
+// Solver methods that drive it (solver.rs).  Errors never cross the ABI as
+// exceptions: each entry point returns an LBM_ERR_* code and records a
+// message on the context.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/lbm_hip.h"
+#include "lbm_aux.hpp"
+#include "lbm_internal.hpp"
+
+using namespace lbm;
+
+struct lbm_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+};
+
+struct lbm_fields {
+  lbm_ctx* ctx = nullptr;
+  int Q = 19;
+  int storage = kF32;
+  Dom d{};
+  Tiling t{};
+  size_t elem = 4;
+  void* f = nullptr;
+  uint32_t* mask = nullptr;
+  uint8_t* solid = nullptr;
+  float* wall_u = nullptr;
+  float* omega = nullptr;
+  int parity = 0;
+  int64_t time = 0;
+  int collision = LBM_COLLISION_TRT;
+  HydroParams hp{};
+  lbm_step_params params{};
+  bool params_set = false;
+  // force fields
+  float* F_cur = nullptr;
+  float* F_mom = nullptr;
+  bool field_present = false;
+  unsigned int open_face_cells = 0;
+  bool field_uniform = false;
+  std::vector<uint32_t> flist_host;
+  uint32_t* flist = nullptr;
+  size_t flist_cap = 0;
+  // reduction scratch
+  double* partials = nullptr;
+  uint32_t npartials = 0;
+  // kernel timing (HIP events around each main hydro launch)
+  bool timing = false;
+  std::vector<hipEvent_t> ev_pool;
+  size_t ev_used = 0;
+  // rotating IBM and scalar distributions
+  IbmState* ibm = nullptr;
+  std::vector<ScalarState*> scalars;
+  size_t bytes = 0;
+};
+
+namespace {
+
+int fail(lbm_ctx* c, int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (c) c->err = buf;
+  return code;
+}
+
+#define HIPCHK(ctx, expr)                                                                              \
+  do {                                                                                                 \
+    hipError_t _e = (expr);                                                                            \
+    if (_e != hipSuccess) return fail((ctx), LBM_ERR_DEVICE, "%s: %s", #expr, hipGetErrorString(_e)); \
+  } while (0)
+
+int dev_alloc(lbm_fields* fl, void** p, size_t bytes) {
+  if (bytes == 0) bytes = 1;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return fail(fl->ctx, LBM_ERR_NOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+  }
+  fl->bytes += bytes;
+  return LBM_OK;
+}
+
+void dev_free(lbm_fields* fl, void* p, size_t bytes) {
+  if (p) {
+    (void)hipFree(p);
+    fl->bytes -= std::min(fl->bytes, bytes);
+  }
+}
+
+size_t N3(const lbm_fields* fl) { return (size_t)fl->d.N * 3; }
+
+HydroParams build_hp(const lbm_step_params& p) {
+  // KParams::new, params.rs:350-374: rates rounded to fp32, Guo prefactors
+  // computed in f64 then rounded.
+  HydroParams h{};
+  h.omega_p = (float)p.omega_p;
+  h.omega_m = (float)p.omega_m;
+  h.omega_shear = (float)(p.collision == LBM_COLLISION_CM ? p.omega_shear : p.omega_p);
+  h.cp = (float)(1.0 - p.omega_p / 2.0);
+  h.cm = (float)(1.0 - p.omega_m / 2.0);
+  for (int a = 0; a < 3; ++a) h.F[a] = p.force[a];
+  h.force_on = (p.force[0] != 0.0f || p.force[1] != 0.0f || p.force[2] != 0.0f) ? 1 : 0;
+  return h;
+}
+
+int ensure_force_arrays(lbm_fields* fl) {
+  if (fl->F_cur) return LBM_OK;
+  int rc = dev_alloc(fl, (void**)&fl->F_cur, N3(fl) * 4);
+  if (rc) return rc;
+  rc = dev_alloc(fl, (void**)&fl->F_mom, N3(fl) * 4);
+  if (rc) return rc;
+  HIPCHK(fl->ctx, hipMemsetAsync(fl->F_cur, 0, N3(fl) * 4, fl->ctx->stream));
+  HIPCHK(fl->ctx, hipMemsetAsync(fl->F_mom, 0, N3(fl) * 4, fl->ctx->stream));
+  return LBM_OK;
+}
+
+// Upload the sorted union of the current list and `extra` cells, and flag
+// them in the mask so the main kernel leaves them to the list kernel.
+int grow_force_list(lbm_fields* fl, const std::vector<uint32_t>& extra) {
+  std::vector<uint32_t> merged;
+  merged.reserve(fl->flist_host.size() + extra.size());
+  std::merge(fl->flist_host.begin(), fl->flist_host.end(), extra.begin(), extra.end(), std::back_inserter(merged));
+  merged.erase(std::unique(merged.begin(), merged.end()), merged.end());
+  if (merged.size() == fl->flist_host.size()) return LBM_OK;
+  fl->flist_host.swap(merged);
+  if (fl->flist_host.size() > fl->flist_cap) {
+    dev_free(fl, fl->flist, fl->flist_cap * 4);
+    fl->flist = nullptr;
+    fl->flist_cap = std::max<size_t>(fl->flist_host.size(), 2 * fl->flist_cap);
+    int rc = dev_alloc(fl, (void**)&fl->flist, fl->flist_cap * 4);
+    if (rc) return rc;
+  }
+  HIPCHK(fl->ctx, hipMemcpyAsync(fl->flist, fl->flist_host.data(), fl->flist_host.size() * 4,
+                                 hipMemcpyHostToDevice, fl->ctx->stream));
+  HIPCHK(fl->ctx, launch_mark_forced(fl->flist, (uint32_t)fl->flist_host.size(), fl->mask, fl->ctx->stream));
+  // the host vector must outlive the async copy
+  HIPCHK(fl->ctx, hipStreamSynchronize(fl->ctx->stream));
+  return LBM_OK;
+}
+
+int rebuild_mask(lbm_fields* fl) {
+  unsigned int* err = nullptr;
+  int rc = dev_alloc(fl, (void**)&err, sizeof(unsigned int));
+  if (rc) return rc;
+  hipStream_t s = fl->ctx->stream;
+  HIPCHK(fl->ctx, hipMemsetAsync(err, 0, sizeof(unsigned int), s));
+  HIPCHK(fl->ctx, launch_build_mask(fl->Q, fl->d, fl->solid, fl->wall_u, fl->mask, err, s));
+  if (!fl->flist_host.empty())
+    HIPCHK(fl->ctx, launch_mark_forced(fl->flist, (uint32_t)fl->flist_host.size(), fl->mask, s));
+  unsigned int nerr = 0;
+  HIPCHK(fl->ctx, hipMemcpyAsync(&nerr, err, sizeof nerr, hipMemcpyDeviceToHost, s));
+  HIPCHK(fl->ctx, hipStreamSynchronize(s));
+  dev_free(fl, err, sizeof(unsigned int));
+  // Fluid on a non-periodic face needs an open-face BC, which is not on this
+  // path; stepping such a domain is refused (lbm_run_span).
+  fl->open_face_cells = nerr;
+  return LBM_OK;
+}
+
+HydroArgs hydro_args(lbm_fields* fl) {
+  HydroArgs a{};
+  a.d = fl->d;
+  a.t = fl->t;
+  a.f = fl->f;
+  a.mask = fl->mask;
+  a.wall_u = fl->wall_u;
+  a.omega = fl->omega;
+  a.p = fl->hp;
+  a.flist = fl->flist;
+  a.nlist = (uint32_t)fl->flist_host.size();
+  a.F_mom = fl->F_mom;
+  a.F_cur = fl->F_cur;
+  return a;
+}
+
+// Cached moments now carry the current field (Backend::update_moments).
+int refresh_moments(lbm_fields* fl) {
+  if (fl->F_mom && !fl->flist_host.empty())
+    HIPCHK(fl->ctx, launch_copy_force_list(fl->flist, (uint32_t)fl->flist_host.size(), fl->F_mom,
+                                           fl->field_present ? fl->F_cur : nullptr, fl->ctx->stream));
+  return LBM_OK;
+}
+
+}  // namespace
+
+// Accessors used by the auxiliary translation unit (IBM / scalar ADE).
+namespace lbm {
+FieldsView view_of(lbm_fields* fl) {
+  FieldsView v{};
+  v.ctx_err = &fl->ctx->err;
+  v.stream = fl->ctx->stream;
+  v.Q = fl->Q;
+  v.storage = fl->storage;
+  v.d = fl->d;
+  v.f = fl->f;
+  v.mask = fl->mask;
+  v.parity = fl->parity;
+  v.hp = fl->hp;
+  v.F_cur = fl->F_cur;
+  v.F_mom = fl->F_mom;
+  v.field_present = fl->field_present;
+  return v;
+}
+}  // namespace lbm
+
+extern "C" {
+
+const char* lbm_build_info(void) {
+  return "lbmflow_amd liblbm_hip: gfx950, AA-pattern fused collide-stream, TRT/BGK (D3Q19/D3Q27), "
+         "central moment (D3Q27), storage f32|f16|bf16 (shifted, 16-bit scaled by 1/w)";
+}
+
+int lbm_ctx_create(int device, lbm_ctx** out) {
+  if (!out) return LBM_ERR_INVALID;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return LBM_ERR_DEVICE;
+  if (device < 0 || device >= n) return LBM_ERR_INVALID;
+  if (hipSetDevice(device) != hipSuccess) return LBM_ERR_DEVICE;
+  lbm_ctx* c = new (std::nothrow) lbm_ctx();
+  if (!c) return LBM_ERR_NOMEM;
+  c->device = device;
+  if (hipStreamCreate(&c->stream) != hipSuccess) {
+    delete c;
+    return LBM_ERR_DEVICE;
+  }
+  *out = c;
+  return LBM_OK;
+}
+
+void lbm_ctx_destroy(lbm_ctx* ctx) {
+  if (!ctx) return;
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+const char* lbm_last_error(const lbm_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int lbm_capabilities(const lbm_ctx* ctx, uint32_t* flags) {
+  if (!ctx || !flags) return LBM_ERR_INVALID;
+  *flags = LBM_CAP_TRT | LBM_CAP_CENTRAL_MOMENT | LBM_CAP_F16_STORAGE | LBM_CAP_BF16_STORAGE | LBM_CAP_FORCE_FIELD |
+           LBM_CAP_OMEGA_FIELD | LBM_CAP_ROTATING_IBM | LBM_CAP_SCALAR | LBM_CAP_PERIODIC;
+  return LBM_OK;
+}
+
+int lbm_fields_alloc(lbm_ctx* ctx, int Q, int storage, const int dims[3], const int periodic[3],
+                     lbm_fields** out) {
+  if (!ctx || !out || !dims) return LBM_ERR_INVALID;
+  *out = nullptr;
+  if (Q != 19 && Q != 27) return fail(ctx, LBM_ERR_UNSUPPORTED, "lattice D3Q%d is not supported (19 or 27)", Q);
+  if (storage < 0 || storage > 2) return fail(ctx, LBM_ERR_INVALID, "storage %d", storage);
+  for (int a = 0; a < 3; ++a)
+    if (dims[a] < 1) return fail(ctx, LBM_ERR_INVALID, "dims must be positive");
+  const uint64_t n = (uint64_t)dims[0] * dims[1] * dims[2];
+  if (n >= (1ull << 31)) return fail(ctx, LBM_ERR_INVALID, "domain of %llu cells exceeds 2^31", (unsigned long long)n);
+  lbm_fields* fl = new (std::nothrow) lbm_fields();
+  if (!fl) return LBM_ERR_NOMEM;
+  fl->ctx = ctx;
+  fl->Q = Q;
+  fl->storage = storage;
+  fl->d.nx = dims[0];
+  fl->d.ny = dims[1];
+  fl->d.nz = dims[2];
+  fl->d.N = (uint32_t)n;
+  for (int a = 0; a < 3; ++a) fl->d.per[a] = periodic ? (periodic[a] != 0) : 0;
+  fl->t = make_tiling(fl->d, 0);
+  fl->elem = storage == kF32 ? 4 : 2;
+  int rc = dev_alloc(fl, &fl->f, (size_t)Q * n * fl->elem);
+  if (!rc) rc = dev_alloc(fl, (void**)&fl->mask, n * 4);
+  if (!rc) rc = dev_alloc(fl, (void**)&fl->solid, n);
+  fl->npartials = 1024;
+  if (!rc) rc = dev_alloc(fl, (void**)&fl->partials, fl->npartials * sizeof(double));
+  if (rc) {
+    lbm_fields_free(fl);
+    return rc;
+  }
+  hipStream_t s = ctx->stream;
+  if (hipMemsetAsync(fl->f, 0, (size_t)Q * n * fl->elem, s) != hipSuccess ||
+      hipMemsetAsync(fl->solid, 0, n, s) != hipSuccess) {
+    lbm_fields_free(fl);
+    return fail(ctx, LBM_ERR_DEVICE, "memset failed");
+  }
+  // default parameters: TRT with nu = 1/6 (tau = 1)
+  lbm_step_params p{};
+  p.collision = LBM_COLLISION_TRT;
+  p.omega_p = 1.0;
+  p.omega_m = 1.0 / ((3.0 / 16.0) / 0.5 + 0.5);
+  p.omega_shear = 1.0;
+  fl->params = p;
+  fl->hp = build_hp(p);
+  fl->collision = p.collision;
+  // all-fluid default (steppable only when every axis is periodic)
+  rc = rebuild_mask(fl);
+  if (rc) {
+    lbm_fields_free(fl);
+    return rc;
+  }
+  ctx->err.clear();
+  *out = fl;
+  return LBM_OK;
+}
+
+void lbm_fields_free(lbm_fields* fl) {
+  if (!fl) return;
+  (void)hipStreamSynchronize(fl->ctx->stream);
+  if (fl->ibm) ibm_destroy(fl->ibm);
+  for (ScalarState* s : fl->scalars) scalar_destroy(s);
+  for (hipEvent_t e : fl->ev_pool) (void)hipEventDestroy(e);
+  for (void* p : {(void*)fl->f, (void*)fl->mask, (void*)fl->solid, (void*)fl->wall_u, (void*)fl->omega,
+                  (void*)fl->F_cur, (void*)fl->F_mom, (void*)fl->flist, (void*)fl->partials})
+    if (p) (void)hipFree(p);
+  delete fl;
+}
+
+size_t lbm_fields_device_bytes(const lbm_fields* fl) { return fl ? fl->bytes : 0; }
+
+int lbm_set_masks(lbm_fields* fl, const uint8_t* solid, const float* wall_u) {
+  if (!fl) return LBM_ERR_INVALID;
+  hipStream_t s = fl->ctx->stream;
+  const size_t n = fl->d.N;
+  if (solid) {
+    HIPCHK(fl->ctx, hipMemcpyAsync(fl->solid, solid, n, hipMemcpyHostToDevice, s));
+  } else {
+    HIPCHK(fl->ctx, hipMemsetAsync(fl->solid, 0, n, s));
+  }
+  bool any = false;
+  if (wall_u)
+    for (size_t k = 0; k < 3 * n && !any; ++k) any = wall_u[k] != 0.0f;
+  if (any) {
+    if (!fl->wall_u) {
+      int rc = dev_alloc(fl, (void**)&fl->wall_u, 3 * n * 4);
+      if (rc) return rc;
+    }
+    HIPCHK(fl->ctx, hipMemcpyAsync(fl->wall_u, wall_u, 3 * n * 4, hipMemcpyHostToDevice, s));
+  } else if (fl->wall_u) {
+    dev_free(fl, fl->wall_u, 3 * n * 4);
+    fl->wall_u = nullptr;
+  }
+  HIPCHK(fl->ctx, hipStreamSynchronize(s));
+  int rc = rebuild_mask(fl);
+  if (rc) return rc;
+  return refresh_moments(fl);
+}
+
+int lbm_read_solid(lbm_fields* fl, uint8_t* solid) {
+  if (!fl || !solid) return LBM_ERR_INVALID;
+  HIPCHK(fl->ctx, hipMemcpyAsync(solid, fl->solid, fl->d.N, hipMemcpyDeviceToHost, fl->ctx->stream));
+  HIPCHK(fl->ctx, hipStreamSynchronize(fl->ctx->stream));
+  return LBM_OK;
+}
+
+int lbm_boundary_links(lbm_fields* fl, int64_t* count, int64_t* cells, uint8_t* qs, int64_t capacity) {
+  if (!fl || !count) return LBM_ERR_INVALID;
+  std::vector<uint32_t> m(fl->d.N);
+  HIPCHK(fl->ctx, hipMemcpyAsync(m.data(), fl->mask, (size_t)fl->d.N * 4, hipMemcpyDeviceToHost, fl->ctx->stream));
+  HIPCHK(fl->ctx, hipStreamSynchronize(fl->ctx->stream));
+  int64_t k = 0;
+  for (uint32_t i = 0; i < fl->d.N; ++i) {
+    if (m[i] & kMaskSolid) continue;
+    for (int q = 1; q < fl->Q; ++q)
+      if ((m[i] >> q) & 1u) {
+        if (cells && k < capacity) {
+          cells[k] = i;
+          qs[k] = (uint8_t)q;
+        }
+        ++k;
+      }
+  }
+  *count = k;
+  if (cells && k > capacity) return fail(fl->ctx, LBM_ERR_INVALID, "capacity %lld < %lld links", (long long)capacity,
+                                         (long long)k);
+  return LBM_OK;
+}
+
+int lbm_set_params(lbm_fields* fl, const lbm_step_params* p) {
+  if (!fl || !p) return LBM_ERR_INVALID;
+  if (p->collision < 0 || p->collision > 2) return fail(fl->ctx, LBM_ERR_INVALID, "collision %d", p->collision);
+  if (p->collision == LBM_COLLISION_CM && fl->Q != 27)
+    return fail(fl->ctx, LBM_ERR_UNSUPPORTED, "central-moment collision is implemented for D3Q27 only");
+  if (!(p->omega_p > 0.0 && p->omega_p < 2.0) || !(p->omega_m > 0.0 && p->omega_m < 2.0))
+    return fail(fl->ctx, LBM_ERR_INVALID, "relaxation rates must lie in (0, 2)");
+  fl->params = *p;
+  fl->hp = build_hp(*p);
+  fl->collision = p->collision;
+  fl->params_set = true;
+  return LBM_OK;
+}
+
+int lbm_set_omega_field(lbm_fields* fl, const float* omega) {
+  if (!fl) return LBM_ERR_INVALID;
+  const size_t n = fl->d.N;
+  if (!omega) {
+    HIPCHK(fl->ctx, hipStreamSynchronize(fl->ctx->stream));
+    dev_free(fl, fl->omega, n * 4);
+    fl->omega = nullptr;
+    return LBM_OK;
+  }
+  if (!fl->omega) {
+    int rc = dev_alloc(fl, (void**)&fl->omega, n * 4);
+    if (rc) return rc;
+  }
+  HIPCHK(fl->ctx, hipMemcpyAsync(fl->omega, omega, n * 4, hipMemcpyHostToDevice, fl->ctx->stream));
+  HIPCHK(fl->ctx, hipStreamSynchronize(fl->ctx->stream));
+  return LBM_OK;
+}
+
+int lbm_stage_in(lbm_fields* fl, const float* f) {
+  if (!fl || !f) return LBM_ERR_INVALID;
+  const size_t bytes = (size_t)fl->Q * fl->d.N * 4;
+  float* tmp = nullptr;
+  int rc = dev_alloc(fl, (void**)&tmp, bytes);
+  if (rc) return rc;
+  hipStream_t s = fl->ctx->stream;
+  HIPCHK(fl->ctx, hipMemcpyAsync(tmp, f, bytes, hipMemcpyHostToDevice, s));
+  HIPCHK(fl->ctx, launch_stage(fl->Q, fl->storage, 0, 0, fl->d, fl->f, tmp, fl->mask, s));
+  HIPCHK(fl->ctx, hipStreamSynchronize(s));
+  dev_free(fl, tmp, bytes);
+  fl->parity = 0;
+  return refresh_moments(fl);
+}
+
+int lbm_stage_out(lbm_fields* fl, float* f) {
+  if (!fl || !f) return LBM_ERR_INVALID;
+  const size_t bytes = (size_t)fl->Q * fl->d.N * 4;
+  float* tmp = nullptr;
+  int rc = dev_alloc(fl, (void**)&tmp, bytes);
+  if (rc) return rc;
+  hipStream_t s = fl->ctx->stream;
+  HIPCHK(fl->ctx, launch_stage(fl->Q, fl->storage, 1, fl->parity, fl->d, fl->f, tmp, fl->mask, s));
+  HIPCHK(fl->ctx, hipMemcpyAsync(f, tmp, bytes, hipMemcpyDeviceToHost, s));
+  HIPCHK(fl->ctx, hipStreamSynchronize(s));
+  dev_free(fl, tmp, bytes);
+  return LBM_OK;
+}
+
+int lbm_init_with(lbm_fields* fl, const float* rho, const float* u) {
+  if (!fl || !rho || !u) return LBM_ERR_INVALID;
+  const size_t n = fl->d.N;
+  float *dr = nullptr, *du = nullptr;
+  int rc = dev_alloc(fl, (void**)&dr, n * 4);
+  if (!rc) rc = dev_alloc(fl, (void**)&du, 3 * n * 4);
+  if (rc) return rc;
+  hipStream_t s = fl->ctx->stream;
+  HIPCHK(fl->ctx, hipMemcpyAsync(dr, rho, n * 4, hipMemcpyHostToDevice, s));
+  HIPCHK(fl->ctx, hipMemcpyAsync(du, u, 3 * n * 4, hipMemcpyHostToDevice, s));
+  // tau = T::r(3 nu + 1/2) with nu recovered from omega_p (solver.rs:2478)
+  const double nu = (1.0 / fl->params.omega_p - 0.5) / 3.0;
+  const float tau = (float)(3.0 * nu + 0.5);
+  HIPCHK(fl->ctx, launch_init_with(fl->Q, fl->storage, fl->d, fl->f, fl->mask, dr, du, tau, s));
+  HIPCHK(fl->ctx, hipStreamSynchronize(s));
+  dev_free(fl, dr, n * 4);
+  dev_free(fl, du, 3 * n * 4);
+  fl->parity = 0;
+  return refresh_moments(fl);
+}
+
+int lbm_update_moments(lbm_fields* fl) {
+  if (!fl) return LBM_ERR_INVALID;
+  return refresh_moments(fl);
+}
+
+int lbm_set_force_field(lbm_fields* fl, const float* ff) {
+  if (!fl) return LBM_ERR_INVALID;
+  if (!ff) return lbm_clear_force_field(fl);
+  int rc = ensure_force_arrays(fl);
+  if (rc) return rc;
+  const size_t n = fl->d.N;
+  std::vector<uint32_t> extra;
+  bool uniform = true;
+  for (size_t i = 0; i < n; ++i) {
+    const float* v = ff + 3 * i;
+    if (v[0] != 0.0f || v[1] != 0.0f || v[2] != 0.0f) extra.push_back((uint32_t)i);
+    if (uniform && (v[0] != ff[0] || v[1] != ff[1] || v[2] != ff[2])) uniform = false;
+  }
+  hipStream_t s = fl->ctx->stream;
+  HIPCHK(fl->ctx, hipMemcpyAsync(fl->F_cur, ff, 3 * n * 4, hipMemcpyHostToDevice, s));
+  HIPCHK(fl->ctx, hipStreamSynchronize(s));
+  rc = grow_force_list(fl, extra);
+  if (rc) return rc;
+  fl->field_present = true;
+  fl->field_uniform = uniform;
+  if (uniform) return refresh_moments(fl);
+  return LBM_OK;
+}
+
+int lbm_clear_force_field(lbm_fields* fl) {
+  if (!fl) return LBM_ERR_INVALID;
+  if (!fl->field_present) return LBM_OK;
+  const bool refresh = fl->field_uniform;
+  if (!fl->flist_host.empty())
+    HIPCHK(fl->ctx, launch_copy_force_list(fl->flist, (uint32_t)fl->flist_host.size(), fl->F_cur, nullptr,
+                                           fl->ctx->stream));
+  fl->field_present = false;
+  fl->field_uniform = false;
+  if (refresh) return refresh_moments(fl);
+  return LBM_OK;
+}
+
+int lbm_read_force_field(lbm_fields* fl, float* ff, int* present) {
+  if (!fl || !ff) return LBM_ERR_INVALID;
+  if (present) *present = fl->field_present ? 1 : 0;
+  if (!fl->field_present || !fl->F_cur) {
+    std::fill(ff, ff + N3(fl), 0.0f);
+    return LBM_OK;
+  }
+  HIPCHK(fl->ctx, hipMemcpyAsync(ff, fl->F_cur, N3(fl) * 4, hipMemcpyDeviceToHost, fl->ctx->stream));
+  HIPCHK(fl->ctx, hipStreamSynchronize(fl->ctx->stream));
+  return LBM_OK;
+}
+
+int lbm_run_span(lbm_fields* fl, int steps) {
+  if (!fl || steps < 0) return LBM_ERR_INVALID;
+  if (fl->open_face_cells)
+    return fail(fl->ctx, LBM_ERR_UNSUPPORTED,
+                "%u fluid cells lie on a non-periodic domain face; open faces are not supported "
+                "(close the domain with solid rims or make the axis periodic)",
+                fl->open_face_cells);
+  hipStream_t s = fl->ctx->stream;
+  for (int k = 0; k < steps; ++k) {
+    HydroArgs a = hydro_args(fl);
+    if (fl->timing) {
+      if (fl->ev_used + 2 > fl->ev_pool.size()) {
+        for (int k2 = 0; k2 < 2; ++k2) {
+          hipEvent_t e;
+          HIPCHK(fl->ctx, hipEventCreate(&e));
+          fl->ev_pool.push_back(e);
+        }
+      }
+      HIPCHK(fl->ctx, hipEventRecord(fl->ev_pool[fl->ev_used], s));
+    }
+    HIPCHK(fl->ctx, launch_hydro(fl->Q, fl->storage, fl->collision, fl->parity, a, s));
+    if (fl->timing) {
+      HIPCHK(fl->ctx, hipEventRecord(fl->ev_pool[fl->ev_used + 1], s));
+      fl->ev_used += 2;
+    }
+    if (a.nlist) HIPCHK(fl->ctx, launch_hydro_list(fl->Q, fl->storage, fl->collision, fl->parity, a, s));
+    fl->parity ^= 1;
+    fl->time += 1;
+    for (ScalarState* sc : fl->scalars) {
+      int rc = scalar_step(view_of(fl), sc);
+      if (rc) return rc;
+    }
+  }
+  return LBM_OK;
+}
+
+int lbm_kernel_timing(lbm_fields* fl, int enable) {
+  if (!fl) return LBM_ERR_INVALID;
+  HIPCHK(fl->ctx, hipStreamSynchronize(fl->ctx->stream));
+  fl->timing = enable != 0;
+  fl->ev_used = 0;
+  return LBM_OK;
+}
+
+int lbm_kernel_timing_read(lbm_fields* fl, double* total_ms, int64_t* launches) {
+  if (!fl || !total_ms || !launches) return LBM_ERR_INVALID;
+  HIPCHK(fl->ctx, hipStreamSynchronize(fl->ctx->stream));
+  double acc = 0.0;
+  for (size_t k = 0; k + 1 < fl->ev_used; k += 2) {
+    float ms = 0.0f;
+    HIPCHK(fl->ctx, hipEventElapsedTime(&ms, fl->ev_pool[k], fl->ev_pool[k + 1]));
+    acc += ms;
+  }
+  *total_ms = acc;
+  *launches = (int64_t)(fl->ev_used / 2);
+  fl->ev_used = 0;
+  return LBM_OK;
+}
+
+int64_t lbm_force_list_size(const lbm_fields* fl) { return fl ? (int64_t)fl->flist_host.size() : -1; }
+
+int lbm_synchronize(lbm_fields* fl) {
+  if (!fl) return LBM_ERR_INVALID;
+  HIPCHK(fl->ctx, hipStreamSynchronize(fl->ctx->stream));
+  return LBM_OK;
+}
+
+int64_t lbm_time(const lbm_fields* fl) { return fl ? fl->time : -1; }
+
+int lbm_read_moments(lbm_fields* fl, float* rho, float* ux, float* uy, float* uz) {
+  if (!fl || !rho || !ux || !uy || !uz) return LBM_ERR_INVALID;
+  const size_t n = fl->d.N;
+  float* buf = nullptr;
+  int rc = dev_alloc(fl, (void**)&buf, 4 * n * 4);
+  if (rc) return rc;
+  hipStream_t s = fl->ctx->stream;
+  HIPCHK(fl->ctx, launch_moments(fl->Q, fl->storage, fl->parity, fl->d, fl->f, fl->mask, fl->F_mom, fl->hp, buf,
+                                 buf + n, buf + 2 * n, buf + 3 * n, s));
+  HIPCHK(fl->ctx, hipMemcpyAsync(rho, buf, n * 4, hipMemcpyDeviceToHost, s));
+  HIPCHK(fl->ctx, hipMemcpyAsync(ux, buf + n, n * 4, hipMemcpyDeviceToHost, s));
+  HIPCHK(fl->ctx, hipMemcpyAsync(uy, buf + 2 * n, n * 4, hipMemcpyDeviceToHost, s));
+  HIPCHK(fl->ctx, hipMemcpyAsync(uz, buf + 3 * n, n * 4, hipMemcpyDeviceToHost, s));
+  HIPCHK(fl->ctx, hipStreamSynchronize(s));
+  dev_free(fl, buf, 4 * n * 4);
+  return LBM_OK;
+}
+
+int lbm_reduce(lbm_fields* fl, int kind, int axis, double* out) {
+  if (!fl || !out || kind < 0 || kind > 2 || axis < 0 || axis > 2) return LBM_ERR_INVALID;
+  hipStream_t s = fl->ctx->stream;
+  const uint32_t np = std::min<uint32_t>(fl->npartials, (fl->d.N + 255) / 256);
+  HIPCHK(fl->ctx, launch_reduce(fl->Q, fl->storage, fl->parity, fl->d, fl->f, fl->mask,
+                                fl->field_present ? fl->F_cur : nullptr, fl->hp, kind, axis, fl->partials, np, s));
+  std::vector<double> h(np);
+  HIPCHK(fl->ctx, hipMemcpyAsync(h.data(), fl->partials, np * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCHK(fl->ctx, hipStreamSynchronize(s));
+  double acc = 0.0;
+  for (double v : h) acc += v;
+  *out = acc;
+  return LBM_OK;
+}
+
+int lbm_geom_wall_rims(lbm_fields* fl, const int is_wall[6], const double u[18]) {
+  if (!fl || !is_wall || !u) return LBM_ERR_INVALID;
+  const size_t n = fl->d.N;
+  bool moving = false;
+  for (int fc = 0; fc < 6; ++fc)
+    if (is_wall[fc] && (u[3 * fc] != 0.0 || u[3 * fc + 1] != 0.0 || u[3 * fc + 2] != 0.0)) moving = true;
+  hipStream_t s = fl->ctx->stream;
+  if (moving && !fl->wall_u) {
+    int rc = dev_alloc(fl, (void**)&fl->wall_u, 3 * n * 4);
+    if (rc) return rc;
+    HIPCHK(fl->ctx, hipMemsetAsync(fl->wall_u, 0, 3 * n * 4, s));
+  }
+  float* wu = fl->wall_u;
+  float* scratch_wu = nullptr;
+  if (!wu) {  // resting walls: velocities are all zero, write into a throwaway buffer
+    int rc = dev_alloc(fl, (void**)&scratch_wu, 3 * n * 4);
+    if (rc) return rc;
+    wu = scratch_wu;
+  }
+  double* best = nullptr;
+  int rc = dev_alloc(fl, (void**)&best, n * 8);
+  if (rc) return rc;
+  HIPCHK(fl->ctx, launch_rims(fl->d, is_wall, u, fl->solid, wu, best, s));
+  HIPCHK(fl->ctx, hipStreamSynchronize(s));
+  dev_free(fl, best, n * 8);
+  if (scratch_wu) dev_free(fl, scratch_wu, 3 * n * 4);
+  rc = rebuild_mask(fl);
+  if (rc) return rc;
+  return refresh_moments(fl);
+}
+
+int lbm_geom_stirred_tank(lbm_fields* fl, double dx, double vessel_diameter, double liquid_height, int baffle_count,
+                          double baffle_width, double baffle_thickness) {
+  if (!fl) return LBM_ERR_INVALID;
+  if (!(dx > 0.0) || !(vessel_diameter > 0.0) || !(liquid_height > 0.0))
+    return fail(fl->ctx, LBM_ERR_INVALID, "tank dimensions must be finite and positive");
+  if (fl->d.nx < 3 || fl->d.ny < 3 || fl->d.nz < 3)
+    return fail(fl->ctx, LBM_ERR_INVALID, "stirred-tank grid dimensions must each be >= 3");
+  if (baffle_count != 0 && baffle_count != 4)
+    return fail(fl->ctx, LBM_ERR_UNSUPPORTED, "custom baffle placement is not implemented; use count = 4");
+  HIPCHK(fl->ctx, launch_tank(fl->d, dx, vessel_diameter, liquid_height, baffle_count, baffle_width,
+                              baffle_thickness, fl->solid, fl->ctx->stream));
+  int rc = rebuild_mask(fl);
+  if (rc) return rc;
+  return refresh_moments(fl);
+}
+
+int lbm_geom_radial_impeller(const int dims[3], double dx, double vessel_diameter, double liquid_height,
+                             double diameter, double clearance, int blades, double blade_angle_deg, double omega_z,
+                             double thickness_fraction, double* pos, double* weight, double* velocity,
+                             double* center, int64_t capacity, int64_t* count) {
+  if (!dims || !count) return LBM_ERR_INVALID;
+  if (!(blade_angle_deg >= 0.0 && blade_angle_deg <= 90.0) || blades <= 0 || !(diameter > 0.0) || !(dx > 0.0))
+    return LBM_ERR_INVALID;
+  // build_radial_impeller, geometry.rs:815-906
+  const double TAU = 6.283185307179586;
+  const double cxy[2] = {0.5 * (double)dims[0] * dx, 0.5 * (double)dims[1] * dx};
+  const double radius = 0.5 * diameter;
+  const double ctr[3] = {cxy[0], cxy[1], clearance};
+  const double radial_inner = blade_angle_deg == 0.0 ? 0.65 * radius : 0.35 * radius;
+  const double dr = std::max(dx, radius / 20.0);
+  const double dz = dx, tstep = dx;
+  const double thickness = std::max(thickness_fraction * diameter, dx);
+  int64_t k = 0;
+  for (int b = 0; b < blades; ++b) {
+    const double th = TAU * (double)b / (double)blades;
+    const double er[2] = {std::cos(th), std::sin(th)};
+    const double et[2] = {-std::sin(th), std::cos(th)};
+    const double pitch = blade_angle_deg * (3.141592653589793 / 180.0);
+    const double z_span =
+        blade_angle_deg == 0.0 ? thickness : thickness + (radius - radial_inner) * std::sin(pitch) * 0.18;
+    const long nr = (long)std::max(std::ceil((radius - radial_inner) / dr), 1.0);
+    const long nt = (long)std::max(std::ceil(thickness / tstep), 1.0);
+    const long nzs = (long)std::max(std::ceil(z_span / dz), 1.0);
+    for (long ir = 0; ir <= nr; ++ir) {
+      const double rr = radial_inner + (radius - radial_inner) * (double)ir / (double)nr;
+      for (long it = 0; it <= nt; ++it) {
+        const double ot = -0.5 * thickness + thickness * (double)it / (double)nt;
+        for (long iz = 0; iz <= nzs; ++iz) {
+          const double oz = -0.5 * z_span + z_span * (double)iz / (double)nzs;
+          const double pz = oz + (rr - radial_inner) * std::sin(pitch) * 0.18;
+          const double p[3] = {ctr[0] + rr * er[0] + ot * et[0], ctr[1] + rr * er[1] + ot * et[1], ctr[2] + pz};
+          const double rel = std::sqrt((p[0] - cxy[0]) * (p[0] - cxy[0]) + (p[1] - cxy[1]) * (p[1] - cxy[1]));
+          if (rel >= 0.5 * vessel_diameter || p[2] <= 0.0 || p[2] >= liquid_height) continue;
+          if (pos && k < capacity) {
+            const double r3[3] = {p[0] - ctr[0], p[1] - ctr[1], p[2] - ctr[2]};
+            const double om[3] = {0.0, 0.0, omega_z};
+            velocity[3 * k + 0] = om[1] * r3[2] - om[2] * r3[1];
+            velocity[3 * k + 1] = om[2] * r3[0] - om[0] * r3[2];
+            velocity[3 * k + 2] = om[0] * r3[1] - om[1] * r3[0];
+            pos[3 * k + 0] = p[0] / dx;
+            pos[3 * k + 1] = p[1] / dx;
+            pos[3 * k + 2] = p[2] / dx;
+            weight[k] = dx * dx;
+          }
+          ++k;
+        }
+      }
+    }
+  }
+  if (center) {
+    center[0] = ctr[0] / dx;
+    center[1] = ctr[1] / dx;
+    center[2] = ctr[2] / dx;
+  }
+  *count = k;
+  return k == 0 ? LBM_ERR_INVALID : LBM_OK;
+}
+
+// ---- rotating IBM -----------------------------------------------------------
+int lbm_ibm_set_body(lbm_fields* fl, int64_t n, const double* pos, const double* weight, const double center[3],
+                     const double omega[3], const lbm_ibm_config* cfg) {
+  if (!fl || n <= 0 || !pos || !weight || !center || !omega) return LBM_ERR_INVALID;
+  lbm_ibm_config c{3, 1e-2, 1, 1.0};
+  if (cfg) c = *cfg;
+  if (c.max_iterations <= 0 || !(c.slip_tolerance >= 0.0) || !(c.relaxation > 0.0 && c.relaxation <= 1.0) ||
+      (c.kernel_radius != 1 && c.kernel_radius != 2))
+    return fail(fl->ctx, LBM_ERR_INVALID, "invalid direct-forcing configuration");
+  if (fl->ibm) {
+    ibm_destroy(fl->ibm);
+    fl->ibm = nullptr;
+  }
+  std::vector<uint32_t> cells;
+  std::string err;
+  IbmState* st = ibm_create(view_of(fl), n, pos, weight, center, omega, c, &cells, &err);
+  if (!st) return fail(fl->ctx, LBM_ERR_INVALID, "%s", err.c_str());
+  fl->ibm = st;
+  int rc = ensure_force_arrays(fl);
+  if (rc) return rc;
+  return grow_force_list(fl, cells);
+}
+
+int lbm_ibm_apply(lbm_fields* fl, lbm_ibm_diag* diag) {
+  if (!fl || !diag) return LBM_ERR_INVALID;
+  if (!fl->ibm) return fail(fl->ctx, LBM_ERR_STATE, "no IBM body registered");
+  int rc = ibm_apply(view_of(fl), fl->ibm, diag);
+  if (rc) return rc;
+  fl->field_present = true;
+  fl->field_uniform = false;
+  return LBM_OK;
+}
+
+// ---- scalar ADE -------------------------------------------------------------
+int lbm_scalar_enable(lbm_fields* fl, double diffusivity, int lattice, int* index) {
+  if (!fl || !index) return LBM_ERR_INVALID;
+  if (!std::isfinite(diffusivity) || !(diffusivity > 0.0))
+    return fail(fl->ctx, LBM_ERR_INVALID, "diffusivity must be finite > 0");
+  if (lattice != LBM_SCALAR_HYDRO_LATTICE && lattice != LBM_SCALAR_D3Q7)
+    return fail(fl->ctx, LBM_ERR_INVALID, "scalar lattice %d", lattice);
+  std::string err;
+  ScalarState* s = scalar_create(view_of(fl), diffusivity, lattice == LBM_SCALAR_D3Q7 ? 7 : fl->Q, &err);
+  if (!s) return fail(fl->ctx, LBM_ERR_NOMEM, "%s", err.c_str());
+  fl->scalars.push_back(s);
+  *index = (int)fl->scalars.size() - 1;
+  return LBM_OK;
+}
+
+int lbm_scalar_set(lbm_fields* fl, int index, const float* conc) {
+  if (!fl || !conc || index < 0 || index >= (int)fl->scalars.size()) return LBM_ERR_INVALID;
+  return scalar_set(view_of(fl), fl->scalars[index], conc);
+}
+
+int lbm_scalar_gather(lbm_fields* fl, int index, float* conc) {
+  if (!fl || !conc || index < 0 || index >= (int)fl->scalars.size()) return LBM_ERR_INVALID;
+  return scalar_gather(view_of(fl), fl->scalars[index], conc);
+}
+
+int lbm_scalar_cv(lbm_fields* fl, int index, double* cv, int* defined) {
+  if (!fl || !cv || !defined || index < 0 || index >= (int)fl->scalars.size()) return LBM_ERR_INVALID;
+  return scalar_cv(view_of(fl), fl->scalars[index], cv, defined);
+}
+
+// ---- QOIs -------------------------------------------------------------------
+int lbm_qoi_power(double torque, double omega, double rho, double diameter, double volume, double out[4]) {
+  // power_qois, qoi.rs:477-505
+  if (!out) return LBM_ERR_INVALID;
+  if (!(std::isfinite(omega) && std::isfinite(rho) && std::isfinite(diameter) && std::isfinite(volume) && rho > 0.0 &&
+        diameter > 0.0 && volume > 0.0))
+    return LBM_ERR_INVALID;
+  const double n_hz = omega / 6.283185307179586;
+  if (!(n_hz > 0.0)) return LBM_ERR_INVALID;
+  const double power = omega * torque;
+  const double denom = rho * (n_hz * n_hz * n_hz) * (diameter * diameter * diameter * diameter * diameter);
+  out[0] = power;
+  out[1] = n_hz;
+  out[2] = power / denom;
+  out[3] = power / volume;
+  return LBM_OK;
+}
+
+int lbm_qoi_mixing_time(int64_t n, const double* t, const double* cv, double out[3]) {
+  // mixing_time_from_cv, qoi.rs:534-556
+  if (!out || n < 0 || (n > 0 && (!t || !cv))) return LBM_ERR_INVALID;
+  int64_t first = -1;
+  for (int64_t i = 0; i < n; ++i)
+    if (std::isfinite(cv[i]) && cv[i] > 0.0) {
+      first = i;
+      break;
+    }
+  if (first < 0) return LBM_ERR_STATE;
+  const double cv0 = cv[first];
+  double t95 = NAN, t99 = NAN;
+  bool h95 = false, h99 = false;
+  for (int64_t i = 0; i < n; ++i) {
+    if (!std::isfinite(cv[i])) continue;
+    if (!h95 && cv[i] <= 0.05 * cv0) {
+      t95 = t[i];
+      h95 = true;
+    }
+    if (!h99 && cv[i] <= 0.01 * cv0) {
+      t99 = t[i];
+      h99 = true;
+    }
+  }
+  out[0] = cv0;
+  out[1] = t95;
+  out[2] = t99;
+  return LBM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,718 @@
+// lbm_aux.hip — rotating immersed-boundary direct forcing and the passive
+// scalar advection-diffusion distributions on the device.
+//
+// IBM (Solver::apply_rotating_ibm, solver.rs:3016-3282, rotating_ibm.rs):
+// marker positions are frozen, as in the reference, so the marker stencils
+// (rotating_ibm.rs:163-224) are built once on the host.  Every per-cell
+// quantity the reference accumulates over markers in marker order
+// (cell_kernel_sum, spread, du) is gathered here cell-centrically over a
+// marker-ordered CSR list, so the f64 sums follow the reference's order and
+// the force field is reproduced bit for bit.  Per-marker quantities
+// (interpolated velocity, mobility, impulse) are marker-parallel.  The
+// diagnostics (torque, net forces, slip statistics) are reduced over markers
+// with a fixed tree.  The slip-tolerance early exit is a device flag, so the
+// host never waits inside a sweep.
+//
+// Scalar ADE: BGK collide toward h_eq = w C (1 + c.u/cs^2) with the cached
+// concentration and the post-step hydro velocity, pull streaming with
+// half-way bounce-back at solids, C = sum h (solver.rs:2113-2205,
+// scalar.rs:6-15).  The hydro lattice (reference) and D3Q7 (cs^2 = 1/4) are
+// both supported.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "lbm_aux.hpp"
+#include "lbm_device.hpp"
+
+namespace lbm {
+
+namespace {
+
+inline uint32_t nblk(size_t n, uint32_t bs = 256) { return (uint32_t)((n + bs - 1) / bs); }
+
+#define AUXCHK(v, expr)                                                      \
+  do {                                                                       \
+    hipError_t _e = (expr);                                                  \
+    if (_e != hipSuccess) {                                                  \
+      if ((v).ctx_err) *(v).ctx_err = std::string(#expr ": ") + hipGetErrorString(_e); \
+      return LBM_ERR_DEVICE;                                                 \
+    }                                                                        \
+  } while (0)
+
+}  // namespace
+
+// ===========================================================================
+// rotating IBM
+// ===========================================================================
+struct IbmDevDiag {
+  int iterations;
+  int converged;
+  double slip_max, slip_rms, slip_max_rel, slip_rms_rel;
+  double torque[3], fluid_force[3], marker_force[3];
+  double mom_err;
+};
+
+struct IbmState {
+  int64_t M = 0;
+  uint32_t P = 0, U = 0;
+  lbm_ibm_config cfg{};
+  double center[3], omega[3], scale = 1.0;
+  // marker-major
+  double* pos = nullptr;     // M x 3
+  double* weight = nullptr;  // M
+  double* target = nullptr;  // M x 3
+  uint32_t* mk_off = nullptr;  // M + 1
+  uint32_t* pt_u = nullptr;    // P: unique-cell index
+  uint32_t* pt_m = nullptr;    // P: marker index
+  double* pt_w = nullptr;      // P
+  // cell-major
+  uint32_t* uc_cell = nullptr;  // U
+  uint32_t* uc_off = nullptr;   // U + 1
+  uint32_t* uc_ent = nullptr;   // P, marker order within a cell
+  double* ksum = nullptr;       // U
+  // per-apply work
+  double* unow = nullptr;    // U x 3
+  double* rho = nullptr;     // U
+  double* du = nullptr;      // U x 3
+  double* spread = nullptr;  // U x 3
+  double* imp = nullptr;     // M x 3
+  double* mrep = nullptr;    // M x 3
+  double* mtq = nullptr;     // M x 3
+  double* mslip = nullptr;   // M
+  uint8_t* mact = nullptr;   // M
+  IbmDevDiag* diag = nullptr;
+};
+
+// u_now = (m + F/2) / rho at the stencil cells, rho the cached density
+// (solver.rs:3040-3084: f64 momentum of the populations, force without the
+// just-cleared field).
+template <int Q, typename S>
+__global__ void k_ibm_gather(Dom d, const S* f, int odd, const uint32_t* uc_cell, uint32_t U, const float* F_cur,
+                             int field_present, HydroParams hp, double* unow, double* rho, double* du,
+                             double* spread) {
+  const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= U) return;
+  const uint32_t i = uc_cell[u];
+  uint32_t x, y, z;
+  cell_xyz(d, i, x, y, z);
+  const Nbr nb(d, x, y, z);
+  float dr = 0.0f;
+  double mom[3] = {0.0, 0.0, 0.0};
+  sfor<0, Q>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    const float v = read_pop<Q, S, q>(f, d.N, nb, i, odd != 0);
+    dr = dr + v;
+    if constexpr (lat_c(q, 0) != 0) mom[0] += (double)lat_c(q, 0) * (double)v;
+    if constexpr (lat_c(q, 1) != 0) mom[1] += (double)lat_c(q, 1) * (double)v;
+    if constexpr (lat_c(q, 2) != 0) mom[2] += (double)lat_c(q, 2) * (double)v;
+  });
+  const float r = 1.0f + dr;
+  double force[3] = {(double)hp.F[0], (double)hp.F[1], (double)hp.F[2]};
+  if (field_present && F_cur)
+    for (int a = 0; a < 3; ++a) force[a] += (double)F_cur[3 * (size_t)i + a];
+  const double inv_rho = 1.0 / fmax((double)r, 1.0e-30);
+  for (int a = 0; a < 3; ++a) {
+    unow[3 * (size_t)u + a] = (mom[a] + 0.5 * force[a]) * inv_rho;
+    du[3 * (size_t)u + a] = 0.0;
+    spread[3 * (size_t)u + a] = 0.0;
+  }
+  rho[u] = (double)r;
+}
+
+__global__ void k_ibm_reset(IbmDevDiag* diag) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    memset(diag, 0, sizeof(IbmDevDiag));
+  }
+}
+
+// per marker: interpolated velocity, mobility row sum, impulse
+__global__ void k_ibm_impulse(int64_t M, const uint32_t* mk_off, const uint32_t* pt_u, const double* pt_w,
+                              const double* target, const double* unow, const double* du, const double* rho,
+                              const double* ksum, double relax, double* imp, const IbmDevDiag* diag) {
+  const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= M || diag->converged) return;
+  double um[3] = {0.0, 0.0, 0.0}, mob = 0.0;
+  for (uint32_t p = mk_off[m]; p < mk_off[m + 1]; ++p) {
+    const uint32_t u = pt_u[p];
+    const double w = pt_w[p];
+    for (int a = 0; a < 3; ++a) um[a] += w * (unow[3 * (size_t)u + a] + du[3 * (size_t)u + a]);
+    mob += w * ksum[u] / fmax(rho[u], 1.0e-30);
+  }
+  for (int a = 0; a < 3; ++a) {
+    const double slip = target[3 * m + a] - um[a];
+    imp[3 * m + a] = mob == 0.0 ? 0.0 : relax * slip / mob;
+  }
+}
+
+// per cell: spread and du accumulated over the cell's markers in marker order
+__global__ void k_ibm_spread(uint32_t U, const uint32_t* uc_off, const uint32_t* uc_ent, const uint32_t* pt_m,
+                             const double* pt_w, const double* imp, const double* rho, double* spread, double* du,
+                             const IbmDevDiag* diag) {
+  const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= U || diag->converged) return;
+  const double inv_rho = 1.0 / fmax(rho[u], 1.0e-30);
+  double s[3], dd[3];
+  for (int a = 0; a < 3; ++a) {
+    s[a] = spread[3 * (size_t)u + a];
+    dd[a] = du[3 * (size_t)u + a];
+  }
+  for (uint32_t e = uc_off[u]; e < uc_off[u + 1]; ++e) {
+    const uint32_t p = uc_ent[e];
+    const uint32_t m = pt_m[p];
+    const double w = pt_w[p];
+    for (int a = 0; a < 3; ++a) {
+      const double cf = imp[3 * (size_t)m + a] * w;
+      s[a] += cf;
+      dd[a] += cf * inv_rho;
+    }
+  }
+  for (int a = 0; a < 3; ++a) {
+    spread[3 * (size_t)u + a] = s[a];
+    du[3 * (size_t)u + a] = dd[a];
+  }
+}
+
+// per marker: represented force, torque contribution, post-sweep slip
+__global__ void k_ibm_marker_diag(int64_t M, const uint32_t* mk_off, const uint32_t* pt_u, const double* pt_w,
+                                  const double* pos, const double* target, const double* unow, const double* du,
+                                  const double* imp, double c0, double c1, double c2, double* mrep, double* mtq,
+                                  double* mslip, uint8_t* mact, const IbmDevDiag* diag) {
+  const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= M || diag->converged) return;
+  double rep[3] = {0.0, 0.0, 0.0}, um[3] = {0.0, 0.0, 0.0};
+  for (uint32_t p = mk_off[m]; p < mk_off[m + 1]; ++p) {
+    const double w = pt_w[p];
+    for (int a = 0; a < 3; ++a) rep[a] += imp[3 * m + a] * w;
+  }
+  for (uint32_t p = mk_off[m]; p < mk_off[m + 1]; ++p) {
+    const uint32_t u = pt_u[p];
+    const double w = pt_w[p];
+    for (int a = 0; a < 3; ++a) um[a] += w * (unow[3 * (size_t)u + a] + du[3 * (size_t)u + a]);
+  }
+  const double r[3] = {pos[3 * m] - c0, pos[3 * m + 1] - c1, pos[3 * m + 2] - c2};
+  mtq[3 * m + 0] = r[1] * rep[2] - r[2] * rep[1];
+  mtq[3 * m + 1] = r[2] * rep[0] - r[0] * rep[2];
+  mtq[3 * m + 2] = r[0] * rep[1] - r[1] * rep[0];
+  for (int a = 0; a < 3; ++a) mrep[3 * m + a] = rep[a];
+  const double s[3] = {target[3 * m] - um[0], target[3 * m + 1] - um[1], target[3 * m + 2] - um[2]};
+  mslip[m] = sqrt(0.0 + s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
+  mact[m] = mk_off[m + 1] > mk_off[m] ? 1 : 0;
+}
+
+// one block: fixed-tree reduction over markers and the convergence test
+__global__ __launch_bounds__(256) void k_ibm_reduce(int64_t M, const double* mrep, const double* mtq,
+                                                    const double* mslip, const uint8_t* mact, const double* weight,
+                                                    int iter, double scale, double tol, IbmDevDiag* diag) {
+  __shared__ double sh[9][256];
+  __shared__ int conv;
+  if (threadIdx.x == 0) conv = diag->converged;
+  __syncthreads();
+  if (conv) return;
+  double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // tq[3], rep[3], slipmax, ssq, wsum
+  for (int64_t m = threadIdx.x; m < M; m += 256) {
+    for (int a = 0; a < 3; ++a) {
+      acc[a] += mtq[3 * m + a];
+      acc[3 + a] += mrep[3 * m + a];
+    }
+    if (mact[m]) {
+      acc[6] = fmax(acc[6], mslip[m]);
+      acc[7] += weight[m] * mslip[m] * mslip[m];
+      acc[8] += weight[m];
+    }
+  }
+  for (int k = 0; k < 9; ++k) sh[k][threadIdx.x] = acc[k];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int k = 0; k < 9; ++k)
+        sh[k][threadIdx.x] = k == 6 ? fmax(sh[k][threadIdx.x], sh[k][threadIdx.x + w])
+                                    : sh[k][threadIdx.x] + sh[k][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  for (int a = 0; a < 3; ++a) {
+    diag->torque[a] -= sh[a][0];
+    diag->marker_force[a] += sh[3 + a][0];
+    diag->fluid_force[a] += sh[3 + a][0];
+  }
+  diag->iterations = iter + 1;
+  diag->slip_max = sh[6][0];
+  diag->slip_rms = sh[8][0] > 0.0 ? sqrt(sh[7][0] / sh[8][0]) : 0.0;
+  diag->slip_max_rel = diag->slip_max / scale;
+  diag->slip_rms_rel = diag->slip_rms / scale;
+  const double* ff = diag->fluid_force;
+  const double* mf = diag->marker_force;
+  const double e[3] = {ff[0] - mf[0], ff[1] - mf[1], ff[2] - mf[2]};
+  const double ne = sqrt(0.0 + e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
+  const double nm = sqrt(0.0 + mf[0] * mf[0] + mf[1] * mf[1] + mf[2] * mf[2]);
+  diag->mom_err = ne / fmax(nm, 1.0e-12);
+  if (diag->slip_max_rel <= tol) diag->converged = 1;
+}
+
+// force_field += T(spread)
+__global__ void k_ibm_commit(uint32_t U, const uint32_t* uc_cell, const double* spread, float* F_cur) {
+  const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= U) return;
+  const size_t i = uc_cell[u];
+  for (int a = 0; a < 3; ++a) F_cur[3 * i + a] = F_cur[3 * i + a] + (float)spread[3 * (size_t)u + a];
+}
+
+// marker_stencil, rotating_ibm.rs:163-224 (host, f64)
+static void marker_stencil(const double p[3], const uint32_t dims[3], int radius, std::vector<int64_t>& cells,
+                           std::vector<double>& w) {
+  std::vector<std::pair<int64_t, double>> axes[3];
+  for (int a = 0; a < 3; ++a) {
+    const int64_t lo = (int64_t)std::floor(p[a]) - radius + 1;
+    const int64_t hi = (int64_t)std::floor(p[a]) + radius;
+    for (int64_t i = lo; i <= hi; ++i) {
+      if (i < 0 || i >= (int64_t)dims[a]) continue;
+      const double r = std::fabs(p[a] - (double)i);
+      double k;
+      if (radius == 1) {
+        k = std::max(1.0 - r, 0.0);
+      } else if (r < 0.5) {
+        k = 0.75 - r * r;
+      } else if (r < 1.5) {
+        k = 0.5 * (1.5 - r) * (1.5 - r);
+      } else {
+        k = 0.0;
+      }
+      if (k != 0.0) axes[a].push_back({i, k});
+    }
+  }
+  cells.clear();
+  w.clear();
+  for (auto& ax : axes[0])
+    for (auto& ay : axes[1])
+      for (auto& az : axes[2]) {
+        cells.push_back((az.first * (int64_t)dims[1] + ay.first) * (int64_t)dims[0] + ax.first);
+        w.push_back(ax.second * ay.second * az.second);
+      }
+}
+
+template <typename T>
+static hipError_t upload(T** dst, const std::vector<T>& src) {
+  const size_t bytes = std::max<size_t>(1, src.size()) * sizeof(T);
+  hipError_t e = hipMalloc((void**)dst, bytes);
+  if (e != hipSuccess) return e;
+  if (!src.empty()) e = hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice);
+  return e;
+}
+
+template <typename T>
+static hipError_t scratch(T** dst, size_t n) {
+  return hipMalloc((void**)dst, std::max<size_t>(1, n) * sizeof(T));
+}
+
+void ibm_destroy(IbmState* s) {
+  if (!s) return;
+  void* ptrs[] = {s->pos,  s->weight, s->target, s->mk_off, s->pt_u,  s->pt_m,  s->pt_w,  s->uc_cell,
+                  s->uc_off, s->uc_ent, s->ksum, s->unow,   s->rho,   s->du,    s->spread, s->imp,
+                  s->mrep, s->mtq,    s->mslip,  s->mact,   s->diag};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  delete s;
+}
+
+IbmState* ibm_create(const FieldsView& v, int64_t n, const double* pos, const double* weight, const double center[3],
+                     const double omega[3], const lbm_ibm_config& cfg, std::vector<uint32_t>* cells,
+                     std::string* err) {
+  const uint32_t dims[3] = {v.d.nx, v.d.ny, v.d.nz};
+  for (int64_t m = 0; m < n; ++m) {
+    if (!(std::isfinite(pos[3 * m]) && std::isfinite(pos[3 * m + 1]) && std::isfinite(pos[3 * m + 2]) &&
+          std::isfinite(weight[m]) && weight[m] > 0.0)) {
+      *err = "IBM marker must have finite position and positive finite weight";
+      return nullptr;
+    }
+  }
+  // solid flags of the domain, for the stencil's solid skip
+  std::vector<uint32_t> mask(v.d.N);
+  if (hipMemcpyAsync(mask.data(), v.mask, (size_t)v.d.N * 4, hipMemcpyDeviceToHost, v.stream) != hipSuccess ||
+      hipStreamSynchronize(v.stream) != hipSuccess) {
+    *err = "mask readback failed";
+    return nullptr;
+  }
+  IbmState* s = new IbmState();
+  s->M = n;
+  s->cfg = cfg;
+  for (int a = 0; a < 3; ++a) {
+    s->center[a] = center[a];
+    s->omega[a] = omega[a];
+  }
+  // targets (RotatingBody::target_velocity, rotating_ibm.rs:89-101) and scale
+  std::vector<double> tgt(3 * n), P(pos, pos + 3 * n), W(weight, weight + n);
+  double max_target = 0.0;
+  for (int64_t m = 0; m < n; ++m) {
+    const double r[3] = {pos[3 * m] - center[0], pos[3 * m + 1] - center[1], pos[3 * m + 2] - center[2]};
+    tgt[3 * m + 0] = omega[1] * r[2] - omega[2] * r[1];
+    tgt[3 * m + 1] = omega[2] * r[0] - omega[0] * r[2];
+    tgt[3 * m + 2] = omega[0] * r[1] - omega[1] * r[0];
+    const double* t = &tgt[3 * m];
+    max_target = std::max(max_target, std::sqrt(0.0 + t[0] * t[0] + t[1] * t[1] + t[2] * t[2]));
+  }
+  s->scale = std::max(max_target, 1.0);
+  // stencils, solid cells dropped
+  std::vector<uint32_t> mk_off(n + 1, 0), pt_cell, pt_m;
+  std::vector<double> pt_w;
+  std::vector<int64_t> sc;
+  std::vector<double> sw;
+  for (int64_t m = 0; m < n; ++m) {
+    marker_stencil(&pos[3 * m], dims, cfg.kernel_radius, sc, sw);
+    if (sc.empty()) {
+      *err = "IBM marker stencil is outside the domain";
+      ibm_destroy(s);
+      return nullptr;
+    }
+    for (size_t k = 0; k < sc.size(); ++k) {
+      if (mask[sc[k]] & kMaskSolid) continue;
+      pt_cell.push_back((uint32_t)sc[k]);
+      pt_m.push_back((uint32_t)m);
+      pt_w.push_back(sw[k]);
+    }
+    mk_off[m + 1] = (uint32_t)pt_cell.size();
+  }
+  const uint32_t Pn = (uint32_t)pt_cell.size();
+  // unique cells, and per cell the points in marker order (stable sort)
+  std::vector<uint32_t> order(Pn);
+  for (uint32_t k = 0; k < Pn; ++k) order[k] = k;
+  std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return pt_cell[a] < pt_cell[b]; });
+  std::vector<uint32_t> uc_cell, uc_off, pt_u(Pn);
+  std::vector<double> ksum;
+  for (uint32_t k = 0; k < Pn; ++k) {
+    const uint32_t p = order[k];
+    if (uc_cell.empty() || uc_cell.back() != pt_cell[p]) {
+      uc_cell.push_back(pt_cell[p]);
+      uc_off.push_back(k);
+      ksum.push_back(0.0);
+    }
+    pt_u[p] = (uint32_t)uc_cell.size() - 1;
+  }
+  uc_off.push_back(Pn);
+  // cell_kernel_sum in marker order (solver.rs:3107-3116)
+  for (uint32_t p = 0; p < Pn; ++p) ksum[pt_u[p]] += pt_w[p];
+  s->P = Pn;
+  s->U = (uint32_t)uc_cell.size();
+  hipError_t e = hipSuccess;
+  auto chain = [&](hipError_t r) {
+    if (e == hipSuccess) e = r;
+  };
+  chain(upload(&s->pos, P));
+  chain(upload(&s->weight, W));
+  chain(upload(&s->target, tgt));
+  chain(upload(&s->mk_off, mk_off));
+  chain(upload(&s->pt_u, pt_u));
+  chain(upload(&s->pt_m, pt_m));
+  chain(upload(&s->pt_w, pt_w));
+  chain(upload(&s->uc_cell, uc_cell));
+  chain(upload(&s->uc_off, uc_off));
+  chain(upload(&s->uc_ent, order));
+  chain(upload(&s->ksum, ksum));
+  chain(scratch(&s->unow, 3 * (size_t)s->U));
+  chain(scratch(&s->rho, s->U));
+  chain(scratch(&s->du, 3 * (size_t)s->U));
+  chain(scratch(&s->spread, 3 * (size_t)s->U));
+  chain(scratch(&s->imp, 3 * (size_t)n));
+  chain(scratch(&s->mrep, 3 * (size_t)n));
+  chain(scratch(&s->mtq, 3 * (size_t)n));
+  chain(scratch(&s->mslip, n));
+  chain(scratch(&s->mact, n));
+  chain(scratch(&s->diag, 1));
+  if (e != hipSuccess) {
+    *err = std::string("IBM device setup failed: ") + hipGetErrorString(e);
+    ibm_destroy(s);
+    return nullptr;
+  }
+  *cells = uc_cell;
+  return s;
+}
+
+template <int Q, typename S>
+static void gather_q(const FieldsView& v, IbmState* s) {
+  hipLaunchKernelGGL((k_ibm_gather<Q, S>), dim3(nblk(s->U)), dim3(256), 0, v.stream, v.d, (const S*)v.f, v.parity,
+                     s->uc_cell, s->U, v.F_cur, v.field_present ? 1 : 0, v.hp, s->unow, s->rho, s->du, s->spread);
+}
+
+int ibm_apply(const FieldsView& v, IbmState* s, lbm_ibm_diag* out) {
+  if (!v.F_cur) {
+    if (v.ctx_err) *v.ctx_err = "force arrays missing";
+    return LBM_ERR_STATE;
+  }
+  hipStream_t st = v.stream;
+  if (s->U) {
+    if (v.Q == 19) {
+      if (v.storage == kF32) gather_q<19, float>(v, s);
+      else if (v.storage == kF16) gather_q<19, _Float16>(v, s);
+      else gather_q<19, bf16s>(v, s);
+    } else {
+      if (v.storage == kF32) gather_q<27, float>(v, s);
+      else if (v.storage == kF16) gather_q<27, _Float16>(v, s);
+      else gather_q<27, bf16s>(v, s);
+    }
+    AUXCHK(v, hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_ibm_reset, dim3(1), dim3(64), 0, st, s->diag);
+  AUXCHK(v, hipGetLastError());
+  const uint32_t gm = nblk((size_t)s->M), gu = nblk(s->U);
+  for (int it = 0; it < s->cfg.max_iterations; ++it) {
+    hipLaunchKernelGGL(k_ibm_impulse, dim3(gm), dim3(256), 0, st, s->M, s->mk_off, s->pt_u, s->pt_w, s->target,
+                       s->unow, s->du, s->rho, s->ksum, s->cfg.relaxation, s->imp, s->diag);
+    if (s->U)
+      hipLaunchKernelGGL(k_ibm_spread, dim3(gu), dim3(256), 0, st, s->U, s->uc_off, s->uc_ent, s->pt_m, s->pt_w,
+                         s->imp, s->rho, s->spread, s->du, s->diag);
+    hipLaunchKernelGGL(k_ibm_marker_diag, dim3(gm), dim3(256), 0, st, s->M, s->mk_off, s->pt_u, s->pt_w, s->pos,
+                       s->target, s->unow, s->du, s->imp, s->center[0], s->center[1], s->center[2], s->mrep, s->mtq,
+                       s->mslip, s->mact, s->diag);
+    hipLaunchKernelGGL(k_ibm_reduce, dim3(1), dim3(256), 0, st, s->M, s->mrep, s->mtq, s->mslip, s->mact, s->weight,
+                       it, s->scale, s->cfg.slip_tolerance, s->diag);
+    AUXCHK(v, hipGetLastError());
+  }
+  if (s->U) {
+    hipLaunchKernelGGL(k_ibm_commit, dim3(gu), dim3(256), 0, st, s->U, s->uc_cell, s->spread, v.F_cur);
+    AUXCHK(v, hipGetLastError());
+  }
+  IbmDevDiag h{};
+  AUXCHK(v, hipMemcpyAsync(&h, s->diag, sizeof h, hipMemcpyDeviceToHost, st));
+  AUXCHK(v, hipStreamSynchronize(st));
+  out->iterations = h.iterations;
+  out->slip_max = h.slip_max;
+  out->slip_rms = h.slip_rms;
+  out->slip_max_rel = h.slip_max_rel;
+  out->slip_rms_rel = h.slip_rms_rel;
+  for (int a = 0; a < 3; ++a) {
+    out->torque[a] = h.torque[a];
+    out->fluid_force[a] = h.fluid_force[a];
+    out->marker_force[a] = h.marker_force[a];
+  }
+  out->momentum_error_rel = h.mom_err;
+  return LBM_OK;
+}
+
+// ===========================================================================
+// passive scalar ADE
+// ===========================================================================
+struct ScalarState {
+  int Qs = 7;
+  double D = 0.0;
+  float omega = 1.0f;
+  float* h = nullptr;
+  float* h2 = nullptr;
+  float* C = nullptr;
+  float* mom = nullptr;  // rho, ux, uy, uz (4 x N)
+  double* partials = nullptr;
+  uint32_t npartials = 1024;
+};
+
+// h_eq of scalar_equilibrium (scalar.rs:6-15) for the hydro lattice; the
+// D3Q7 form uses 1/cs^2 = 4.
+template <int Qs, int q>
+__device__ __forceinline__ float heq_q(float conc, float ux, float uy, float uz) {
+  const float cu = cdot<q>(ux, uy, uz);
+  constexpr float k = Qs == 7 ? 4.0f : 3.0f;
+  return lat_wf(Qs, q) * conc * (1.0f + k * cu);
+}
+
+template <int Qs>
+__global__ void k_scalar_set(Dom d, const float* conc, const float* mom, float* h, float* C) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= d.N) return;
+  const float c = conc[i];
+  const float ux = mom[d.N + i], uy = mom[2 * (size_t)d.N + i], uz = mom[3 * (size_t)d.N + i];
+  C[i] = c;
+  sfor<0, Qs>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    h[(size_t)q * d.N + i] = heq_q<Qs, q>(c, ux, uy, uz);
+  });
+}
+
+template <int Qs>
+__global__ void k_scalar_collide(Dom d, const uint32_t* mask, float* h, const float* C, const float* mom,
+                                 float omega) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= d.N || (mask[i] & kMaskSolid)) return;
+  const float c = C[i];
+  const float ux = mom[d.N + i], uy = mom[2 * (size_t)d.N + i], uz = mom[3 * (size_t)d.N + i];
+  sfor<0, Qs>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    float* p = h + (size_t)q * d.N + i;
+    const float hq = *p;
+    *p = hq - omega * (hq - heq_q<Qs, q>(c, ux, uy, uz));
+  });
+}
+
+template <int Qs>
+__global__ void k_scalar_stream(Dom d, const uint32_t* mask, const float* h, float* h2, float* C) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= d.N) return;
+  const uint32_t m = mask[i];
+  if (m & kMaskSolid) {
+    C[i] = 0.0f;
+    sfor<0, Qs>([&](auto qc) { h2[(size_t)decltype(qc)::value * d.N + i] = 0.0f; });
+    return;
+  }
+  uint32_t x, y, z;
+  cell_xyz(d, i, x, y, z);
+  const Nbr nb(d, x, y, z);
+  float sum = 0.0f;
+  sfor<0, Qs>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    constexpr int o = lat_opp(q);
+    float v;
+    if ((m >> q) & 1u) {  // source i - c_q is solid: bounce back own h_opp
+      v = h[(size_t)o * d.N + i];
+    } else {
+      const uint32_t s = i + (uint32_t)nb.template off<-lat_c(q, 0), -lat_c(q, 1), -lat_c(q, 2)>();
+      v = h[(size_t)q * d.N + s];
+    }
+    h2[(size_t)q * d.N + i] = v;
+    sum = sum + v;
+  });
+  C[i] = sum;
+}
+
+// two-pass CV over fluid cells (qoi::scalar_cv, qoi.rs:507-532)
+__global__ __launch_bounds__(256) void k_cv_pass(Dom d, const uint32_t* mask, const float* C, int pass, double mean,
+                                                 double* partials) {
+  __shared__ double s0[256], s1[256];
+  double a = 0.0, b = 0.0;
+  for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < d.N; i += gridDim.x * 256) {
+    if (mask[i] & kMaskSolid) continue;
+    const double v = (double)C[i];
+    if (pass == 0) {
+      a += 1.0;
+      b += v;
+    } else {
+      const double dv = v - mean;
+      a += dv * dv;
+    }
+  }
+  s0[threadIdx.x] = a;
+  s1[threadIdx.x] = b;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      s0[threadIdx.x] += s0[threadIdx.x + w];
+      s1[threadIdx.x] += s1[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    partials[2 * blockIdx.x] = s0[0];
+    partials[2 * blockIdx.x + 1] = s1[0];
+  }
+}
+
+ScalarState* scalar_create(const FieldsView& v, double diffusivity, int Qs, std::string* err) {
+  ScalarState* s = new ScalarState();
+  s->Qs = Qs;
+  s->D = diffusivity;
+  // omega = T(1/(3D + 1/2)) on the hydro lattice (solver.rs:2131);
+  // D3Q7 maps tau = D/cs^2 + 1/2 = 4D + 1/2.
+  s->omega = Qs == 7 ? (float)(1.0 / (4.0 * diffusivity + 0.5)) : (float)(1.0 / (3.0 * diffusivity + 0.5));
+  const size_t N = v.d.N;
+  hipError_t e = hipMalloc((void**)&s->h, (size_t)Qs * N * 4);
+  if (e == hipSuccess) e = hipMalloc((void**)&s->h2, (size_t)Qs * N * 4);
+  if (e == hipSuccess) e = hipMalloc((void**)&s->C, N * 4);
+  if (e == hipSuccess) e = hipMalloc((void**)&s->mom, 4 * N * 4);
+  if (e == hipSuccess) e = hipMalloc((void**)&s->partials, 2 * s->npartials * sizeof(double));
+  if (e == hipSuccess) e = hipMemsetAsync(s->h, 0, (size_t)Qs * N * 4, v.stream);
+  if (e == hipSuccess) e = hipMemsetAsync(s->h2, 0, (size_t)Qs * N * 4, v.stream);
+  if (e == hipSuccess) e = hipMemsetAsync(s->C, 0, N * 4, v.stream);
+  if (e != hipSuccess) {
+    *err = std::string("scalar allocation failed: ") + hipGetErrorString(e);
+    scalar_destroy(s);
+    return nullptr;
+  }
+  return s;
+}
+
+void scalar_destroy(ScalarState* s) {
+  if (!s) return;
+  for (void* p : {(void*)s->h, (void*)s->h2, (void*)s->C, (void*)s->mom, (void*)s->partials})
+    if (p) (void)hipFree(p);
+  delete s;
+}
+
+static int refresh_mom(const FieldsView& v, ScalarState* s) {
+  const size_t N = v.d.N;
+  AUXCHK(v, launch_moments(v.Q, v.storage, v.parity, v.d, v.f, v.mask, v.F_mom, v.hp, s->mom, s->mom + N,
+                           s->mom + 2 * N, s->mom + 3 * N, v.stream));
+  return LBM_OK;
+}
+
+int scalar_set(const FieldsView& v, ScalarState* s, const float* conc) {
+  const size_t N = v.d.N;
+  float* dc = nullptr;
+  AUXCHK(v, hipMalloc((void**)&dc, N * 4));
+  AUXCHK(v, hipMemcpyAsync(dc, conc, N * 4, hipMemcpyHostToDevice, v.stream));
+  int rc = refresh_mom(v, s);
+  if (rc) return rc;
+  if (s->Qs == 7)
+    hipLaunchKernelGGL((k_scalar_set<7>), dim3(nblk(N)), dim3(256), 0, v.stream, v.d, dc, s->mom, s->h, s->C);
+  else if (s->Qs == 19)
+    hipLaunchKernelGGL((k_scalar_set<19>), dim3(nblk(N)), dim3(256), 0, v.stream, v.d, dc, s->mom, s->h, s->C);
+  else
+    hipLaunchKernelGGL((k_scalar_set<27>), dim3(nblk(N)), dim3(256), 0, v.stream, v.d, dc, s->mom, s->h, s->C);
+  AUXCHK(v, hipGetLastError());
+  AUXCHK(v, hipStreamSynchronize(v.stream));
+  (void)hipFree(dc);
+  return LBM_OK;
+}
+
+int scalar_gather(const FieldsView& v, ScalarState* s, float* conc) {
+  AUXCHK(v, hipMemcpyAsync(conc, s->C, (size_t)v.d.N * 4, hipMemcpyDeviceToHost, v.stream));
+  AUXCHK(v, hipStreamSynchronize(v.stream));
+  return LBM_OK;
+}
+
+template <int Qs>
+static void scalar_step_q(const FieldsView& v, ScalarState* s) {
+  const uint32_t g = nblk(v.d.N);
+  hipLaunchKernelGGL((k_scalar_collide<Qs>), dim3(g), dim3(256), 0, v.stream, v.d, v.mask, s->h, s->C, s->mom,
+                     s->omega);
+  hipLaunchKernelGGL((k_scalar_stream<Qs>), dim3(g), dim3(256), 0, v.stream, v.d, v.mask, s->h, s->h2, s->C);
+}
+
+int scalar_step(const FieldsView& v, ScalarState* s) {
+  int rc = refresh_mom(v, s);
+  if (rc) return rc;
+  if (s->Qs == 7)
+    scalar_step_q<7>(v, s);
+  else if (s->Qs == 19)
+    scalar_step_q<19>(v, s);
+  else
+    scalar_step_q<27>(v, s);
+  AUXCHK(v, hipGetLastError());
+  std::swap(s->h, s->h2);
+  return LBM_OK;
+}
+
+int scalar_cv(const FieldsView& v, ScalarState* s, double* cv, int* defined) {
+  const uint32_t np = std::min<uint32_t>(s->npartials, nblk(v.d.N));
+  std::vector<double> h(2 * np);
+  hipLaunchKernelGGL(k_cv_pass, dim3(np), dim3(256), 0, v.stream, v.d, v.mask, s->C, 0, 0.0, s->partials);
+  AUXCHK(v, hipGetLastError());
+  AUXCHK(v, hipMemcpyAsync(h.data(), s->partials, 2 * np * sizeof(double), hipMemcpyDeviceToHost, v.stream));
+  AUXCHK(v, hipStreamSynchronize(v.stream));
+  double n = 0.0, sum = 0.0;
+  for (uint32_t k = 0; k < np; ++k) {
+    n += h[2 * k];
+    sum += h[2 * k + 1];
+  }
+  *defined = 0;
+  *cv = 0.0;
+  if (n == 0.0) return LBM_OK;
+  const double mean = sum / n;
+  if (mean == 0.0) return LBM_OK;
+  hipLaunchKernelGGL(k_cv_pass, dim3(np), dim3(256), 0, v.stream, v.d, v.mask, s->C, 1, mean, s->partials);
+  AUXCHK(v, hipGetLastError());
+  AUXCHK(v, hipMemcpyAsync(h.data(), s->partials, 2 * np * sizeof(double), hipMemcpyDeviceToHost, v.stream));
+  AUXCHK(v, hipStreamSynchronize(v.stream));
+  double var = 0.0;
+  for (uint32_t k = 0; k < np; ++k) var += h[2 * k];
+  *cv = std::sqrt(var / n) / std::fabs(mean);
+  *defined = 1;
+  return LBM_OK;
+}
+
+}  // namespace lbm

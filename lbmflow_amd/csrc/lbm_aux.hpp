@@ -1,0 +1,48 @@
+// lbm_aux.hpp — host interface of the rotating-IBM and scalar-ADE device
+// paths (lbm_aux.hip), used by the C-ABI layer.  Not part of the public ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/lbm_hip.h"
+#include "lbm_internal.hpp"
+
+namespace lbm {
+
+// The slice of a domain's state the auxiliary paths need.
+struct FieldsView {
+  std::string* ctx_err;
+  hipStream_t stream;
+  int Q, storage;
+  Dom d;
+  void* f;
+  const uint32_t* mask;
+  int parity;
+  HydroParams hp;
+  float* F_cur;
+  float* F_mom;
+  bool field_present;
+};
+
+struct IbmState;
+struct ScalarState;
+
+// Rotating IBM (Solver::apply_rotating_ibm, solver.rs:3016-3282).  `cells`
+// receives the sorted fluid cells the body can force.
+IbmState* ibm_create(const FieldsView& v, int64_t n, const double* pos, const double* weight, const double center[3],
+                     const double omega[3], const lbm_ibm_config& cfg, std::vector<uint32_t>* cells,
+                     std::string* err);
+void ibm_destroy(IbmState* s);
+int ibm_apply(const FieldsView& v, IbmState* s, lbm_ibm_diag* diag);
+
+// Passive scalar ADE (Solver::step_scalar_distributions, solver.rs:2113-2205).
+ScalarState* scalar_create(const FieldsView& v, double diffusivity, int Qs, std::string* err);
+void scalar_destroy(ScalarState* s);
+int scalar_set(const FieldsView& v, ScalarState* s, const float* conc);
+int scalar_gather(const FieldsView& v, ScalarState* s, float* conc);
+int scalar_step(const FieldsView& v, ScalarState* s);
+int scalar_cv(const FieldsView& v, ScalarState* s, double* cv, int* defined);
+
+}  // namespace lbm

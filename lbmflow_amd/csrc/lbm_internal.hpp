@@ -1,0 +1,83 @@
+// lbm_internal.hpp — structures shared by the C-ABI layer (lbm_api.cpp) and
+// the HIP kernels (lbm_kernels.hip).  Not part of the public ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace lbm {
+
+// Per-cell mask word.  Bits 0..26: half-way bounce-back links in the pull
+// convention of stream_row (kernels.rs:572-603): bit q is set on a fluid
+// cell i when i - c_q is solid.  High bits are cell flags.
+constexpr uint32_t kMaskSolid = 1u << 31;
+constexpr uint32_t kMaskForced = 1u << 30;  // handled by the force-list kernel
+constexpr uint32_t kMaskMoving = 1u << 29;  // some linked solid has a nonzero wall velocity
+constexpr uint32_t kMaskLinks = (1u << 27) - 1u;
+
+enum Storage : int { kF32 = 0, kF16 = 1, kBF16 = 2 };
+
+struct Dom {
+  uint32_t nx, ny, nz;
+  uint32_t N;        // nx*ny*nz (< 2^31)
+  uint32_t per[3];   // periodic axes
+};
+
+// Launch geometry of the cell-parallel kernels: blocks of TX x TY threads
+// tile the (x, y) plane; block ids are remapped so each XCD walks a
+// contiguous slab of the domain.
+struct Tiling {
+  uint32_t tx, ty;
+  uint32_t bx, by;   // blocks along x and y
+  uint32_t nblocks;  // bx * by * nz
+};
+
+struct HydroParams {
+  float omega_p, omega_m, cp, cm, omega_shear;
+  float F[3];        // uniform Guo force (params.force)
+  int force_on;      // F != 0
+};
+
+struct HydroArgs {
+  Dom d;
+  Tiling t;
+  void* f;                  // Q planes of N storage elements
+  const uint32_t* mask;     // N
+  const float* wall_u;      // N x 3 or null
+  const float* omega;       // N or null (per-cell omega_plus)
+  HydroParams p;
+  // force-list kernel only
+  const uint32_t* flist;    // listed cells
+  uint32_t nlist;
+  float* F_mom;             // N x 3: force of the cached moments
+  const float* F_cur;       // N x 3: current field
+};
+
+// Host-side launchers (lbm_kernels.hip).  All enqueue on `s`.
+hipError_t launch_hydro(int Q, int storage, int collision, int odd, const HydroArgs& a, hipStream_t s);
+hipError_t launch_hydro_list(int Q, int storage, int collision, int odd, const HydroArgs& a, hipStream_t s);
+hipError_t launch_build_mask(int Q, const Dom& d, const uint8_t* solid, const float* wall_u, uint32_t* mask,
+                             unsigned int* err_count, hipStream_t s);
+hipError_t launch_mark_forced(const uint32_t* flist, uint32_t n, uint32_t* mask, hipStream_t s);
+hipError_t launch_stage(int Q, int storage, int dir, int odd, const Dom& d, void* f, float* host_layout,
+                        const uint32_t* mask, hipStream_t s);
+hipError_t launch_init_with(int Q, int storage, const Dom& d, void* f, const uint32_t* mask, const float* rho,
+                            const float* u, float tau, hipStream_t s);
+hipError_t launch_moments(int Q, int storage, int odd, const Dom& d, const void* f, const uint32_t* mask,
+                          const float* F_mom, HydroParams p, float* rho, float* ux, float* uy, float* uz,
+                          hipStream_t s);
+hipError_t launch_reduce(int Q, int storage, int odd, const Dom& d, const void* f, const uint32_t* mask,
+                         const float* F_mom, HydroParams p, int kind, int axis, double* partials,
+                         uint32_t npartials, hipStream_t s);
+hipError_t launch_tank(const Dom& d, double dx, double diameter, double height, int baffles, double bw,
+                       double bt, uint8_t* solid, hipStream_t s);
+hipError_t launch_rims(const Dom& d, const int is_wall[6], const double u[18], uint8_t* solid, float* wall_u,
+                       double* best, hipStream_t s);
+hipError_t launch_fill_f32(float* p, size_t n, float v, hipStream_t s);
+hipError_t launch_copy_force_list(const uint32_t* flist, uint32_t n, float* F_dst, const float* F_src,
+                                  hipStream_t s);
+
+Tiling make_tiling(const Dom& d, uint32_t tx_pref);
+
+}  // namespace lbm

@@ -1,0 +1,970 @@
+// lbm_kernels.hip — CDNA4 (gfx950) kernels for lbm-core's lattice-Boltzmann
+// step: fused in-place (AA-pattern) collide-stream over SoA populations with
+// half-way bounce-back, moving walls and Guo forcing; TRT/BGK and the
+// factorised central-moment collision; masks, device geometry, staging,
+// moments and f64 reductions.
+//
+// Storage: q-major planes f[q*N + cell] of the deviation f - w_q in fp32,
+// or scaled by 1/w_q in fp16 / bf16.  Arithmetic is fp32.  Reductions are
+// fp64.  The whole file is compiled with -ffp-contract=off so the TRT path
+// keeps the reference's separately rounded multiply/add sequence
+// (collision.rs:123-202) and reproduces the CpuScalar f32 trajectory
+// bit for bit; the central-moment path opts into contraction locally.
+//
+// AA pattern (in-place streaming, one buffer):
+//   even step  cell i reads f_q from slot (i, q), writes the post-collision
+//              value moving along c_q to slot (i, opp q);
+//   odd step   cell i reads f_q from slot (i - c_q, opp q), writes the value
+//              moving along c_q to slot (i + c_q, q).
+// A post-collision population heading into a solid neighbour s = i + c_q is
+// reflected immediately: on even steps it is written to the solid's slot
+// (s, q) -- where the odd step's uniform read of i looks for the incoming
+// opp(q) population -- and on odd steps to (i, opp q), where the next even
+// step reads it.  Each slot is read and written by exactly one cell per
+// step, so cells are independent and no second buffer is needed.
+#include <type_traits>
+
+#include "lbm_device.hpp"
+
+namespace lbm {
+
+// ---------------------------------------------------------------------------
+// TRT/BGK with Guo forcing: TrtGuo::relax (collision.rs:123-202) with the
+// equilibrium of SecondOrderEq::feq (81-102) and GuoForcing::source
+// (104-121).  Same operand order, no contraction.
+// ---------------------------------------------------------------------------
+template <int Q>
+__device__ __forceinline__ void collide_trt(float* fv, float r, float ux, float uy, float uz, const float Fc[3],
+                                            bool fon, float op, float cp, const HydroParams& p) {
+  const float half = 0.5f;
+  float usq = ux * ux;
+  usq = usq + uy * uy;
+  usq = usq + uz * uz;
+  float uf = 0.0f;
+  if (fon) {
+    uf = ux * Fc[0];
+    uf = uf + uy * Fc[1];
+    uf = uf + uz * Fc[2];
+  }
+  const float drho = r - 1.0f;
+  const float f15_usq = 1.5f * usq;
+  auto feq_of = [&](float cu, float w) {
+    float three_cu = 3.0f * cu;
+    float f45_cu = 4.5f * cu;
+    float f45_cu2 = f45_cu * cu;
+    float sum = three_cu + f45_cu2;
+    float inner = sum - f15_usq;
+    float rho_inner = r * inner;
+    float body = drho + rho_inner;
+    return w * body;
+  };
+  auto src_of = [&](float cu, float cf, float w) {
+    float cf_minus_uf = cf - uf;
+    float first = 3.0f * cf_minus_uf;
+    float nine_cu = 9.0f * cu;
+    float second = nine_cu * cf;
+    float body = first + second;
+    return w * body;
+  };
+  // rest population
+  {
+    const float w0 = lat_wf(Q, 0);
+    float feq0 = feq_of(0.0f * ux, w0);  // cu of the rest velocity is a signed zero
+    float f0 = fv[0];
+    float relaxed = f0 - op * (f0 - feq0);
+    if (fon) {
+      float cf0 = 0.0f;
+      float src0 = src_of(0.0f, cf0, w0);
+      relaxed = relaxed + cp * src0;
+    }
+    fv[0] = relaxed;
+  }
+  const float om = p.omega_m, cm = p.cm;
+  // pairs (q, q+1), q odd, ascending (lattice.rs PAIRS)
+  sfor<0, (Q - 1) / 2>([&](auto kc) {
+    constexpr int qa = 2 * decltype(kc)::value + 1;
+    constexpr int qb = qa + 1;
+    constexpr float w = lat_wf(Q, qa);
+    const float cua = cdot<qa>(ux, uy, uz);
+    const float cub = cdot<qb>(ux, uy, uz);
+    const float ea = feq_of(cua, w);
+    const float eb = feq_of(cub, w);
+    const float fa = fv[qa], fb = fv[qb];
+    const float fp = half * (fa + fb);
+    const float fm = half * (fa - fb);
+    const float ep = half * (ea + eb);
+    const float em = half * (ea - eb);
+    const float rp = op * (fp - ep);
+    const float rm = om * (fm - em);
+    float va = (fa - rp) - rm;
+    float vb = (fb - rp) + rm;
+    if (fon) {
+      const float sa = src_of(cua, cdot<qa>(Fc[0], Fc[1], Fc[2]), w);
+      const float sb = src_of(cub, cdot<qb>(Fc[0], Fc[1], Fc[2]), w);
+      const float sp = half * (sa + sb);
+      const float sm = half * (sa - sb);
+      const float cp_sp = cp * sp;
+      const float cm_sm = cm * sm;
+      va = va + (cp_sp + cm_sm);
+      vb = vb + (cp_sp - cm_sm);
+    }
+    fv[qa] = va;
+    fv[qb] = vb;
+  });
+}
+
+// ---------------------------------------------------------------------------
+// Central-moment collision (collide_row_central_moment, kernels.rs:406-533)
+// in non-equilibrium form.  With feq the deviation equilibrium and
+// K = central moments of (f - feq), S = central moments of the Guo source,
+// the reference's post-collision populations are feq + B(d), where B is the
+// inverse central-moment transform and
+//   order 0,1:       d = K + S
+//   order 2, a!=b:   d = (1-os) K + (1-os/2) S
+//   order 2, a==b:   d = (1-os)(K_aa - K_bulk) + S_bulk/2 + (1-os/2)(S_aa - S_bulk)
+//   order >= 3:      d = S/2
+// (the reference relaxes the trace and all orders >= 3 at rate 1).  The
+// D3Q27 transforms factorise into three passes of 1-D three-point lines
+// (tensor-product basis, exponents 0..2 per axis), so no 27x27 matrix
+// is ever formed.  Working on f - feq keeps the fp32 arithmetic on small
+// numbers; the rounding difference to the reference's f64 internals stays
+// well below the 1e-6 population gate.
+// ---------------------------------------------------------------------------
+constexpr int gidx(int cx, int cy, int cz) { return (cx + 1) * 9 + (cy + 1) * 3 + (cz + 1); }
+
+// forward 1-D central line: (g-, g0, g+) -> (k0, k1, k2)
+__device__ __forceinline__ void fwd_line(float& a0, float& a1, float& a2, float u) {
+#pragma clang fp contract(fast)
+  const float gm = a0, g0 = a1, gp = a2;
+  const float s = gp + gm, t = gp - gm;
+  const float k0 = s + g0;
+  const float k1 = t - u * k0;
+  const float k2 = (s - 2.0f * u * t) + u * u * k0;
+  a0 = k0;
+  a1 = k1;
+  a2 = k2;
+}
+// inverse 1-D central line: (k0, k1, k2) -> (g-, g0, g+)
+__device__ __forceinline__ void inv_line(float& a0, float& a1, float& a2, float u) {
+#pragma clang fp contract(fast)
+  const float k0 = a0, k1 = a1, k2 = a2;
+  const float m1 = k1 + u * k0;
+  const float m2 = (k2 + 2.0f * u * k1) + u * u * k0;
+  a0 = 0.5f * (m2 - m1);
+  a1 = k0 - m2;
+  a2 = 0.5f * (m1 + m2);
+}
+
+// Full 27-point transforms on G[(a*9 + b*3 + c)], a/b/c = x/y/z index 0..2.
+__device__ __forceinline__ void fwd27(float* G, float ux, float uy, float uz) {
+#pragma unroll
+  for (int ab = 0; ab < 9; ++ab) fwd_line(G[ab * 3 + 0], G[ab * 3 + 1], G[ab * 3 + 2], uz);
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) fwd_line(G[a * 9 + 0 + c], G[a * 9 + 3 + c], G[a * 9 + 6 + c], uy);
+#pragma unroll
+  for (int bc = 0; bc < 9; ++bc) fwd_line(G[bc], G[9 + bc], G[18 + bc], ux);
+}
+__device__ __forceinline__ void inv27(float* G, float ux, float uy, float uz) {
+#pragma unroll
+  for (int bc = 0; bc < 9; ++bc) inv_line(G[bc], G[9 + bc], G[18 + bc], ux);
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) inv_line(G[a * 9 + 0 + c], G[a * 9 + 3 + c], G[a * 9 + 6 + c], uy);
+#pragma unroll
+  for (int ab = 0; ab < 9; ++ab) inv_line(G[ab * 3 + 0], G[ab * 3 + 1], G[ab * 3 + 2], uz);
+}
+
+// deviation equilibrium of all 27 directions (same formula as the TRT
+// path, here with contraction allowed)
+__device__ __forceinline__ void feq27(float* feq, float r, float ux, float uy, float uz) {
+#pragma clang fp contract(fast)
+  const float usq = ux * ux + uy * uy + uz * uz;
+  const float drho = r - 1.0f;
+  const float b = drho - r * 1.5f * usq;
+  sfor<0, 27>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    const float cu = cdot<q>(ux, uy, uz);
+    feq[q] = lat_wf(27, q) * (b + r * cu * (3.0f + 4.5f * cu));
+  });
+}
+
+__device__ __forceinline__ float cm_rate(float os_base, float ux, float uy, float uz) {
+#pragma clang fp contract(fast)
+  const float usq = ux * ux + uy * uy + uz * uz;
+  return fminf(os_base * (1.0f - 0.16f * usq), 2.0f);
+}
+
+// General form (any force): full transforms of f - feq and of the source.
+__device__ __forceinline__ void collide_cm27_general(float* fv, float r, float ux, float uy, float uz,
+                                                     const float F[3], bool fon, float os_base) {
+#pragma clang fp contract(fast)
+  float feq[27];
+  feq27(feq, r, ux, uy, uz);
+  float K[27], S[27];
+  sfor<0, 27>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    constexpr int g = gidx(lat_c(q, 0), lat_c(q, 1), lat_c(q, 2));
+    K[g] = fv[q] - feq[q];
+  });
+  fwd27(K, ux, uy, uz);
+  if (fon) {
+    const float uf = ux * F[0] + uy * F[1] + uz * F[2];
+    sfor<0, 27>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      constexpr int g = gidx(lat_c(q, 0), lat_c(q, 1), lat_c(q, 2));
+      const float cu = cdot<q>(ux, uy, uz);
+      const float cf = cdot<q>(F[0], F[1], F[2]);
+      S[g] = lat_wf(27, q) * (3.0f * (cf - uf) + 9.0f * cu * cf);
+    });
+    fwd27(S, ux, uy, uz);
+  } else {
+#pragma unroll
+    for (int g = 0; g < 27; ++g) S[g] = 0.0f;
+  }
+  const float os = cm_rate(os_base, ux, uy, uz);
+  const float a1 = 1.0f - os, a2 = 1.0f - 0.5f * os;
+  const int ixx = 2 * 9, iyy = 2 * 3, izz = 2;
+  const float kb = (K[ixx] + K[iyy] + K[izz]) * (1.0f / 3.0f);
+  const float sb = (S[ixx] + S[iyy] + S[izz]) * (1.0f / 3.0f);
+  float D[27];
+#pragma unroll
+  for (int g = 0; g < 27; ++g) {
+    const int ex = g / 9, ey = (g / 3) % 3, ez = g % 3;
+    const int order = ex + ey + ez;
+    if (order <= 1)
+      D[g] = K[g] + S[g];
+    else if (order == 2 && (ex == 2 || ey == 2 || ez == 2))
+      D[g] = a1 * (K[g] - kb) + 0.5f * sb + a2 * (S[g] - sb);
+    else if (order == 2)
+      D[g] = a1 * K[g] + a2 * S[g];
+    else
+      D[g] = 0.5f * S[g];
+  }
+  inv27(D, ux, uy, uz);
+  sfor<0, 27>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    constexpr int g = gidx(lat_c(q, 0), lat_c(q, 1), lat_c(q, 2));
+    fv[q] = feq[q] + D[g];
+  });
+}
+
+// Force-free form: only moments of total order <= 2 are needed on the way
+// in, and only they are nonzero on the way out; the passes skip the rest.
+__device__ __forceinline__ void collide_cm27_free(float* fv, float r, float ux, float uy, float uz,
+                                                  float os_base) {
+#pragma clang fp contract(fast)
+  float feq[27];
+  feq27(feq, r, ux, uy, uz);
+  float G[27];
+  sfor<0, 27>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    constexpr int g = gidx(lat_c(q, 0), lat_c(q, 1), lat_c(q, 2));
+    G[g] = fv[q] - feq[q];
+  });
+  // raw moments of order <= 2 by partial sums: z, then y, then x
+  float Z0[9], Z1[9], Z2[9];
+#pragma unroll
+  for (int ab = 0; ab < 9; ++ab) {
+    const float gm = G[ab * 3 + 0], g0 = G[ab * 3 + 1], gp = G[ab * 3 + 2];
+    Z2[ab] = gp + gm;
+    Z1[ab] = gp - gm;
+    Z0[ab] = Z2[ab] + g0;
+  }
+  float Y00[3], Y10[3], Y20[3], Y01[3], Y11[3], Y02[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float* z0 = Z0 + 3 * a;
+    const float* z1 = Z1 + 3 * a;
+    const float* z2 = Z2 + 3 * a;
+    Y20[a] = z0[2] + z0[0];
+    Y10[a] = z0[2] - z0[0];
+    Y00[a] = Y20[a] + z0[1];
+    Y11[a] = z1[2] - z1[0];
+    Y01[a] = (z1[2] + z1[0]) + z1[1];
+    Y02[a] = (z2[2] + z2[0]) + z2[1];
+  }
+  const float R200 = Y00[2] + Y00[0], R100 = Y00[2] - Y00[0], R000 = R200 + Y00[1];
+  const float R110 = Y10[2] - Y10[0], R010 = (Y10[2] + Y10[0]) + Y10[1];
+  const float R020 = (Y20[2] + Y20[0]) + Y20[1];
+  const float R101 = Y01[2] - Y01[0], R001 = (Y01[2] + Y01[0]) + Y01[1];
+  const float R011 = (Y11[2] + Y11[0]) + Y11[1];
+  const float R002 = (Y02[2] + Y02[0]) + Y02[1];
+  // central moments
+  const float K000 = R000;
+  const float K100 = R100 - ux * R000, K010 = R010 - uy * R000, K001 = R001 - uz * R000;
+  const float K200 = (R200 - 2.0f * ux * R100) + ux * ux * R000;
+  const float K020 = (R020 - 2.0f * uy * R010) + uy * uy * R000;
+  const float K002 = (R002 - 2.0f * uz * R001) + uz * uz * R000;
+  const float K110 = ((R110 - ux * R010) - uy * R100) + ux * uy * R000;
+  const float K101 = ((R101 - ux * R001) - uz * R100) + ux * uz * R000;
+  const float K011 = ((R011 - uy * R001) - uz * R010) + uy * uz * R000;
+  // relaxation
+  const float os = cm_rate(os_base, ux, uy, uz);
+  const float a1 = 1.0f - os;
+  const float kb = (K200 + K020 + K002) * (1.0f / 3.0f);
+  const float d200 = a1 * (K200 - kb), d020 = a1 * (K020 - kb), d002 = a1 * (K002 - kb);
+  const float d110 = a1 * K110, d101 = a1 * K101, d011 = a1 * K011;
+  // inverse, x lines (ey, ez); X[cx][ey][ez] stored as X[(cx)*9 + ey*3 + ez]
+  float X[27];
+#pragma unroll
+  for (int g = 0; g < 27; ++g) X[g] = 0.0f;
+  const float ux2 = ux * ux, uy2 = uy * uy, uz2 = uz * uz;
+  auto inv3 = [](float k0, float k1, float k2, float u, float u2, float& gm, float& g0, float& gp) {
+    const float m1 = k1 + u * k0;
+    const float m2 = (k2 + 2.0f * u * k1) + u2 * k0;
+    gm = 0.5f * (m2 - m1);
+    g0 = k0 - m2;
+    gp = 0.5f * (m1 + m2);
+  };
+  auto inv2 = [](float k0, float k1, float u, float u2, float& gm, float& g0, float& gp) {
+    const float m1 = k1 + u * k0;
+    const float m2 = 2.0f * u * k1 + u2 * k0;
+    gm = 0.5f * (m2 - m1);
+    g0 = k0 - m2;
+    gp = 0.5f * (m1 + m2);
+  };
+  auto inv1 = [](float k0, float u, float u2, float& gm, float& g0, float& gp) {
+    const float m1 = u * k0;
+    const float m2 = u2 * k0;
+    gm = 0.5f * (m2 - m1);
+    g0 = k0 - m2;
+    gp = 0.5f * (m1 + m2);
+  };
+  inv3(K000, K100, d200, ux, ux2, X[0 * 9 + 0], X[1 * 9 + 0], X[2 * 9 + 0]);       // (ey,ez)=(0,0)
+  inv2(K010, d110, ux, ux2, X[0 * 9 + 3], X[1 * 9 + 3], X[2 * 9 + 3]);             // (1,0)
+  inv2(K001, d101, ux, ux2, X[0 * 9 + 1], X[1 * 9 + 1], X[2 * 9 + 1]);             // (0,1)
+  inv1(d020, ux, ux2, X[0 * 9 + 6], X[1 * 9 + 6], X[2 * 9 + 6]);                   // (2,0)
+  inv1(d002, ux, ux2, X[0 * 9 + 2], X[1 * 9 + 2], X[2 * 9 + 2]);                   // (0,2)
+  inv1(d011, ux, ux2, X[0 * 9 + 4], X[1 * 9 + 4], X[2 * 9 + 4]);                   // (1,1)
+  // y lines (cx, ez)
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    float* L = X + 9 * a;
+    float g_m, g_0, g_p;
+    inv3(L[0], L[3], L[6], uy, uy2, g_m, g_0, g_p);  // ez = 0
+    L[0] = g_m;
+    L[3] = g_0;
+    L[6] = g_p;
+    inv2(L[1], L[4], uy, uy2, g_m, g_0, g_p);  // ez = 1
+    L[1] = g_m;
+    L[4] = g_0;
+    L[7] = g_p;
+    inv1(L[2], uy, uy2, g_m, g_0, g_p);  // ez = 2
+    L[2] = g_m;
+    L[5] = g_0;
+    L[8] = g_p;
+  }
+  // z lines (cx, cy)
+#pragma unroll
+  for (int ab = 0; ab < 9; ++ab) {
+    float g_m, g_0, g_p;
+    inv3(X[ab * 3 + 0], X[ab * 3 + 1], X[ab * 3 + 2], uz, uz2, g_m, g_0, g_p);
+    X[ab * 3 + 0] = g_m;
+    X[ab * 3 + 1] = g_0;
+    X[ab * 3 + 2] = g_p;
+  }
+  sfor<0, 27>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    constexpr int g = gidx(lat_c(q, 0), lat_c(q, 1), lat_c(q, 2));
+    fv[q] = feq[q] + X[g];
+  });
+}
+
+// ---------------------------------------------------------------------------
+// one cell: load (parity), moments, collide, bounce-back-aware store
+// ---------------------------------------------------------------------------
+template <int Q, typename S, int KIND, bool ODD, bool FON>
+__device__ __forceinline__ void cell_step(const HydroArgs& a, uint32_t i, uint32_t x, uint32_t y, uint32_t z,
+                                          uint32_t m, const float Fm[3], const float Fc[3]) {
+  constexpr bool fon = FON;
+  S* base = (S*)a.f;
+  const size_t N = a.d.N;
+  const Nbr nb(a.d, x, y, z);
+  float fv[Q];
+  sfor<0, Q>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    constexpr int o = lat_opp(q);
+    if constexpr (!ODD) {
+      fv[q] = pop_ld<S, Q, q>(base + (size_t)q * N, i);
+    } else {
+      const uint32_t s = i + (uint32_t)nb.template off<-lat_c(q, 0), -lat_c(q, 1), -lat_c(q, 2)>();
+      fv[q] = pop_ld<S, Q, q>(base + (size_t)o * N, s);
+    }
+  });
+  float r, ux, uy, uz;
+  moments<Q>(fv, Fm, fon, r, ux, uy, uz);
+  float op = a.p.omega_p, cp = a.p.cp, os = a.p.omega_shear;
+  if (a.omega) {  // per-cell omega_plus (kernels.rs:151-156)
+    op = a.omega[i];
+    cp = 1.0f - op / 2.0f;
+    os = op;
+  }
+  if constexpr (KIND == 0) {
+    collide_trt<Q>(fv, r, ux, uy, uz, Fc, fon, op, cp, a.p);
+  } else {
+    static_assert(Q == 27, "central-moment kernels are D3Q27");
+    if constexpr (FON)
+      collide_cm27_general(fv, r, ux, uy, uz, Fc, true, os);
+    else
+      collide_cm27_free(fv, r, ux, uy, uz, os);
+  }
+  const bool moving = (m & kMaskMoving) != 0;
+  sfor<0, Q>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    constexpr int o = lat_opp(q);
+    constexpr int cx = lat_c(q, 0), cy = lat_c(q, 1), cz = lat_c(q, 2);
+    float v = fv[q];
+    if constexpr (q == 0) {
+      pop_st<S, Q, q>(base, i, v);
+    } else {
+      const uint32_t fwd = i + (uint32_t)nb.template off<cx, cy, cz>();
+      if ((m >> o) & 1u) {
+        // half-way bounce-back off the solid at i + c_q (stream_row,
+        // kernels.rs:591-603): the reflected population enters i along
+        // c_o with fin = fout + 6 w_o rho (c_o . u_wall)
+        if (moving) {
+          const float* wu = a.wall_u + 3 * (size_t)fwd;
+          const float cu = cdot<o>(wu[0], wu[1], wu[2]);
+          constexpr float six_w = 6.0f * lat_wf(Q, o);
+          v = v + six_w * r * cu;
+        }
+        if constexpr (!ODD)
+          pop_st<S, Q, q>(base + (size_t)q * N, fwd, v);
+        else
+          pop_st<S, Q, q>(base + (size_t)o * N, i, v);
+      } else {
+        if constexpr (!ODD)
+          pop_st<S, Q, q>(base + (size_t)o * N, i, v);
+        else
+          pop_st<S, Q, q>(base + (size_t)q * N, fwd, v);
+      }
+    }
+  });
+}
+
+template <int Q, typename S, int KIND, bool ODD, bool FON>
+__global__ __launch_bounds__(256) void k_hydro(HydroArgs a) {
+  const uint32_t b = xcd_remap(blockIdx.x, a.t.nblocks);
+  const uint32_t bxi = b % a.t.bx, t = b / a.t.bx;
+  const uint32_t byi = t % a.t.by, z = t / a.t.by;
+  const uint32_t x = bxi * a.t.tx + threadIdx.x, y = byi * a.t.ty + threadIdx.y;
+  if (x >= a.d.nx || y >= a.d.ny) return;
+  const uint32_t i = (z * a.d.ny + y) * a.d.nx + x;
+  const uint32_t m = a.mask[i];
+  if (m & (kMaskSolid | kMaskForced)) return;
+  cell_step<Q, S, KIND, ODD, FON>(a, i, x, y, z, m, a.p.F, a.p.F);
+}
+
+// Cells carrying a per-cell force: the velocity uses the force of the cached
+// moments (F_mom, set when moments were last refreshed), the Guo source the
+// current field (F_cur), as the reference's stale-moment step does
+// (solver.rs:2805-2822 keeps moments across a non-uniform clear).  After the
+// step the cached moments carry F_cur.
+template <int Q, typename S, int KIND, bool ODD>
+__global__ __launch_bounds__(256) void k_hydro_list(HydroArgs a) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= a.nlist) return;
+  const uint32_t i = a.flist[k];
+  const uint32_t m = a.mask[i];
+  if (m & kMaskSolid) return;
+  const uint32_t x = i % a.d.nx, t = i / a.d.nx;
+  const uint32_t y = t % a.d.ny, z = t / a.d.ny;
+  float Fm[3], Fc[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    Fm[c] = a.p.F[c] + a.F_mom[3 * (size_t)i + c];
+    Fc[c] = a.p.F[c] + a.F_cur[3 * (size_t)i + c];
+  }
+  cell_step<Q, S, KIND, ODD, true>(a, i, x, y, z, m, Fm, Fc);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) a.F_mom[3 * (size_t)i + c] = a.F_cur[3 * (size_t)i + c];
+}
+
+// moments readback (moments_row with the cached-moment force F_u + F_mom)
+template <int Q, typename S>
+__global__ void k_moments(Dom d, const S* f, const uint32_t* mask, const float* F_mom, HydroParams p, int odd,
+                          float* rho, float* ux, float* uy, float* uz) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= d.N) return;
+  if (mask[i] & kMaskSolid) {
+    rho[i] = 1.0f;
+    ux[i] = uy[i] = uz[i] = 0.0f;
+    return;
+  }
+  uint32_t x, y, z;
+  cell_xyz(d, i, x, y, z);
+  const Nbr nb(d, x, y, z);
+  float fv[Q];
+  sfor<0, Q>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    fv[q] = read_pop<Q, S, q>(f, d.N, nb, i, odd != 0);
+  });
+  float Fm[3] = {p.F[0], p.F[1], p.F[2]};
+  bool fon = p.force_on != 0;
+  if (F_mom) {
+    for (int c = 0; c < 3; ++c) Fm[c] = p.F[c] + F_mom[3 * (size_t)i + c];
+    fon = true;
+  }
+  float r, a, b, c;
+  moments<Q>(fv, Fm, fon, r, a, b, c);
+  rho[i] = r;
+  ux[i] = a;
+  uy[i] = b;
+  uz[i] = c;
+}
+
+// stage in (dir 0: host layout -> device, parity reset to even) /
+// stage out (dir 1: device -> host layout, de-permuted by parity).
+template <int Q, typename S>
+__global__ void k_stage(Dom d, S* f, float* h, const uint32_t* mask, int dir, int odd) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= d.N) return;
+  if (dir == 0) {
+    sfor<0, Q>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      pop_st<S, Q, q>(f + (size_t)q * d.N, i, h[(size_t)q * d.N + i]);
+    });
+    return;
+  }
+  const bool solid = (mask[i] & kMaskSolid) != 0;
+  uint32_t x, y, z;
+  cell_xyz(d, i, x, y, z);
+  const Nbr nb(d, x, y, z);
+  sfor<0, Q>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    h[(size_t)q * d.N + i] = solid ? 0.0f : read_pop<Q, S, q>(f, d.N, nb, i, odd != 0);
+  });
+}
+
+// Solver::init_with, solver.rs:2474-2623 (pass 2): feq of every cell plus
+// the gradient non-equilibrium part on fluid cells.
+template <int Q, typename S>
+__global__ void k_init_with(Dom d, S* f, const uint32_t* mask, const float* rho, const float* u, float tau) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= d.N) return;
+  uint32_t x, y, z;
+  cell_xyz(d, i, x, y, z);
+  const float r = rho[i];
+  const float own[3] = {u[3 * (size_t)i], u[3 * (size_t)i + 1], u[3 * (size_t)i + 2]};
+  // equilibrium(), kernels.rs:103-121
+  const float three = 3.0f, f45 = 4.5f, f15 = 1.5f, half = 0.5f;
+  float usq = own[0] * own[0];
+  usq = usq + own[1] * own[1];
+  usq = usq + own[2] * own[2];
+  const float drho = r - 1.0f;
+  float fv[Q];
+  sfor<0, Q>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    const float cu = cdot<q>(own[0], own[1], own[2]);
+    fv[q] = lat_wf(Q, q) * (drho + r * (three * cu + f45 * cu * cu - f15 * usq));
+  });
+  const bool solid = (mask[i] & kMaskSolid) != 0;
+  if (!solid) {
+    const uint32_t n3[3] = {d.nx, d.ny, d.nz};
+    const uint32_t c3[3] = {x, y, z};
+    float grad[3][3];
+    for (int ax = 0; ax < 3; ++ax) {
+      float pv[3], mv[3];
+      bool hp = false, hm = false;
+      for (int sgn = -1; sgn <= 1; sgn += 2) {
+        int64_t g[3] = {(int64_t)x, (int64_t)y, (int64_t)z};
+        g[ax] += sgn;
+        bool ok = true;
+        if (g[ax] < 0 || g[ax] >= (int64_t)n3[ax]) {
+          if (d.per[ax])
+            g[ax] = (g[ax] + n3[ax]) % n3[ax];
+          else
+            ok = false;
+        }
+        if (ok) {
+          const uint32_t j = (uint32_t)((g[2] * d.ny + g[1]) * d.nx + g[0]);
+          if (mask[j] & kMaskSolid) ok = false;
+          if (ok) {
+            float* dst = sgn > 0 ? pv : mv;
+            dst[0] = u[3 * (size_t)j];
+            dst[1] = u[3 * (size_t)j + 1];
+            dst[2] = u[3 * (size_t)j + 2];
+          }
+        }
+        (sgn > 0 ? hp : hm) = ok;
+      }
+      (void)c3;
+      for (int b = 0; b < 3; ++b) {
+        if (hp && hm)
+          grad[ax][b] = (pv[b] - mv[b]) * half;
+        else if (hp)
+          grad[ax][b] = pv[b] - own[b];
+        else if (hm)
+          grad[ax][b] = own[b] - mv[b];
+        else
+          grad[ax][b] = 0.0f;
+      }
+    }
+    float div = grad[0][0];
+    div = div + grad[1][1];
+    div = div + grad[2][2];
+    sfor<0, Q>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      constexpr float c0 = lat_c(q, 0), c1 = lat_c(q, 1), c2 = lat_c(q, 2);
+      float ccgu = c0 * c0 * grad[0][0];
+      ccgu = ccgu + c0 * c1 * (grad[0][1] + grad[1][0]);
+      ccgu = ccgu + c0 * c2 * (grad[0][2] + grad[2][0]);
+      ccgu = ccgu + c1 * c1 * grad[1][1];
+      ccgu = ccgu + c1 * c2 * (grad[1][2] + grad[2][1]);
+      ccgu = ccgu + c2 * c2 * grad[2][2];
+      const float fneq = -lat_wf(Q, q) * r * tau * (three * ccgu - div);
+      fv[q] = fv[q] + fneq;
+    });
+  }
+  sfor<0, Q>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    pop_st<S, Q, q>(f + (size_t)q * d.N, i, fv[q]);
+  });
+}
+
+// f64 reductions (reduce_impl, backend.rs:920-970): one partial per block,
+// fixed tree inside the block, partials summed in block order on the host.
+template <int Q, typename S>
+__global__ __launch_bounds__(256) void k_reduce(Dom d, const S* f, const uint32_t* mask, const float* F_cur,
+                                                HydroParams p, int odd, int kind, int axis, double* partials) {
+  __shared__ double sm[256];
+  double acc = 0.0;
+  for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < d.N; i += gridDim.x * 256) {
+    if (mask[i] & kMaskSolid) continue;
+    if (kind == 0) {
+      acc += 1.0;
+      continue;
+    }
+    uint32_t x, y, z;
+    cell_xyz(d, i, x, y, z);
+    const Nbr nb(d, x, y, z);
+    double s = 0.0, mom = 0.0;
+    sfor<0, Q>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      const double v = (double)read_pop<Q, S, q>(f, d.N, nb, i, odd != 0);
+      s += v;
+      if (kind == 2) {
+        const int c = axis == 0 ? lat_c(q, 0) : (axis == 1 ? lat_c(q, 1) : lat_c(q, 2));
+        mom += (double)c * v;
+      }
+    });
+    if (kind == 1) {
+      acc += s;
+    } else {
+      double fa = (double)p.F[axis];
+      if (F_cur) fa = fa + ((double)F_cur[3 * (size_t)i + axis] + 0.0);
+      acc += mom + 0.5 * fa;
+    }
+  }
+  sm[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sm[threadIdx.x] += sm[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partials[blockIdx.x] = sm[0];
+}
+
+// masks: links in the pull convention plus cell flags
+template <int Q>
+__global__ void k_build_mask(Dom d, const uint8_t* solid, const float* wall_u, uint32_t* mask,
+                             unsigned int* err) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= d.N) return;
+  if (solid[i]) {
+    mask[i] = kMaskSolid;
+    return;
+  }
+  uint32_t x, y, z;
+  cell_xyz(d, i, x, y, z);
+  const bool edge = (!d.per[0] && (x == 0 || x == d.nx - 1)) || (!d.per[1] && (y == 0 || y == d.ny - 1)) ||
+                    (!d.per[2] && (z == 0 || z == d.nz - 1));
+  if (edge) atomicAdd(err, 1u);
+  const Nbr nb(d, x, y, z);
+  uint32_t m = 0;
+  sfor<1, Q>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    const uint32_t s = i + (uint32_t)nb.template off<-lat_c(q, 0), -lat_c(q, 1), -lat_c(q, 2)>();
+    if (solid[s]) {
+      m |= 1u << q;
+      if (wall_u && (wall_u[3 * (size_t)s] != 0.0f || wall_u[3 * (size_t)s + 1] != 0.0f ||
+                     wall_u[3 * (size_t)s + 2] != 0.0f))
+        m |= kMaskMoving;
+    }
+  });
+  mask[i] = m;
+}
+
+__global__ void k_mark_forced(const uint32_t* flist, uint32_t n, uint32_t* mask) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const uint32_t i = flist[k];
+  if (!(mask[i] & kMaskSolid)) mask[i] |= kMaskForced;
+}
+
+// apply_cylindrical_tank_mask + apply_baffle_mask (geometry.rs:657-723),
+// f64 with the same expressions (no contraction in this file).
+__global__ void k_tank(Dom d, double dx, double diameter, double height, int baffles, double bw, double bt,
+                       uint8_t* solid) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= d.N) return;
+  uint32_t x, y, z;
+  cell_xyz(d, i, x, y, z);
+  const double radius = 0.5 * diameter;
+  const double cx = 0.5 * (double)d.nx * dx, cy = 0.5 * (double)d.ny * dx;
+  const double z_m = ((double)z + 0.5) * dx;
+  const double pxm = ((double)x + 0.5) * dx, pym = ((double)y + 0.5) * dx;
+  const double r = sqrt((pxm - cx) * (pxm - cx) + (pym - cy) * (pym - cy));
+  const bool rim = x == 0 || y == 0 || z == 0 || x + 1 == d.nx || y + 1 == d.ny || z + 1 == d.nz;
+  const bool outside = r >= radius || z_m >= height;
+  bool s = rim || outside;
+  if (baffles == 4 && !rim && !(z_m >= height)) {
+    const double rx = pxm - cx, ry = pym - cy;
+    const double rb = sqrt(rx * rx + ry * ry);
+    const double radial_inner = radius - bw;
+    if (!(rb >= radius || rb < radial_inner)) {
+      if (fabs(ry) <= 0.5 * bt || fabs(rx) <= 0.5 * bt) s = true;
+    }
+  }
+  if (s) solid[i] = 1;
+}
+
+// build_wall_rims, solver.rs:1357-1406: one launch per face in the
+// reference's order; the faster wall keeps the corner.
+__global__ void k_rim_face(Dom d, int face, double speed, float u0, float u1, float u2, uint8_t* solid,
+                           float* wall_u, double* best) {
+  const int a = face / 2;
+  const int t1 = a == 0 ? 1 : 0, t2 = a == 2 ? 1 : 2;
+  const uint32_t n3[3] = {d.nx, d.ny, d.nz};
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n3[t1] * n3[t2]) return;
+  uint32_t pos[3];
+  pos[a] = (face % 2 == 0) ? 0 : n3[a] - 1;
+  pos[t1] = k % n3[t1];
+  pos[t2] = k / n3[t1];
+  const uint32_t i = (pos[2] * d.ny + pos[1]) * d.nx + pos[0];
+  solid[i] = 1;
+  // best[] holds the squared speed of the velocity stored so far (-1 = none)
+  if (speed > best[i]) {
+    best[i] = speed;
+    wall_u[3 * (size_t)i + 0] = u0;
+    wall_u[3 * (size_t)i + 1] = u1;
+    wall_u[3 * (size_t)i + 2] = u2;
+  }
+}
+
+__global__ void k_fill(float* p, size_t n, float v) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void k_copy_list(const uint32_t* flist, uint32_t n, float* dst, const float* src) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const size_t i = flist[k];
+  dst[3 * i] = src ? src[3 * i] : 0.0f;
+  dst[3 * i + 1] = src ? src[3 * i + 1] : 0.0f;
+  dst[3 * i + 2] = src ? src[3 * i + 2] : 0.0f;
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+Tiling make_tiling(const Dom& d, uint32_t tx_pref) {
+  Tiling t;
+  uint32_t tx = tx_pref ? tx_pref : 256;
+  while (tx > 64 && tx / 2 >= d.nx) tx /= 2;
+  t.tx = tx;
+  t.ty = 256 / tx;
+  t.bx = (d.nx + t.tx - 1) / t.tx;
+  t.by = (d.ny + t.ty - 1) / t.ty;
+  t.nblocks = t.bx * t.by * d.nz;
+  return t;
+}
+
+static inline uint32_t blocks_for(size_t n, uint32_t bs = 256) { return (uint32_t)((n + bs - 1) / bs); }
+
+template <int Q, typename S, int KIND>
+static hipError_t hydro_q(int odd, const HydroArgs& a, hipStream_t s, bool list) {
+  if (list) {
+    if (a.nlist == 0) return hipSuccess;
+    const dim3 grid(blocks_for(a.nlist)), block(256);
+    if (odd)
+      hipLaunchKernelGGL((k_hydro_list<Q, S, KIND, true>), grid, block, 0, s, a);
+    else
+      hipLaunchKernelGGL((k_hydro_list<Q, S, KIND, false>), grid, block, 0, s, a);
+    return hipGetLastError();
+  }
+  const dim3 grid(a.t.nblocks), block(a.t.tx, a.t.ty);
+  const bool fon = a.p.force_on != 0;
+  if (odd && fon)
+    hipLaunchKernelGGL((k_hydro<Q, S, KIND, true, true>), grid, block, 0, s, a);
+  else if (odd)
+    hipLaunchKernelGGL((k_hydro<Q, S, KIND, true, false>), grid, block, 0, s, a);
+  else if (fon)
+    hipLaunchKernelGGL((k_hydro<Q, S, KIND, false, true>), grid, block, 0, s, a);
+  else
+    hipLaunchKernelGGL((k_hydro<Q, S, KIND, false, false>), grid, block, 0, s, a);
+  return hipGetLastError();
+}
+
+template <typename S>
+static hipError_t hydro_s(int Q, int collision, int odd, const HydroArgs& a, hipStream_t s, bool list) {
+  const int kind = collision == 2 ? 1 : 0;
+  if (Q == 19 && kind == 0) return hydro_q<19, S, 0>(odd, a, s, list);
+  if (Q == 27 && kind == 0) return hydro_q<27, S, 0>(odd, a, s, list);
+  if (Q == 27 && kind == 1) return hydro_q<27, S, 1>(odd, a, s, list);
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_hydro(int Q, int storage, int collision, int odd, const HydroArgs& a, hipStream_t s) {
+  if (storage == kF32) return hydro_s<float>(Q, collision, odd, a, s, false);
+  if (storage == kF16) return hydro_s<_Float16>(Q, collision, odd, a, s, false);
+  return hydro_s<bf16s>(Q, collision, odd, a, s, false);
+}
+
+hipError_t launch_hydro_list(int Q, int storage, int collision, int odd, const HydroArgs& a, hipStream_t s) {
+  if (storage == kF32) return hydro_s<float>(Q, collision, odd, a, s, true);
+  if (storage == kF16) return hydro_s<_Float16>(Q, collision, odd, a, s, true);
+  return hydro_s<bf16s>(Q, collision, odd, a, s, true);
+}
+
+hipError_t launch_build_mask(int Q, const Dom& d, const uint8_t* solid, const float* wall_u, uint32_t* mask,
+                             unsigned int* err, hipStream_t s) {
+  if (Q == 19)
+    hipLaunchKernelGGL((k_build_mask<19>), dim3(blocks_for(d.N)), dim3(256), 0, s, d, solid, wall_u, mask, err);
+  else
+    hipLaunchKernelGGL((k_build_mask<27>), dim3(blocks_for(d.N)), dim3(256), 0, s, d, solid, wall_u, mask, err);
+  return hipGetLastError();
+}
+
+hipError_t launch_mark_forced(const uint32_t* flist, uint32_t n, uint32_t* mask, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_mark_forced, dim3(blocks_for(n)), dim3(256), 0, s, flist, n, mask);
+  return hipGetLastError();
+}
+
+template <typename S>
+static hipError_t stage_s(int Q, int dir, int odd, const Dom& d, void* f, float* h, const uint32_t* mask,
+                          hipStream_t s) {
+  if (Q == 19)
+    hipLaunchKernelGGL((k_stage<19, S>), dim3(blocks_for(d.N)), dim3(256), 0, s, d, (S*)f, h, mask, dir, odd);
+  else
+    hipLaunchKernelGGL((k_stage<27, S>), dim3(blocks_for(d.N)), dim3(256), 0, s, d, (S*)f, h, mask, dir, odd);
+  return hipGetLastError();
+}
+
+hipError_t launch_stage(int Q, int storage, int dir, int odd, const Dom& d, void* f, float* h, const uint32_t* mask,
+                        hipStream_t s) {
+  if (storage == kF32) return stage_s<float>(Q, dir, odd, d, f, h, mask, s);
+  if (storage == kF16) return stage_s<_Float16>(Q, dir, odd, d, f, h, mask, s);
+  return stage_s<bf16s>(Q, dir, odd, d, f, h, mask, s);
+}
+
+template <typename S>
+static hipError_t init_s(int Q, const Dom& d, void* f, const uint32_t* mask, const float* rho, const float* u,
+                         float tau, hipStream_t s) {
+  if (Q == 19)
+    hipLaunchKernelGGL((k_init_with<19, S>), dim3(blocks_for(d.N)), dim3(256), 0, s, d, (S*)f, mask, rho, u, tau);
+  else
+    hipLaunchKernelGGL((k_init_with<27, S>), dim3(blocks_for(d.N)), dim3(256), 0, s, d, (S*)f, mask, rho, u, tau);
+  return hipGetLastError();
+}
+
+hipError_t launch_init_with(int Q, int storage, const Dom& d, void* f, const uint32_t* mask, const float* rho,
+                            const float* u, float tau, hipStream_t s) {
+  if (storage == kF32) return init_s<float>(Q, d, f, mask, rho, u, tau, s);
+  if (storage == kF16) return init_s<_Float16>(Q, d, f, mask, rho, u, tau, s);
+  return init_s<bf16s>(Q, d, f, mask, rho, u, tau, s);
+}
+
+template <typename S>
+static hipError_t moments_s(int Q, int odd, const Dom& d, const void* f, const uint32_t* mask, const float* F_mom,
+                            HydroParams p, float* rho, float* ux, float* uy, float* uz, hipStream_t s) {
+  if (Q == 19)
+    hipLaunchKernelGGL((k_moments<19, S>), dim3(blocks_for(d.N)), dim3(256), 0, s, d, (const S*)f, mask, F_mom, p,
+                       odd, rho, ux, uy, uz);
+  else
+    hipLaunchKernelGGL((k_moments<27, S>), dim3(blocks_for(d.N)), dim3(256), 0, s, d, (const S*)f, mask, F_mom, p,
+                       odd, rho, ux, uy, uz);
+  return hipGetLastError();
+}
+
+hipError_t launch_moments(int Q, int storage, int odd, const Dom& d, const void* f, const uint32_t* mask,
+                          const float* F_mom, HydroParams p, float* rho, float* ux, float* uy, float* uz,
+                          hipStream_t s) {
+  if (storage == kF32) return moments_s<float>(Q, odd, d, f, mask, F_mom, p, rho, ux, uy, uz, s);
+  if (storage == kF16) return moments_s<_Float16>(Q, odd, d, f, mask, F_mom, p, rho, ux, uy, uz, s);
+  return moments_s<bf16s>(Q, odd, d, f, mask, F_mom, p, rho, ux, uy, uz, s);
+}
+
+template <typename S>
+static hipError_t reduce_s(int Q, int odd, const Dom& d, const void* f, const uint32_t* mask, const float* F,
+                           HydroParams p, int kind, int axis, double* partials, uint32_t np, hipStream_t s) {
+  if (Q == 19)
+    hipLaunchKernelGGL((k_reduce<19, S>), dim3(np), dim3(256), 0, s, d, (const S*)f, mask, F, p, odd, kind, axis,
+                       partials);
+  else
+    hipLaunchKernelGGL((k_reduce<27, S>), dim3(np), dim3(256), 0, s, d, (const S*)f, mask, F, p, odd, kind, axis,
+                       partials);
+  return hipGetLastError();
+}
+
+hipError_t launch_reduce(int Q, int storage, int odd, const Dom& d, const void* f, const uint32_t* mask,
+                         const float* F, HydroParams p, int kind, int axis, double* partials, uint32_t np,
+                         hipStream_t s) {
+  if (storage == kF32) return reduce_s<float>(Q, odd, d, f, mask, F, p, kind, axis, partials, np, s);
+  if (storage == kF16) return reduce_s<_Float16>(Q, odd, d, f, mask, F, p, kind, axis, partials, np, s);
+  return reduce_s<bf16s>(Q, odd, d, f, mask, F, p, kind, axis, partials, np, s);
+}
+
+hipError_t launch_tank(const Dom& d, double dx, double diameter, double height, int baffles, double bw, double bt,
+                       uint8_t* solid, hipStream_t s) {
+  hipLaunchKernelGGL(k_tank, dim3(blocks_for(d.N)), dim3(256), 0, s, d, dx, diameter, height, baffles, bw, bt,
+                     solid);
+  return hipGetLastError();
+}
+
+__global__ void k_filld(double* p, size_t n, double v) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+hipError_t launch_rims(const Dom& d, const int is_wall[6], const double u[18], uint8_t* solid, float* wall_u,
+                       double* best, hipStream_t s) {
+  hipLaunchKernelGGL(k_filld, dim3(blocks_for(d.N)), dim3(256), 0, s, best, (size_t)d.N, -1.0);
+  const int order[6] = {2, 3, 0, 1, 4, 5};  // YNeg, YPos, XNeg, XPos, ZNeg, ZPos
+  const uint32_t n3[3] = {d.nx, d.ny, d.nz};
+  for (int k = 0; k < 6; ++k) {
+    const int face = order[k];
+    if (!is_wall[face]) continue;
+    const int a = face / 2;
+    const int t1 = a == 0 ? 1 : 0, t2 = a == 2 ? 1 : 2;
+    const float u0 = (float)u[3 * face], u1 = (float)u[3 * face + 1], u2 = (float)u[3 * face + 2];
+    double speed = (double)u0 * (double)u0 + (double)u1 * (double)u1;
+    speed += (double)u2 * (double)u2;
+    const size_t cnt = (size_t)n3[t1] * n3[t2];
+    hipLaunchKernelGGL(k_rim_face, dim3(blocks_for(cnt)), dim3(256), 0, s, d, face, speed, u0, u1, u2, solid,
+                       wall_u, best);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+hipError_t launch_fill_f32(float* p, size_t n, float v, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill, dim3(blocks_for(n)), dim3(256), 0, s, p, n, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_copy_force_list(const uint32_t* flist, uint32_t n, float* dst, const float* src, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_copy_list, dim3(blocks_for(n)), dim3(256), 0, s, flist, n, dst, src);
+  return hipGetLastError();
+}
+
+}  // namespace lbm

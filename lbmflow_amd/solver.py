@@ -1,0 +1,373 @@
+"""Host-side mirror of lbm-core's ``Solver`` over the MI355X backend.
+
+The names, argument meanings and error behaviour follow the reference's
+``Solver`` (crates/lbm-core/src/solver.rs) and ``Backend`` trait
+(crates/lbm-core/src/backend.rs:126-363) for the single-phase stirred-tank
+path: build from solid/wall-velocity masks, step/run, init_with, body-force
+fields, rotating IBM, scalar distributions and the f64 reductions.  Every
+call goes through the C ABI of ``liblbm_hip.so``; there is no CPU path.
+
+Array conventions: compact global order ``cell = (z*ny + y)*nx + x``,
+populations q-major ``f[q, cell]`` in deviation form ``f - w``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib as _L
+
+MAGIC_STD = 3.0 / 16.0  # CollisionKind::MAGIC_STD, params.rs:202
+
+
+def omegas(collision: str, nu: float, magic: float = MAGIC_STD):
+    """``CollisionKind::omegas`` (params.rs:205-217)."""
+    tau = 3.0 * nu + 0.5
+    omega_p = 1.0 / tau
+    if collision == "trt":
+        lam_p = tau - 0.5
+        omega_m = 1.0 / (magic / lam_p + 0.5)
+    else:
+        omega_m = omega_p
+    return omega_p, omega_m
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _u8p(a):
+    return a.ctypes.data_as(C.POINTER(C.c_uint8))
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+class Context:
+    """One device and one HIP stream (calls on it are serialized)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = _L.lib()
+        h = C.c_void_p()
+        rc = self.lib.lbm_ctx_create(device, C.byref(h))
+        if rc != 0:
+            raise _L.LbmError(rc, f"cannot create a context on device {device}")
+        self.h = h
+
+    def check(self, rc):
+        if rc != 0:
+            msg = self.lib.lbm_last_error(self.h)
+            raise _L.LbmError(rc, msg.decode() if msg else "")
+
+    def capabilities(self) -> int:
+        v = C.c_uint32()
+        self.check(self.lib.lbm_capabilities(self.h, C.byref(v)))
+        return v.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.lbm_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+_default_ctx = None
+
+
+def default_context() -> Context:
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+@dataclass
+class DirectForcingConfig:
+    """``DirectForcingConfig`` (rotating_ibm.rs:104-130)."""
+
+    max_iterations: int = 3
+    slip_tolerance: float = 1.0e-2
+    kernel_radius: int = 1
+    relaxation: float = 1.0
+
+
+class Solver:
+    """A monolithic domain on one GPU (``Solver<L, f32, HipBackend, LocalPeriodic>``)."""
+
+    def __init__(self, lattice: int, dims, *, nu: float, collision: str = "trt", magic: float = MAGIC_STD,
+                 omega_shear: float | None = None, storage: str = "f32", periodic=(False, False, False),
+                 force=(0.0, 0.0, 0.0), solid=None, wall_u=None, ctx: Context | None = None):
+        self.ctx = ctx or default_context()
+        self.lib = self.ctx.lib
+        self.Q = int(lattice)
+        self.dims = tuple(int(v) for v in dims)
+        self.n = self.dims[0] * self.dims[1] * self.dims[2]
+        self.storage = storage
+        self.collision = collision
+        self.nu = float(nu)
+        d = np.array(self.dims, np.int32)
+        per = np.array([int(bool(p)) for p in periodic], np.int32)
+        h = C.c_void_p()
+        self.ctx.check(self.lib.lbm_fields_alloc(self.ctx.h, self.Q, _L.STORAGE[storage], _ip(d), _ip(per),
+                                                 C.byref(h)))
+        self.h = h
+        op, om = omegas(collision, self.nu, magic)
+        self.params = _L.StepParams()
+        self.params.collision = _L.COLLISION[collision]
+        self.params.omega_p = op
+        self.params.omega_m = om
+        self.params.omega_shear = op if omega_shear is None else float(omega_shear)
+        for a in range(3):
+            self.params.force[a] = float(force[a])
+        self.ctx.check(self.lib.lbm_set_params(self.h, C.byref(self.params)))
+        if solid is not None or wall_u is not None or not all(periodic):
+            self.set_masks(solid, wall_u)
+        self._ibm_key = None
+        self.scalars = []
+
+    # ---- lifetime ---------------------------------------------------------
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.lbm_fields_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _chk(self, rc):
+        self.ctx.check(rc)
+
+    # ---- masks / geometry -------------------------------------------------
+    def set_masks(self, solid=None, wall_u=None):
+        s = None if solid is None else np.ascontiguousarray(solid, np.uint8).reshape(-1)
+        w = None if wall_u is None else np.ascontiguousarray(wall_u, np.float32).reshape(-1)
+        if s is not None and s.size != self.n:
+            raise ValueError("solid mask has the wrong size")
+        if w is not None and w.size != 3 * self.n:
+            raise ValueError("wall velocity has the wrong size")
+        self._chk(self.lib.lbm_set_masks(self.h, None if s is None else _u8p(s), None if w is None else _fp(w)))
+
+    def build_wall_rims(self, is_wall, u):
+        """``build_wall_rims`` (solver.rs:1357-1406) on the device."""
+        iw = np.array([int(bool(v)) for v in is_wall], np.int32)
+        uu = np.ascontiguousarray(u, np.float64).reshape(18)
+        self._chk(self.lib.lbm_geom_wall_rims(self.h, _ip(iw), _dp(uu)))
+
+    def build_stirred_tank(self, dx, vessel_diameter, liquid_height, baffle_count=4, baffle_width=0.0,
+                           baffle_thickness=0.0):
+        """Tank mask + four wall baffles (geometry.rs:657-723) on the device."""
+        self._chk(self.lib.lbm_geom_stirred_tank(self.h, dx, vessel_diameter, liquid_height, baffle_count,
+                                                 baffle_width, baffle_thickness))
+
+    def solid(self):
+        out = np.zeros(self.n, np.uint8)
+        self._chk(self.lib.lbm_read_solid(self.h, _u8p(out)))
+        return out
+
+    def boundary_links(self):
+        cnt = C.c_int64()
+        self._chk(self.lib.lbm_boundary_links(self.h, C.byref(cnt), None, None, 0))
+        cells = np.zeros(cnt.value, np.int64)
+        qs = np.zeros(cnt.value, np.uint8)
+        self._chk(self.lib.lbm_boundary_links(self.h, C.byref(cnt), cells.ctypes.data_as(C.POINTER(C.c_int64)),
+                                              _u8p(qs), cnt.value))
+        return cells, qs
+
+    # ---- state ------------------------------------------------------------
+    def set_populations(self, f):
+        """``Backend::stage_in`` (q-major compact deviation populations)."""
+        f = np.ascontiguousarray(f, np.float32).reshape(-1)
+        if f.size != self.Q * self.n:
+            raise ValueError("population array has the wrong size")
+        self._chk(self.lib.lbm_stage_in(self.h, _fp(f)))
+
+    def populations(self):
+        """``Backend::stage_out``; solid cells read as zero."""
+        out = np.zeros((self.Q, self.n), np.float32)
+        self._chk(self.lib.lbm_stage_out(self.h, _fp(out)))
+        return out
+
+    def init_with(self, rho, u):
+        """``Solver::init_with`` (solver.rs:2474-2623) from per-cell rho and u."""
+        r = np.ascontiguousarray(np.broadcast_to(np.asarray(rho, np.float32), (self.n,)))
+        uu = np.ascontiguousarray(np.broadcast_to(np.asarray(u, np.float32), (self.n, 3)))
+        self._chk(self.lib.lbm_init_with(self.h, _fp(r), _fp(uu)))
+
+    def set_omega_field(self, omega):
+        if omega is None:
+            self._chk(self.lib.lbm_set_omega_field(self.h, None))
+        else:
+            om = np.ascontiguousarray(omega, np.float32).reshape(-1)
+            self._chk(self.lib.lbm_set_omega_field(self.h, _fp(om)))
+
+    def set_body_force_field(self, ff):
+        """``Solver::set_body_force_field`` (solver.rs:2776-2801)."""
+        ff = np.ascontiguousarray(ff, np.float32).reshape(-1)
+        if ff.size != 3 * self.n:
+            raise ValueError("force field has the wrong size")
+        self._chk(self.lib.lbm_set_force_field(self.h, _fp(ff)))
+
+    def clear_body_force_field(self):
+        """``Solver::clear_body_force_field`` (solver.rs:2805-2822)."""
+        self._chk(self.lib.lbm_clear_force_field(self.h))
+
+    def body_force_field(self):
+        out = np.zeros((self.n, 3), np.float32)
+        present = C.c_int()
+        self._chk(self.lib.lbm_read_force_field(self.h, _fp(out), C.byref(present)))
+        return out if present.value else None
+
+    # ---- stepping -----------------------------------------------------------
+    def step(self):
+        self._chk(self.lib.lbm_run_span(self.h, 1))
+
+    def run(self, steps: int):
+        self._chk(self.lib.lbm_run_span(self.h, int(steps)))
+
+    def synchronize(self):
+        self._chk(self.lib.lbm_synchronize(self.h))
+
+    def kernel_timing(self, enable: bool):
+        self._chk(self.lib.lbm_kernel_timing(self.h, int(bool(enable))))
+
+    def kernel_timing_read(self):
+        ms = C.c_double()
+        n = C.c_int64()
+        self._chk(self.lib.lbm_kernel_timing_read(self.h, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def force_list_size(self) -> int:
+        return int(self.lib.lbm_force_list_size(self.h))
+
+    @property
+    def time(self) -> int:
+        return int(self.lib.lbm_time(self.h))
+
+    # ---- readback / reductions ---------------------------------------------
+    def moments(self):
+        r, a, b, c = (np.zeros(self.n, np.float32) for _ in range(4))
+        self._chk(self.lib.lbm_read_moments(self.h, _fp(r), _fp(a), _fp(b), _fp(c)))
+        return r, a, b, c
+
+    def reduce(self, kind: int, axis: int = 0) -> float:
+        out = C.c_double()
+        self._chk(self.lib.lbm_reduce(self.h, kind, axis, C.byref(out)))
+        return out.value
+
+    def fluid_cells(self) -> float:
+        return self.reduce(0)
+
+    def mass_deviation(self) -> float:
+        return self.reduce(1)
+
+    def total_mass_f64(self) -> float:
+        """``Solver::total_mass_f64`` (solver.rs:5114-5135)."""
+        return self.fluid_cells() + self.mass_deviation()
+
+    def momentum(self):
+        return np.array([self.reduce(2, a) for a in range(3)])
+
+    def device_bytes(self) -> int:
+        return int(self.lib.lbm_fields_device_bytes(self.h))
+
+    # ---- rotating IBM ------------------------------------------------------
+    def set_rotating_body(self, positions, weights, center, omega, cfg: DirectForcingConfig | None = None):
+        cfg = cfg or DirectForcingConfig()
+        pos = np.ascontiguousarray(positions, np.float64).reshape(-1)
+        w = np.ascontiguousarray(weights, np.float64).reshape(-1)
+        c = np.ascontiguousarray(center, np.float64)
+        o = np.ascontiguousarray(omega, np.float64)
+        ic = _L.IbmConfig(cfg.max_iterations, cfg.slip_tolerance, cfg.kernel_radius, cfg.relaxation)
+        self._chk(self.lib.lbm_ibm_set_body(self.h, w.size, _dp(pos), _dp(w), _dp(c), _dp(o), C.byref(ic)))
+
+    def apply_rotating_ibm(self):
+        """``Solver::apply_rotating_ibm`` (solver.rs:3016-3282) for the registered body."""
+        d = _L.IbmDiag()
+        self._chk(self.lib.lbm_ibm_apply(self.h, C.byref(d)))
+        return {
+            "iterations": d.iterations, "slip_max": d.slip_max, "slip_rms": d.slip_rms,
+            "slip_max_rel": d.slip_max_rel, "slip_rms_rel": d.slip_rms_rel, "torque": np.array(d.torque[:]),
+            "fluid_force": np.array(d.fluid_force[:]), "marker_force": np.array(d.marker_force[:]),
+            "momentum_error_rel": d.momentum_error_rel,
+        }
+
+    # ---- scalar distributions ----------------------------------------------
+    def enable_scalar_distribution(self, diffusivity: float, lattice: str = "hydro") -> int:
+        idx = C.c_int()
+        lat = 7 if lattice == "d3q7" else 0
+        self._chk(self.lib.lbm_scalar_enable(self.h, float(diffusivity), lat, C.byref(idx)))
+        self.scalars.append(lattice)
+        return idx.value
+
+    def set_scalar_concentration(self, index: int, values):
+        v = np.ascontiguousarray(values, np.float32).reshape(-1)
+        if v.size != self.n:
+            raise ValueError(f"scalar concentration length {v.size} does not match cell count {self.n}")
+        self._chk(self.lib.lbm_scalar_set(self.h, index, _fp(v)))
+
+    def gather_scalar_concentration(self, index: int):
+        out = np.zeros(self.n, np.float32)
+        self._chk(self.lib.lbm_scalar_gather(self.h, index, _fp(out)))
+        return out
+
+    def scalar_cv(self, index: int):
+        cv = C.c_double()
+        ok = C.c_int()
+        self._chk(self.lib.lbm_scalar_cv(self.h, index, C.byref(cv), C.byref(ok)))
+        return cv.value if ok.value else None
+
+
+# ---- geometry and QOI helpers ---------------------------------------------------
+
+def radial_impeller(dims, dx, vessel_diameter, liquid_height, diameter, clearance, blades=6, blade_angle_deg=0.0,
+                    omega_z=0.0, thickness_fraction=0.04):
+    """``build_radial_impeller`` (geometry.rs:815-906): markers in lattice units."""
+    lib = _L.lib()
+    d = np.array(dims, np.int32)
+    cnt = C.c_int64()
+    rc = lib.lbm_geom_radial_impeller(_ip(d), dx, vessel_diameter, liquid_height, diameter, clearance, blades,
+                                      blade_angle_deg, omega_z, thickness_fraction, None, None, None, None, 0,
+                                      C.byref(cnt))
+    if rc != 0:
+        raise _L.LbmError(rc, "impeller marker generation produced no markers")
+    n = cnt.value
+    pos = np.zeros((n, 3))
+    w = np.zeros(n)
+    vel = np.zeros((n, 3))
+    c = np.zeros(3)
+    rc = lib.lbm_geom_radial_impeller(_ip(d), dx, vessel_diameter, liquid_height, diameter, clearance, blades,
+                                      blade_angle_deg, omega_z, thickness_fraction, _dp(pos), _dp(w), _dp(vel),
+                                      _dp(c), n, C.byref(cnt))
+    if rc != 0:
+        raise _L.LbmError(rc, "impeller marker generation failed")
+    return pos, w, vel, c
+
+
+def power_qois(torque, omega, rho, diameter, volume):
+    """``qoi::power_qois`` (qoi.rs:477-505)."""
+    out = np.zeros(4)
+    rc = _L.lib().lbm_qoi_power(torque, omega, rho, diameter, volume, _dp(out))
+    if rc != 0:
+        raise ValueError("power QOI inputs must be finite and positive where dimensional")
+    return {"power": out[0], "n_hz": out[1], "np": out[2], "p_over_v": out[3]}
+
+
+def mixing_time_from_cv(t, cv):
+    """``qoi::mixing_time_from_cv`` (qoi.rs:534-556)."""
+    t = np.ascontiguousarray(t, np.float64)
+    cv = np.ascontiguousarray(cv, np.float64)
+    out = np.zeros(3)
+    rc = _L.lib().lbm_qoi_mixing_time(t.size, _dp(t), _dp(cv), _dp(out))
+    if rc != 0:
+        return None
+    f = lambda v: None if np.isnan(v) else float(v)  # noqa: E731
+    return {"cv0": float(out[0]), "t95": f(out[1]), "t99": f(out[2])}

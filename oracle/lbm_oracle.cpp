@@ -1,0 +1,1562 @@
+// lbm_oracle.cpp — CPU restatement of lbm-core's CpuScalar lattice-Boltzmann
+// path (TakuTsuzuki/lbmflow).  TEST INFRASTRUCTURE ONLY: nothing in the
+// product library links or calls this file; tests, the smoke check and the
+// bench's CPU-baseline leg load it as the checker.
+//
+// Every routine cites the reference file:line it restates.  Floating-point
+// expressions keep the reference's operand order and grouping; the build
+// uses -ffp-contract=off so no multiply-add is fused behind our back.
+//
+// Pins (see oracle/README-pins in tests/test_oracle_*.py):
+//   lattice invariants, omegas, equilibrium fixed point, CM rest fixed point,
+//   uniform-u uniformity, closed-box mass conservation, TGV decay, geometry
+//   counts/volumes, QOI closed forms, IBM single-marker exactness.
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+namespace orc {
+
+// ---------------------------------------------------------------------------
+// Lattice tables — lattice.rs:295-442 (D3Q19/D3Q27 velocity order and
+// weights), opp/pairs derived as in lattice.rs:158-191.  D3Q7 follows the
+// scalar-lattice definition (rest + 6 axis links, w0 = 1/4, w1..6 = 1/8),
+// docs/archive/2026-07-07-pivot/proposals/WSCAL_MULTICOMPONENT_SPEC.md:68-71.
+// ---------------------------------------------------------------------------
+static const int8_t C27[27][3] = {
+    {0, 0, 0},   {1, 0, 0},   {-1, 0, 0}, {0, 1, 0},   {0, -1, 0}, {0, 0, 1},  {0, 0, -1},
+    {1, 1, 0},   {-1, -1, 0}, {1, 0, 1},  {-1, 0, -1}, {0, 1, 1},  {0, -1, -1}, {1, -1, 0},
+    {-1, 1, 0},  {1, 0, -1},  {-1, 0, 1}, {0, 1, -1},  {0, -1, 1}, {1, 1, 1},   {-1, -1, -1},
+    {1, 1, -1},  {-1, -1, 1}, {1, -1, 1}, {-1, 1, -1}, {1, -1, -1}, {-1, 1, 1}};
+
+struct Lat {
+  int Q = 0;
+  int D = 3;
+  int8_t c[27][3];
+  double w[27];
+  int opp[27];
+  int npairs = 0;
+  int pairs[13][2];
+};
+
+static Lat make_lat(int Q) {
+  Lat L;
+  L.Q = Q;
+  for (int q = 0; q < Q; ++q)
+    for (int a = 0; a < 3; ++a) L.c[q][a] = C27[q][a];
+  for (int q = 0; q < Q; ++q) {
+    if (Q == 19) L.w[q] = q == 0 ? 1.0 / 3.0 : (q < 7 ? 1.0 / 18.0 : 1.0 / 36.0);
+    if (Q == 27) L.w[q] = q == 0 ? 8.0 / 27.0 : (q < 7 ? 2.0 / 27.0 : (q < 19 ? 1.0 / 54.0 : 1.0 / 216.0));
+    if (Q == 7) L.w[q] = q == 0 ? 1.0 / 4.0 : 1.0 / 8.0;
+  }
+  for (int q = 0; q < Q; ++q) {
+    L.opp[q] = -1;
+    for (int r = 0; r < Q; ++r)
+      if (L.c[r][0] == -L.c[q][0] && L.c[r][1] == -L.c[q][1] && L.c[r][2] == -L.c[q][2]) L.opp[q] = r;
+  }
+  for (int q = 0; q < Q; ++q)
+    if (q != 0 && q < L.opp[q]) {
+      L.pairs[L.npairs][0] = q;
+      L.pairs[L.npairs][1] = L.opp[q];
+      L.npairs++;
+    }
+  return L;
+}
+
+// ---------------------------------------------------------------------------
+// Central-moment transform — kernels.rs:162-392.
+// ---------------------------------------------------------------------------
+struct MomentTransform {
+  int Q;
+  uint8_t basis[27][3];
+  int exp_index[3][3][3];
+  double raw_from_pop[27][27];
+  double pop_from_raw[27][27];
+};
+
+static double pow_upto2(double x, int e) { return e == 0 ? 1.0 : (e == 1 ? x : x * x); }
+static double binom_upto2(int n, int k) {
+  if (k == 0) return 1.0;
+  if (n == 1 && k == 1) return 1.0;
+  if (n == 2 && k == 1) return 2.0;
+  return 1.0;  // (2,2)
+}
+
+static MomentTransform build_transform(const Lat& L) {
+  MomentTransform t;
+  t.Q = L.Q;
+  int n = 0;
+  // central_basis, kernels.rs:162-183
+  for (int ax = 0; ax <= 2; ++ax)
+    for (int ay = 0; ay <= 2; ++ay)
+      for (int az = 0; az <= 2; ++az) {
+        if (L.Q == 19 && ax > 0 && ay > 0 && az > 0) continue;
+        if (n < L.Q) {
+          t.basis[n][0] = ax;
+          t.basis[n][1] = ay;
+          t.basis[n][2] = az;
+          n++;
+        }
+      }
+  for (int i = 0; i < 27; ++i) t.exp_index[i / 9][(i / 3) % 3][i % 3] = -1;
+  // build_moment_transform, kernels.rs:279-297; raw_phi kernels.rs:195-201
+  for (int m = 0; m < L.Q; ++m) {
+    const uint8_t* e = t.basis[m];
+    t.exp_index[e[0]][e[1]][e[2]] = m;
+    for (int q = 0; q < L.Q; ++q) {
+      double v = pow_upto2((double)L.c[q][0], e[0]);
+      v *= pow_upto2((double)L.c[q][1], e[1]);
+      v *= pow_upto2((double)L.c[q][2], e[2]);
+      t.raw_from_pop[m][q] = v;
+    }
+  }
+  // invert_raw_moment_matrix, kernels.rs:227-277 (Gauss-Jordan, partial pivot)
+  const int N = L.Q;
+  static double a[27][54];
+  for (int r = 0; r < N; ++r) {
+    for (int c = 0; c < 2 * N; ++c) a[r][c] = 0.0;
+    for (int c = 0; c < N; ++c) a[r][c] = t.raw_from_pop[r][c];
+    a[r][N + r] = 1.0;
+  }
+  for (int col = 0; col < N; ++col) {
+    int pivot = col;
+    for (int row = col + 1; row < N; ++row)
+      if (std::fabs(a[row][col]) > std::fabs(a[pivot][col])) pivot = row;
+    if (pivot != col)
+      for (int j = 0; j < 2 * N; ++j) std::swap(a[pivot][j], a[col][j]);
+    double inv = 1.0 / a[col][col];
+    for (int j = col; j < 2 * N; ++j) a[col][j] *= inv;
+    for (int row = 0; row < N; ++row) {
+      if (row == col) continue;
+      double factor = a[row][col];
+      if (factor == 0.0) continue;
+      for (int j = col; j < 2 * N; ++j) a[row][j] -= factor * a[col][j];
+    }
+  }
+  for (int r = 0; r < N; ++r)
+    for (int c = 0; c < N; ++c) t.pop_from_raw[r][c] = a[r][N + c];
+  return t;
+}
+
+static const MomentTransform& transform_for(int Q) {
+  static MomentTransform t19 = build_transform(make_lat(19));
+  static MomentTransform t27 = build_transform(make_lat(27));
+  return Q == 19 ? t19 : t27;
+}
+
+// shifted_moments, kernels.rs:332-363
+static void shifted_moments(const MomentTransform& t, const double u[3], const double* in, double* out,
+                            bool raw_to_central) {
+  for (int m = 0; m < t.Q; ++m) {
+    const uint8_t* e = t.basis[m];
+    double acc = 0.0;
+    for (int kx = 0; kx <= e[0]; ++kx)
+      for (int ky = 0; ky <= e[1]; ++ky)
+        for (int kz = 0; kz <= e[2]; ++kz) {
+          int idx = t.exp_index[kx][ky][kz];
+          double coef = binom_upto2(e[0], kx) * binom_upto2(e[1], ky) * binom_upto2(e[2], kz);
+          double sx = raw_to_central ? -u[0] : u[0];
+          double sy = raw_to_central ? -u[1] : u[1];
+          double sz = raw_to_central ? -u[2] : u[2];
+          coef *= pow_upto2(sx, e[0] - kx);
+          coef *= pow_upto2(sy, e[1] - ky);
+          coef *= pow_upto2(sz, e[2] - kz);
+          acc += coef * in[idx];
+        }
+    out[m] = acc;
+  }
+}
+
+// populations_to_central_moments, kernels.rs:318-330
+static void pops_to_central(const MomentTransform& t, const double u[3], const double* pop, double* out) {
+  double raw[27];
+  for (int m = 0; m < t.Q; ++m) {
+    raw[m] = 0.0;
+    for (int q = 0; q < t.Q; ++q) raw[m] += t.raw_from_pop[m][q] * pop[q];
+  }
+  shifted_moments(t, u, raw, out, true);
+}
+
+// ---------------------------------------------------------------------------
+// Solver state — fields.rs:214-303 (q-major padded SoA deviation storage,
+// compact moments), subdomain.rs:31-44 (monolithic, halo 1).
+// ---------------------------------------------------------------------------
+template <typename T>
+struct ScalarDist {
+  double diffusivity;
+  Lat L;  // hydro lattice (reference) or D3Q7 (new)
+  bool d3q7;
+  std::vector<T> h, htmp, conc;
+};
+
+template <typename T>
+struct Solver {
+  Lat L;
+  int nx, ny, nz;
+  bool periodic[3];
+  int px, py, pz;
+  size_t np, nc;
+  std::vector<T> f, ftmp;
+  std::vector<T> rho, ux, uy, uz;
+  std::vector<uint8_t> solid;  // padded
+  std::vector<T> wall_u;       // padded x3
+  bool has_ff = false;
+  std::vector<T> ff;  // compact x3
+  bool has_omega = false;
+  std::vector<T> omega_field;  // compact
+  int collision;               // 0 BGK, 1 TRT, 2 CM
+  double nu, omega_p, omega_m, omega_shear_in;
+  T force[3];
+  // KParams, params.rs:350-374
+  T kp_op, kp_om, kp_os, kp_cp, kp_cm;
+  T cr[27][3], wr[27];
+  std::vector<ScalarDist<T>> scalars;
+  long long time = 0;
+
+  size_t pidx(long x, long y, long z) const {
+    return ((size_t)(z + 1) * py + (size_t)(y + 1)) * px + (size_t)(x + 1);
+  }
+  size_t cidx(long x, long y, long z) const { return ((size_t)z * ny + y) * nx + x; }
+
+  bool force_on() const {  // params.rs:376-383 (no gravity on this path)
+    return force[0] != T(0) || force[1] != T(0) || force[2] != T(0) || has_ff;
+  }
+  void force_at(size_t c, T out[3]) const {  // params.rs:385-405
+    if (has_ff) {
+      out[0] = force[0] + ff[3 * c + 0];
+      out[1] = force[1] + ff[3 * c + 1];
+      out[2] = force[2] + ff[3 * c + 2];
+    } else {
+      out[0] = force[0];
+      out[1] = force[1];
+      out[2] = force[2];
+    }
+  }
+
+  // Fill halo cells of periodic axes with the wrapped core values (the
+  // LocalPeriodic exchange, halo.rs:77-100; every plane copied).
+  template <typename V>
+  void wrap_halo(std::vector<V>& arr, int planes, int comps) {
+    for (long z = -1; z <= nz; ++z)
+      for (long y = -1; y <= ny; ++y)
+        for (long x = -1; x <= nx; ++x) {
+          bool in_halo = x < 0 || x >= nx || y < 0 || y >= ny || z < 0 || z >= nz;
+          if (!in_halo) continue;
+          long s[3] = {x, y, z};
+          long n[3] = {nx, ny, nz};
+          bool ok = true;
+          for (int a = 0; a < 3; ++a) {
+            if (s[a] < 0 || s[a] >= n[a]) {
+              if (!periodic[a]) {
+                ok = false;
+                break;
+              }
+              s[a] = (s[a] + n[a]) % n[a];
+            }
+          }
+          if (!ok) continue;
+          size_t di = pidx(x, y, z), si = pidx(s[0], s[1], s[2]);
+          for (int p = 0; p < planes; ++p)
+            for (int k = 0; k < comps; ++k) arr[(p * np + di) * comps + k] = arr[(p * np + si) * comps + k];
+        }
+  }
+
+  void init(int Q, const int dims[3], const int per[3], int coll, double nu_, double magic, double os,
+            const double frc[3]) {
+    L = make_lat(Q);
+    nx = dims[0];
+    ny = dims[1];
+    nz = dims[2];
+    for (int a = 0; a < 3; ++a) periodic[a] = per[a] != 0;
+    px = nx + 2;
+    py = ny + 2;
+    pz = nz + 2;
+    np = (size_t)px * py * pz;
+    nc = (size_t)nx * ny * nz;
+    f.assign((size_t)Q * np, T(0));
+    ftmp.assign((size_t)Q * np, T(0));
+    rho.assign(nc, T(1));
+    ux.assign(nc, T(0));
+    uy.assign(nc, T(0));
+    uz.assign(nc, T(0));
+    solid.assign(np, 0);
+    wall_u.assign(np * 3, T(0));
+    collision = coll;
+    nu = nu_;
+    // CollisionKind::omegas, params.rs:205-217
+    double tau = 3.0 * nu + 0.5;
+    omega_p = 1.0 / tau;
+    if (coll == 1) {
+      double lam_p = tau - 0.5;
+      omega_m = 1.0 / (magic / lam_p + 0.5);
+    } else {
+      omega_m = omega_p;
+    }
+    omega_shear_in = coll == 2 ? os : omega_p;
+    for (int a = 0; a < 3; ++a) force[a] = (T)frc[a];
+    kp_op = (T)omega_p;
+    kp_om = (T)omega_m;
+    kp_os = (T)omega_shear_in;
+    kp_cp = (T)(1.0 - omega_p / 2.0);
+    kp_cm = (T)(1.0 - omega_m / 2.0);
+    for (int q = 0; q < 27; ++q) {
+      for (int a = 0; a < 3; ++a) cr[q][a] = q < Q ? (T)(double)L.c[q][a] : T(0);
+      wr[q] = q < Q ? (T)L.w[q] : T(0);
+    }
+  }
+
+  // Solver::build tail, solver.rs:1829-1891: masks into padded core,
+  // exchange_masks, update_moments.
+  void set_masks(const uint8_t* solid_c, const double* wall_c) {
+    for (long z = 0; z < nz; ++z)
+      for (long y = 0; y < ny; ++y)
+        for (long x = 0; x < nx; ++x) {
+          size_t gi = cidx(x, y, z), pi = pidx(x, y, z);
+          solid[pi] = solid_c ? solid_c[gi] : 0;
+          for (int a = 0; a < 3; ++a) wall_u[3 * pi + a] = wall_c ? (T)wall_c[3 * gi + a] : T(0);
+        }
+    wrap_halo(solid, 1, 1);
+    wrap_halo(wall_u, 1, 3);
+    update_moments();
+  }
+
+  // ---- kernels ------------------------------------------------------------
+  // TrtGuo::relax via collide_row, kernels.rs:129-160, collision.rs:123-202.
+  void collide_trt_cell(long x, long y, long z) {
+    size_t i = pidx(x, y, z), c = cidx(x, y, z);
+    const int Q = L.Q;
+    T r = rho[c];
+    T u[3] = {ux[c], uy[c], uz[c]};
+    T fv[3];
+    force_at(c, fv);
+    T op = has_omega ? omega_field[c] : kp_op;
+    T cp = has_omega ? (T(1) - op / T(2.0)) : kp_cp;
+    bool fon = force_on();
+    const T half = T(0.5);
+    T usq = u[0] * u[0];
+    for (int d = 1; d < 3; ++d) {
+      T ud2 = u[d] * u[d];
+      usq = usq + ud2;
+    }
+    T uf = u[0] * fv[0];
+    for (int d = 1; d < 3; ++d) {
+      T udfd = u[d] * fv[d];
+      uf = uf + udfd;
+    }
+    T feq[27], src[27];
+    for (int q = 0; q < Q; ++q) {
+      // dot_u, collision.rs:53-65
+      T cu = cr[q][0] * u[0];
+      for (int d = 1; d < 3; ++d) {
+        T term = cr[q][d] * u[d];
+        cu = cu + term;
+      }
+      // SecondOrderEq::feq, collision.rs:81-102
+      T drho = r - T(1.0);
+      T three_cu = T(3.0) * cu;
+      T f45_cu = T(4.5) * cu;
+      T f45_cu2 = f45_cu * cu;
+      T f15_usq = T(1.5) * usq;
+      T sum = three_cu + f45_cu2;
+      T inner = sum - f15_usq;
+      T rho_inner = r * inner;
+      T body = drho + rho_inner;
+      feq[q] = wr[q] * body;
+      // GuoForcing::source, collision.rs:104-121
+      if (!fon) {
+        src[q] = T(0.0);
+      } else {
+        T cf = cr[q][0] * fv[0];
+        for (int d = 1; d < 3; ++d) {
+          T term = cr[q][d] * fv[d];
+          cf = cf + term;
+        }
+        T cf_minus_uf = cf - uf;
+        T first = T(3.0) * cf_minus_uf;
+        T nine_cu = T(9.0) * cu;
+        T second = nine_cu * cf;
+        T b = first + second;
+        src[q] = wr[q] * b;
+      }
+    }
+    T f0 = f[0 * np + i];
+    T f0_minus_e = f0 - feq[0];
+    T op_delta = op * f0_minus_e;
+    T relaxed = f0 - op_delta;
+    T cp_src = cp * src[0];
+    f[0 * np + i] = relaxed + cp_src;
+    for (int k = 0; k < L.npairs; ++k) {
+      int qa = L.pairs[k][0], qb = L.pairs[k][1];
+      T fa = f[qa * np + i], fb = f[qb * np + i];
+      T fp = half * (fa + fb);
+      T fm = half * (fa - fb);
+      T ep = half * (feq[qa] + feq[qb]);
+      T em = half * (feq[qa] - feq[qb]);
+      T sp = half * (src[qa] + src[qb]);
+      T sm = half * (src[qa] - src[qb]);
+      T rp = op * (fp - ep);
+      T rm = kp_om * (fm - em);
+      T cp_sp = cp * sp;
+      T cm_sm = kp_cm * sm;
+      T fa_rp = fa - rp;
+      T fa_rp_rm = fa_rp - rm;
+      T src_a = cp_sp + cm_sm;
+      T va = fa_rp_rm + src_a;
+      T fb_rp = fb - rp;
+      T fb_rp_rm = fb_rp + rm;
+      T src_b = cp_sp - cm_sm;
+      T vb = fb_rp_rm + src_b;
+      f[qa * np + i] = va;
+      f[qb * np + i] = vb;
+    }
+  }
+
+  // collide_row_central_moment, kernels.rs:406-533 (f64 internally).
+  void collide_cm_cell(long x, long y, long z) {
+    const MomentTransform& tr = transform_for(L.Q);
+    const int Q = L.Q;
+    size_t i = pidx(x, y, z), c = cidx(x, y, z);
+    T r_t = rho[c];
+    double r = (double)r_t;
+    double u[3] = {(double)ux[c], (double)uy[c], (double)uz[c]};
+    T fv_t[3];
+    force_at(c, fv_t);
+    double fv[3] = {(double)fv_t[0], (double)fv_t[1], (double)fv_t[2]};
+    bool fon = fv[0] != 0.0 || fv[1] != 0.0 || fv[2] != 0.0;
+    double phys[27], src[27];
+    for (int q = 0; q < 27; ++q) src[q] = 0.0;
+    bool feq_rest = u[0] == 0.0 && u[1] == 0.0 && u[2] == 0.0 && !fon;
+    bool exact_rest = feq_rest;
+    for (int q = 0; q < Q; ++q) {
+      T fq = f[q * np + i];
+      phys[q] = (double)fq + L.w[q];
+      if (exact_rest) {
+        T expected = (T)(L.w[q] * (r - 1.0));
+        double a = (double)fq, b = (double)expected;
+        uint64_t ba, bb;
+        std::memcpy(&ba, &a, 8);
+        std::memcpy(&bb, &b, 8);
+        exact_rest = exact_rest && ba == bb;
+      }
+    }
+    if (exact_rest) return;
+    if (fon) {
+      double uf = u[0] * fv[0];
+      for (int d = 1; d < 3; ++d) uf += u[d] * fv[d];
+      for (int q = 0; q < Q; ++q) {
+        double cu = (double)L.c[q][0] * u[0];
+        double cf = (double)L.c[q][0] * fv[0];
+        for (int d = 1; d < 3; ++d) {
+          cu += (double)L.c[q][d] * u[d];
+          cf += (double)L.c[q][d] * fv[d];
+        }
+        src[q] = L.w[q] * (3.0 * (cf - uf) + 9.0 * cu * cf);
+      }
+    }
+    double usq = u[0] * u[0];
+    for (int d = 1; d < 3; ++d) usq += u[d] * u[d];
+    double feq_phys[27];
+    for (int q = 0; q < Q; ++q) {
+      double cu = (double)L.c[q][0] * u[0];
+      for (int d = 1; d < 3; ++d) cu += (double)L.c[q][d] * u[d];
+      feq_phys[q] = L.w[q] * r * (1.0 + 3.0 * cu + 4.5 * cu * cu - 1.5 * usq);
+    }
+    double mom[27], eq[27], src_mom[27];
+    pops_to_central(tr, u, phys, mom);
+    pops_to_central(tr, u, feq_phys, eq);
+    pops_to_central(tr, u, src, src_mom);
+    double os_base = has_omega ? (double)omega_field[c] : (double)kp_os;
+    double usq2 = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+    double vc = -0.16 * usq2;
+    double os = std::min(os_base * (1.0 + vc), 2.0);
+    double post[27];
+    int diag[3] = {-1, -1, -1};
+    for (int m = 0; m < Q; ++m) {
+      const uint8_t* e = tr.basis[m];
+      int order = e[0] + e[1] + e[2];
+      if (order == 2)
+        for (int a = 0; a < 3; ++a) {
+          uint8_t de[3] = {0, 0, 0};
+          de[a] = 2;
+          if (e[0] == de[0] && e[1] == de[1] && e[2] == de[2]) diag[a] = m;
+        }
+      double rate = order <= 1 ? 0.0 : (order == 2 ? os : 1.0);
+      post[m] = mom[m] - rate * (mom[m] - eq[m]) + (1.0 - 0.5 * rate) * src_mom[m];
+    }
+    double inv_d = 1.0 / 3.0;
+    double trace_neq = 0.0, trace_src = 0.0;
+    for (int a = 0; a < 3; ++a) {
+      trace_neq += mom[diag[a]] - eq[diag[a]];
+      trace_src += src_mom[diag[a]];
+    }
+    double bulk_neq = trace_neq * inv_d;
+    double bulk_src = trace_src * inv_d;
+    for (int a = 0; a < 3; ++a) {
+      int idx = diag[a];
+      double dev_neq = mom[idx] - eq[idx] - bulk_neq;
+      double dev_src = src_mom[idx] - bulk_src;
+      post[idx] = eq[idx] + (1.0 - os) * dev_neq + 0.5 * bulk_src + (1.0 - 0.5 * os) * dev_src;
+    }
+    double raw_post[27];
+    shifted_moments(tr, u, post, raw_post, false);
+    for (int q = 0; q < Q; ++q) {
+      double o = 0.0;
+      for (int m = 0; m < Q; ++m) o += tr.pop_from_raw[q][m] * raw_post[m];
+      f[q * np + i] = (T)(o - L.w[q]);
+    }
+  }
+
+  void collide() {
+#pragma omp parallel for schedule(static)
+    for (long r = 0; r < (long)ny * nz; ++r) {
+      long y = r % ny, z = r / ny;
+      for (long x = 0; x < nx; ++x) {
+        if (solid[pidx(x, y, z)]) continue;
+        if (collision == 2)
+          collide_cm_cell(x, y, z);
+        else
+          collide_trt_cell(x, y, z);
+      }
+    }
+  }
+
+  // stream_row, kernels.rs:547-622 (no probe: the momentum-exchange probe
+  // is not on this path).
+  void stream() {
+    const T six = T(6.0);
+    const int Q = L.Q;
+#pragma omp parallel for schedule(static)
+    for (long r = 0; r < (long)ny * nz; ++r) {
+      long y = r % ny, z = r / ny;
+      for (long x = 0; x < nx; ++x) {
+        size_t i = pidx(x, y, z);
+        if (solid[i]) continue;
+        T rho_x = rho[cidx(x, y, z)];
+        for (int q = 0; q < Q; ++q) {
+          long s[3] = {x - L.c[q][0], y - L.c[q][1], z - L.c[q][2]};
+          long n[3] = {nx, ny, nz};
+          bool skip = false;
+          for (int a = 0; a < 3; ++a) {
+            if (s[a] < 0) {
+              if (!periodic[a]) skip = true;
+            } else if (s[a] >= n[a] && !periodic[a]) {
+              skip = true;
+            }
+          }
+          if (skip) continue;
+          size_t si = pidx(s[0], s[1], s[2]);
+          if (solid[si]) {
+            T fout = f[L.opp[q] * np + i];
+            const T* wu = &wall_u[3 * si];
+            T cu = cr[q][0] * wu[0];
+            for (int d = 1; d < 3; ++d) cu = cu + cr[q][d] * wu[d];
+            T fin = fout + six * wr[q] * rho_x * cu;
+            ftmp[q * np + i] = fin;
+          } else {
+            ftmp[q * np + i] = f[q * np + si];
+          }
+        }
+      }
+    }
+  }
+
+  // moments_row via update_moments_impl, kernels.rs:627-668, backend.rs:858-916
+  void update_moments() {
+    const T half = T(0.5);
+    const int Q = L.Q;
+#pragma omp parallel for schedule(static)
+    for (long r = 0; r < (long)ny * nz; ++r) {
+      long y = r % ny, z = r / ny;
+      for (long x = 0; x < nx; ++x) {
+        size_t i = pidx(x, y, z), c = cidx(x, y, z);
+        if (solid[i]) continue;
+        T dr = T(0);
+        T m[3] = {T(0), T(0), T(0)};
+        for (int q = 0; q < Q; ++q) {
+          T fq = f[q * np + i];
+          dr = dr + fq;
+          m[0] = m[0] + cr[q][0] * fq;
+          m[1] = m[1] + cr[q][1] * fq;
+          m[2] = m[2] + cr[q][2] * fq;
+        }
+        T rr = T(1) + dr;
+        T fv[3];
+        force_at(c, fv);
+        rho[c] = rr;
+        T inv = T(1) / rr;
+        ux[c] = (m[0] + half * fv[0]) * inv;
+        uy[c] = (m[1] + half * fv[1]) * inv;
+        uz[c] = (m[2] + half * fv[2]) * inv;
+      }
+    }
+  }
+
+  // Backend::run_span default body, backend.rs:266-328 (single part,
+  // single pass): collide -> exchange_f -> stream -> swap -> moments.
+  void hydro_step() {
+    collide();
+    wrap_halo(f, L.Q, 1);
+    stream();
+    std::swap(f, ftmp);
+    update_moments();
+  }
+
+  // Solver::step, solver.rs:2237-2280 (hydro, then scalars).
+  void step() {
+    hydro_step();
+    time += 1;
+    step_scalars();
+  }
+
+  // reduce_impl, backend.rs:920-970
+  double reduce(int kind, int axis) const {
+    double acc = 0.0;
+    for (long z = 0; z < nz; ++z)
+      for (long y = 0; y < ny; ++y)
+        for (long x = 0; x < nx; ++x) {
+          size_t i = pidx(x, y, z), c = cidx(x, y, z);
+          if (solid[i]) continue;
+          if (kind == 0) {
+            acc += 1.0;
+          } else if (kind == 1) {
+            for (int q = 0; q < L.Q; ++q) acc += (double)f[q * np + i];
+          } else {
+            double m = 0.0;
+            for (int q = 0; q < L.Q; ++q) m += (double)L.c[q][axis] * (double)f[q * np + i];
+            double fa = has_ff ? (double)force[axis] + ((double)ff[3 * c + axis] + 0.0) : (double)force[axis] + 0.0;
+            acc += m + 0.5 * fa;
+          }
+        }
+    return acc;
+  }
+
+  bool ff_uniform() const {  // solver.rs:2058-2073
+    if (!has_ff || nc == 0) return false;
+    for (size_t c = 1; c < nc; ++c)
+      for (int a = 0; a < 3; ++a)
+        if (ff[3 * c + a] != ff[a]) return false;
+    return true;
+  }
+  // set_body_force_field, solver.rs:2776-2801
+  void set_force_field(const double* v) {
+    ff.assign(nc * 3, T(0));
+    for (size_t k = 0; k < nc * 3; ++k) ff[k] = (T)v[k];
+    has_ff = true;
+    if (ff_uniform()) update_moments();
+  }
+  // clear_body_force_field, solver.rs:2805-2822
+  void clear_force_field() {
+    if (!has_ff) return;
+    bool refresh = ff_uniform();
+    has_ff = false;
+    ff.clear();
+    if (refresh) update_moments();
+  }
+
+  // equilibrium(), kernels.rs:103-121
+  void equilibrium(T r, const T u[3], T* feq) const {
+    const T three = T(3.0), f45 = T(4.5), f15 = T(1.5);
+    T usq = u[0] * u[0];
+    for (int d = 1; d < 3; ++d) usq = usq + u[d] * u[d];
+    T drho = r - T(1);
+    for (int q = 0; q < L.Q; ++q) {
+      T cu = cr[q][0] * u[0];
+      for (int d = 1; d < 3; ++d) cu = cu + cr[q][d] * u[d];
+      feq[q] = wr[q] * (drho + r * (three * cu + f45 * cu * cu - f15 * usq));
+    }
+  }
+
+  // Solver::init_with, solver.rs:2474-2623: f = feq + f_neq(grad u), then
+  // update_moments.  rho_in/u_in are compact global arrays.
+  void init_with(const double* rho_in, const double* u_in) {
+    const T tau = (T)(3.0 * nu + 0.5);
+    const T three = T(3.0), half = T(0.5);
+    for (size_t c = 0; c < nc; ++c) {
+      rho[c] = (T)rho_in[c];
+      ux[c] = (T)u_in[3 * c + 0];
+      uy[c] = (T)u_in[3 * c + 1];
+      uz[c] = (T)u_in[3 * c + 2];
+    }
+    long dims[3] = {nx, ny, nz};
+    for (long z = 0; z < nz; ++z)
+      for (long y = 0; y < ny; ++y)
+        for (long x = 0; x < nx; ++x) {
+          size_t c = cidx(x, y, z), pi = pidx(x, y, z);
+          T uu[3] = {ux[c], uy[c], uz[c]};
+          T feq[27];
+          equilibrium(rho[c], uu, feq);
+          for (int q = 0; q < L.Q; ++q) f[q * np + pi] = feq[q];
+          if (solid[pi]) continue;
+          auto sample = [&](const long da[3], T out[3]) -> bool {
+            long g[3] = {x + da[0], y + da[1], z + da[2]};
+            for (int a = 0; a < 3; ++a)
+              if (g[a] < 0 || g[a] >= dims[a]) {
+                if (periodic[a])
+                  g[a] = (g[a] + dims[a]) % dims[a];
+                else
+                  return false;
+              }
+            if (solid[pidx(x + da[0], y + da[1], z + da[2])]) return false;
+            size_t gc = cidx(g[0], g[1], g[2]);
+            out[0] = (T)u_in[3 * gc + 0];
+            out[1] = (T)u_in[3 * gc + 1];
+            out[2] = (T)u_in[3 * gc + 2];
+            return true;
+          };
+          T grad[3][3];
+          for (int a = 0; a < 3; ++a) {
+            long dp[3] = {0, 0, 0}, dm[3] = {0, 0, 0};
+            dp[a] = 1;
+            dm[a] = -1;
+            T pv[3], mv[3];
+            bool hp = sample(dp, pv), hm = sample(dm, mv);
+            for (int b = 0; b < 3; ++b) {
+              if (hp && hm)
+                grad[a][b] = (pv[b] - mv[b]) * half;
+              else if (hp)
+                grad[a][b] = pv[b] - uu[b];
+              else if (hm)
+                grad[a][b] = uu[b] - mv[b];
+              else
+                grad[a][b] = T(0);
+            }
+          }
+          T div = grad[0][0];
+          for (int a = 1; a < 3; ++a) div = div + grad[a][a];
+          for (int q = 0; q < L.Q; ++q) {
+            const T* cq = cr[q];
+            T ccgu = cq[0] * cq[0] * grad[0][0];
+            for (int a = 0; a < 3; ++a)
+              for (int b = a; b < 3; ++b) {
+                if (a == 0 && b == 0) continue;
+                if (a == b)
+                  ccgu = ccgu + cq[a] * cq[a] * grad[a][a];
+                else
+                  ccgu = ccgu + cq[a] * cq[b] * (grad[a][b] + grad[b][a]);
+              }
+            T fneq = -wr[q] * rho[c] * tau * (three * ccgu - div);
+            f[q * np + pi] = f[q * np + pi] + fneq;
+          }
+        }
+    update_moments();
+  }
+
+  // ---- rotating IBM, solver.rs:3016-3282 -----------------------------------
+  struct SP {
+    long x, y, z;
+    double w;
+  };
+  // marker_stencil, rotating_ibm.rs:163-224
+  static void marker_stencil(const double p[3], const int dims[3], int radius, std::vector<SP>& out) {
+    std::vector<std::pair<long, double>> axes[3];
+    for (int a = 0; a < 3; ++a) {
+      long lo = (long)std::floor(p[a]) - radius + 1;
+      long hi = (long)std::floor(p[a]) + radius;
+      for (long i = lo; i <= hi; ++i) {
+        if (i < 0 || i >= dims[a]) continue;
+        double rr = std::fabs(p[a] - (double)i);
+        double w;
+        if (radius == 1) {
+          w = std::max(1.0 - rr, 0.0);
+        } else {
+          if (rr < 0.5)
+            w = 0.75 - rr * rr;
+          else if (rr < 1.5)
+            w = 0.5 * (1.5 - rr) * (1.5 - rr);
+          else
+            w = 0.0;
+        }
+        if (w != 0.0) axes[a].push_back({i, w});
+      }
+    }
+    out.clear();
+    for (auto& ax : axes[0])
+      for (auto& ay : axes[1])
+        for (auto& az : axes[2]) out.push_back({ax.first, ay.first, az.first, ax.second * ay.second * az.second});
+  }
+
+  static double norm3(const double v[3]) { return std::sqrt(0.0 + v[0] * v[0] + v[1] * v[1] + v[2] * v[2]); }
+
+  // diag layout: iterations, slip_max, slip_rms, slip_max_rel, slip_rms_rel,
+  // torque[3], fluid_force[3], marker_force[3], momentum_error_rel
+  int apply_ibm(int nm, const double* pos, const double* weight, const double center[3], const double omega[3],
+                int max_iter, double tol, int radius, double relax, double* diag_out) {
+    int dims[3] = {nx, ny, nz};
+    size_t n = nc;
+    std::vector<double> u_now(3 * n, 0.0);
+    std::vector<uint8_t> sol(n, 0);
+    for (long z = 0; z < nz; ++z)
+      for (long y = 0; y < ny; ++y)
+        for (long x = 0; x < nx; ++x) {
+          size_t gi = cidx(x, y, z), pi = pidx(x, y, z);
+          sol[gi] = solid[pi];
+          if (sol[gi]) continue;
+          double mom[3] = {0.0, 0.0, 0.0};
+          for (int q = 0; q < L.Q; ++q) {
+            double fq = (double)f[q * np + pi];
+            for (int a = 0; a < 3; ++a) mom[a] += (double)L.c[q][a] * fq;
+          }
+          double frc[3] = {(double)force[0], (double)force[1], (double)force[2]};
+          if (has_ff)
+            for (int a = 0; a < 3; ++a) frc[a] += (double)ff[3 * gi + a];
+          double inv_rho = 1.0 / std::max((double)rho[gi], 1.0e-30);
+          for (int a = 0; a < 3; ++a) u_now[3 * gi + a] = (mom[a] + 0.5 * frc[a]) * inv_rho;
+        }
+    std::vector<double> du(3 * n, 0.0), spread(3 * n, 0.0);
+    double iterations = 0, slip_max = 0, slip_rms = 0, slip_max_rel = 0, slip_rms_rel = 0;
+    double torque[3] = {0, 0, 0}, fluid_force[3] = {0, 0, 0}, marker_force[3] = {0, 0, 0};
+    double mom_err = 0;
+    auto target = [&](const double* p, double out[3]) {  // rotating_ibm.rs:89-101
+      double r[3] = {p[0] - center[0], p[1] - center[1], p[2] - center[2]};
+      out[0] = omega[1] * r[2] - omega[2] * r[1];
+      out[1] = omega[2] * r[0] - omega[0] * r[2];
+      out[2] = omega[0] * r[1] - omega[1] * r[0];
+    };
+    double max_target = 0.0;
+    for (int m = 0; m < nm; ++m) {
+      double t[3];
+      target(&pos[3 * m], t);
+      max_target = std::max(max_target, norm3(t));
+    }
+    double scale = std::max(max_target, 1.0);
+    std::vector<std::vector<SP>> st(nm);
+    for (int m = 0; m < nm; ++m) {
+      marker_stencil(&pos[3 * m], dims, radius, st[m]);
+      if (st[m].empty()) return -1;
+    }
+    std::vector<double> ksum(n, 0.0);
+    for (int m = 0; m < nm; ++m)
+      for (auto& sp : st[m]) {
+        size_t gi = cidx(sp.x, sp.y, sp.z);
+        if (sol[gi]) continue;
+        ksum[gi] += sp.w;
+      }
+    std::vector<double> imp(3 * (size_t)nm);
+    for (int iter = 0; iter < max_iter; ++iter) {
+      std::fill(imp.begin(), imp.end(), 0.0);
+      for (int m = 0; m < nm; ++m) {
+        double t[3];
+        target(&pos[3 * m], t);
+        double um[3] = {0, 0, 0}, mob = 0.0;
+        for (auto& sp : st[m]) {
+          size_t gi = cidx(sp.x, sp.y, sp.z);
+          if (sol[gi]) continue;
+          double rc = (double)rho[gi];
+          for (int a = 0; a < 3; ++a) um[a] += sp.w * (u_now[3 * gi + a] + du[3 * gi + a]);
+          mob += sp.w * ksum[gi] / std::max(rc, 1.0e-30);
+        }
+        if (mob == 0.0) continue;
+        for (int a = 0; a < 3; ++a) imp[3 * m + a] = relax * (t[a] - um[a]) / mob;
+      }
+      for (int m = 0; m < nm; ++m) {
+        double rep[3] = {0, 0, 0};
+        for (auto& sp : st[m]) {
+          size_t gi = cidx(sp.x, sp.y, sp.z);
+          if (sol[gi]) continue;
+          double cf[3] = {imp[3 * m + 0] * sp.w, imp[3 * m + 1] * sp.w, imp[3 * m + 2] * sp.w};
+          for (int a = 0; a < 3; ++a) {
+            spread[3 * gi + a] += cf[a];
+            fluid_force[a] += cf[a];
+            rep[a] += cf[a];
+          }
+          double inv_rho = 1.0 / std::max((double)rho[gi], 1.0e-30);
+          for (int a = 0; a < 3; ++a) du[3 * gi + a] += cf[a] * inv_rho;
+        }
+        for (int a = 0; a < 3; ++a) marker_force[a] += rep[a];
+        const double* p = &pos[3 * m];
+        double r[3] = {p[0] - center[0], p[1] - center[1], p[2] - center[2]};
+        double tq[3] = {r[1] * rep[2] - r[2] * rep[1], r[2] * rep[0] - r[0] * rep[2], r[0] * rep[1] - r[1] * rep[0]};
+        for (int a = 0; a < 3; ++a) torque[a] -= tq[a];
+      }
+      iterations = iter + 1;
+      slip_max = 0.0;
+      double ssq = 0.0, wsum = 0.0;
+      for (int m = 0; m < nm; ++m) {
+        double t[3];
+        target(&pos[3 * m], t);
+        double um[3] = {0, 0, 0};
+        bool active = false;
+        for (auto& sp : st[m]) {
+          size_t gi = cidx(sp.x, sp.y, sp.z);
+          if (sol[gi]) continue;
+          active = true;
+          for (int a = 0; a < 3; ++a) um[a] += sp.w * (u_now[3 * gi + a] + du[3 * gi + a]);
+        }
+        if (!active) continue;
+        double s[3] = {t[0] - um[0], t[1] - um[1], t[2] - um[2]};
+        double sm = norm3(s);
+        slip_max = std::max(slip_max, sm);
+        ssq += weight[m] * sm * sm;
+        wsum += weight[m];
+      }
+      slip_rms = wsum > 0.0 ? std::sqrt(ssq / wsum) : 0.0;
+      slip_max_rel = slip_max / scale;
+      slip_rms_rel = slip_rms / scale;
+      double err[3] = {fluid_force[0] - marker_force[0], fluid_force[1] - marker_force[1],
+                       fluid_force[2] - marker_force[2]};
+      mom_err = norm3(err) / std::max(norm3(marker_force), 1.0e-12);
+      if (slip_max_rel <= tol) break;
+    }
+    // force_field += T(spread)
+    if (!has_ff) {
+      ff.assign(nc * 3, T(0));
+      has_ff = true;
+    }
+    for (size_t c = 0; c < nc; ++c)
+      for (int a = 0; a < 3; ++a) ff[3 * c + a] = ff[3 * c + a] + (T)spread[3 * c + a];
+    double d[15] = {iterations,    slip_max,      slip_rms,        slip_max_rel,    slip_rms_rel,
+                    torque[0],     torque[1],     torque[2],       fluid_force[0],  fluid_force[1],
+                    fluid_force[2], marker_force[0], marker_force[1], marker_force[2], mom_err};
+    std::memcpy(diag_out, d, sizeof(d));
+    return 0;
+  }
+
+  // ---- passive scalar ADE, solver.rs:2113-2205 / scalar.rs:6-15 --------------
+  void scalar_eq(const ScalarDist<T>& s, T conc, const T u[3], T* out) const {
+    if (!s.d3q7) {
+      const T three = T(3.0);
+      for (int q = 0; q < s.L.Q; ++q) {
+        T cu = (T)(double)s.L.c[q][0] * u[0] + (T)(double)s.L.c[q][1] * u[1] + (T)(double)s.L.c[q][2] * u[2];
+        out[q] = (T)s.L.w[q] * conc * (T(1) + three * cu);
+      }
+    } else {
+      // D3Q7: h_eq = w C (1 + c.u / cs^2), cs^2 = 1/4
+      const T four = T(4.0);
+      for (int q = 0; q < 7; ++q) {
+        T cu = (T)(double)s.L.c[q][0] * u[0] + (T)(double)s.L.c[q][1] * u[1] + (T)(double)s.L.c[q][2] * u[2];
+        out[q] = (T)s.L.w[q] * conc * (T(1) + four * cu);
+      }
+    }
+  }
+  int scalar_enable(double D, bool d3q7) {
+    ScalarDist<T> s;
+    s.diffusivity = D;
+    s.d3q7 = d3q7;
+    s.L = make_lat(d3q7 ? 7 : L.Q);
+    s.h.assign((size_t)s.L.Q * np, T(0));
+    s.htmp.assign((size_t)s.L.Q * np, T(0));
+    s.conc.assign(nc, T(0));
+    scalars.push_back(s);
+    return (int)scalars.size() - 1;
+  }
+  // set_scalar_concentration, solver.rs:2910-2965
+  void scalar_set(int k, const double* vals) {
+    ScalarDist<T>& s = scalars[k];
+    for (long z = 0; z < nz; ++z)
+      for (long y = 0; y < ny; ++y)
+        for (long x = 0; x < nx; ++x) {
+          size_t c = cidx(x, y, z), pi = pidx(x, y, z);
+          s.conc[c] = (T)vals[c];
+          T u[3] = {ux[c], uy[c], uz[c]};
+          T heq[27];
+          scalar_eq(s, (T)vals[c], u, heq);
+          for (int q = 0; q < s.L.Q; ++q) {
+            s.h[q * np + pi] = heq[q];
+            s.htmp[q * np + pi] = T(0);
+          }
+        }
+  }
+  void step_scalars() {
+    for (auto& s : scalars) {
+      const int Q = s.L.Q;
+      T omega = s.d3q7 ? (T)(1.0 / (4.0 * s.diffusivity + 0.5)) : (T)(1.0 / (3.0 * s.diffusivity + 0.5));
+#pragma omp parallel for schedule(static)
+      for (long r = 0; r < (long)ny * nz; ++r) {
+        long y = r % ny, z = r / ny;
+        for (long x = 0; x < nx; ++x) {
+          size_t pi = pidx(x, y, z), c = cidx(x, y, z);
+          if (solid[pi]) continue;
+          T u[3] = {ux[c], uy[c], uz[c]};
+          T heq[27];
+          scalar_eq(s, s.conc[c], u, heq);
+          for (int q = 0; q < Q; ++q) {
+            size_t k = q * np + pi;
+            s.h[k] = s.h[k] - omega * (s.h[k] - heq[q]);
+          }
+        }
+      }
+      wrap_halo(s.h, Q, 1);
+      std::fill(s.htmp.begin(), s.htmp.end(), T(0));
+#pragma omp parallel for schedule(static)
+      for (long r = 0; r < (long)ny * nz; ++r) {
+        long y = r % ny, z = r / ny;
+        for (long x = 0; x < nx; ++x) {
+          size_t pi = pidx(x, y, z);
+          if (solid[pi]) {
+            s.conc[cidx(x, y, z)] = T(0);
+            continue;
+          }
+          for (int q = 0; q < Q; ++q) {
+            size_t src = pidx(x - s.L.c[q][0], y - s.L.c[q][1], z - s.L.c[q][2]);
+            T v = solid[src] ? s.h[s.L.opp[q] * np + pi] : s.h[q * np + src];
+            s.htmp[q * np + pi] = v;
+          }
+        }
+      }
+      std::swap(s.h, s.htmp);
+#pragma omp parallel for schedule(static)
+      for (long r = 0; r < (long)ny * nz; ++r) {
+        long y = r % ny, z = r / ny;
+        for (long x = 0; x < nx; ++x) {
+          size_t pi = pidx(x, y, z), c = cidx(x, y, z);
+          if (solid[pi]) {
+            s.conc[c] = T(0);
+            continue;
+          }
+          T sum = T(0);
+          for (int q = 0; q < Q; ++q) sum = sum + s.h[q * np + pi];
+          s.conc[c] = sum;
+        }
+      }
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Geometry — solver.rs:1357-1406 (wall rims); geometry.rs:657-723 (tank and
+// baffles), 815-920 (radial impeller, wall-velocity stamp), 922-947 helpers.
+// ---------------------------------------------------------------------------
+static void build_wall_rims(const int dims[3], const int is_wall[6], const double u[6][3], uint8_t* solid,
+                            double* wall_u) {
+  size_t n = (size_t)dims[0] * dims[1] * dims[2];
+  std::vector<double> best(n, -1.0);
+  for (size_t i = 0; i < n; ++i) {
+    solid[i] = 0;
+    wall_u[3 * i] = wall_u[3 * i + 1] = wall_u[3 * i + 2] = 0.0;
+  }
+  // Face index order XNeg=0, XPos=1, YNeg=2, YPos=3, ZNeg=4, ZPos=5;
+  // application order YNeg, YPos, XNeg, XPos, ZNeg, ZPos.
+  const int order[6] = {2, 3, 0, 1, 4, 5};
+  for (int k = 0; k < 6; ++k) {
+    int face = order[k];
+    int a = face / 2;
+    if (!is_wall[face]) continue;
+    const double* uu = u[face];
+    double speed = uu[0] * uu[0] + uu[1] * uu[1];
+    speed += uu[2] * uu[2];
+    long fixed = (face % 2 == 0) ? 0 : dims[a] - 1;
+    int t1 = a == 0 ? 1 : 0, t2 = a == 2 ? 1 : 2;
+    for (long c2 = 0; c2 < dims[t2]; ++c2)
+      for (long c1 = 0; c1 < dims[t1]; ++c1) {
+        long pos[3];
+        pos[a] = fixed;
+        pos[t1] = c1;
+        pos[t2] = c2;
+        size_t i = ((size_t)pos[2] * dims[1] + pos[1]) * dims[0] + pos[0];
+        solid[i] = 1;
+        if (speed > best[i]) {
+          best[i] = speed;
+          for (int d = 0; d < 3; ++d) wall_u[3 * i + d] = uu[d];
+        }
+      }
+  }
+}
+
+static void tank_mask(const int dims[3], double dx, double diameter, double height, uint8_t* solid) {
+  double radius = 0.5 * diameter;
+  double cx = 0.5 * (double)dims[0] * dx, cy = 0.5 * (double)dims[1] * dx;
+  for (long z = 0; z < dims[2]; ++z) {
+    double z_m = ((double)z + 0.5) * dx;
+    for (long y = 0; y < dims[1]; ++y)
+      for (long x = 0; x < dims[0]; ++x) {
+        double pxm = ((double)x + 0.5) * dx, pym = ((double)y + 0.5) * dx;
+        double r = std::sqrt((pxm - cx) * (pxm - cx) + (pym - cy) * (pym - cy));
+        bool rim = x == 0 || y == 0 || z == 0 || x + 1 == dims[0] || y + 1 == dims[1] || z + 1 == dims[2];
+        bool outside = r >= radius || z_m >= height;
+        if (rim || outside) solid[((size_t)z * dims[1] + y) * dims[0] + x] = 1;
+      }
+  }
+}
+
+static void baffle_mask(const int dims[3], double dx, double diameter, double height, double width,
+                        double thickness, uint8_t* solid, uint8_t* baffle) {
+  double radius = 0.5 * diameter;
+  double cx = 0.5 * (double)dims[0] * dx, cy = 0.5 * (double)dims[1] * dx;
+  double radial_inner = radius - width;
+  for (long z = 1; z < dims[2] - 1; ++z) {
+    double z_m = ((double)z + 0.5) * dx;
+    if (z_m >= height) continue;
+    for (long y = 1; y < dims[1] - 1; ++y)
+      for (long x = 1; x < dims[0] - 1; ++x) {
+        double pxm = ((double)x + 0.5) * dx, pym = ((double)y + 0.5) * dx;
+        double rx = pxm - cx, ry = pym - cy;
+        double r = std::sqrt(rx * rx + ry * ry);
+        if (r >= radius || r < radial_inner) continue;
+        bool nxw = std::fabs(ry) <= 0.5 * thickness;
+        bool nyw = std::fabs(rx) <= 0.5 * thickness;
+        if (nxw || nyw) {
+          size_t i = ((size_t)z * dims[1] + y) * dims[0] + x;
+          solid[i] = 1;
+          if (baffle) baffle[i] = 1;
+        }
+      }
+  }
+}
+
+// build_radial_impeller (blade angle 0 => Rushton), geometry.rs:815-906.
+// Returns the marker count; fills pos (lattice units), weight, and the
+// target velocity used by the wall-velocity stamp.
+static int radial_impeller(const int dims[3], double dx, double vessel_diameter, double liquid_height,
+                           double diameter, double clearance, int blades, double blade_angle_deg, double omega_z,
+                           double thickness_fraction, int inner_kind, std::vector<double>& pos,
+                           std::vector<double>& weight, std::vector<double>& vel, double center_out[3]) {
+  const double TAU = 6.283185307179586;
+  double cxy[2] = {0.5 * (double)dims[0] * dx, 0.5 * (double)dims[1] * dx};
+  double radius = 0.5 * diameter;
+  double center[3] = {cxy[0], cxy[1], clearance};
+  double radial_inner = inner_kind == 0 ? 0.65 * radius : (inner_kind == 1 ? 0.35 * radius : 0.0);
+  double dr = std::max(dx, radius / 20.0);
+  double dz = dx;
+  double tstep = dx;
+  double thickness = std::max(thickness_fraction * diameter, dx);
+  pos.clear();
+  weight.clear();
+  vel.clear();
+  for (int b = 0; b < blades; ++b) {
+    double th = TAU * (double)b / (double)blades;
+    double er[3] = {std::cos(th), std::sin(th), 0.0};
+    double et[3] = {-std::sin(th), std::cos(th), 0.0};
+    double pitch = blade_angle_deg * (3.141592653589793 / 180.0);
+    double z_span = blade_angle_deg == 0.0 ? thickness : thickness + (radius - radial_inner) * std::sin(pitch) * 0.18;
+    long nr = (long)std::max(std::ceil((radius - radial_inner) / dr), 1.0);
+    long nt = (long)std::max(std::ceil(thickness / tstep), 1.0);
+    long nzs = (long)std::max(std::ceil(z_span / dz), 1.0);
+    for (long ir = 0; ir <= nr; ++ir) {
+      double rr = radial_inner + (radius - radial_inner) * (double)ir / (double)nr;
+      for (long it = 0; it <= nt; ++it) {
+        double offset_t = -0.5 * thickness + thickness * (double)it / (double)nt;
+        for (long iz = 0; iz <= nzs; ++iz) {
+          double offset_z = -0.5 * z_span + z_span * (double)iz / (double)nzs;
+          double pitched_z = offset_z + (rr - radial_inner) * std::sin(pitch) * 0.18;
+          double p[3] = {center[0] + rr * er[0] + offset_t * et[0], center[1] + rr * er[1] + offset_t * et[1],
+                         center[2] + pitched_z};
+          double rel_r = std::sqrt((p[0] - cxy[0]) * (p[0] - cxy[0]) + (p[1] - cxy[1]) * (p[1] - cxy[1]));
+          if (rel_r >= 0.5 * vessel_diameter || p[2] <= 0.0 || p[2] >= liquid_height) continue;
+          double r[3] = {p[0] - center[0], p[1] - center[1], p[2] - center[2]};
+          double om[3] = {0.0, 0.0, omega_z};
+          vel.push_back(om[1] * r[2] - om[2] * r[1]);
+          vel.push_back(om[2] * r[0] - om[0] * r[2]);
+          vel.push_back(om[0] * r[1] - om[1] * r[0]);
+          pos.push_back(p[0] / dx);
+          pos.push_back(p[1] / dx);
+          pos.push_back(p[2] / dx);
+          weight.push_back(dx * dx);
+        }
+      }
+    }
+  }
+  center_out[0] = center[0] / dx;
+  center_out[1] = center[1] / dx;
+  center_out[2] = center[2] / dx;
+  return (int)weight.size();
+}
+
+}  // namespace orc
+
+using namespace orc;
+
+// ---------------------------------------------------------------------------
+// C ABI for the Python checker (ctypes).  All arrays are compact global
+// order (z, y, x) with q-major planes for populations.
+// ---------------------------------------------------------------------------
+struct Handle {
+  int prec;
+  Solver<float>* s32;
+  Solver<double>* s64;
+};
+
+#define DISPATCH(h, ...)         \
+  do {                           \
+    if ((h)->prec == 4) {        \
+      auto* S = (h)->s32;        \
+      __VA_ARGS__;               \
+    } else {                     \
+      auto* S = (h)->s64;        \
+      __VA_ARGS__;               \
+    }                            \
+  } while (0)
+
+extern "C" {
+
+int orc_lattice(int Q, int8_t* c_out, double* w_out, int* opp_out) {
+  if (Q != 7 && Q != 19 && Q != 27) return -1;
+  Lat L = make_lat(Q);
+  for (int q = 0; q < Q; ++q) {
+    for (int a = 0; a < 3; ++a) c_out[3 * q + a] = L.c[q][a];
+    w_out[q] = L.w[q];
+    opp_out[q] = L.opp[q];
+  }
+  return L.npairs;
+}
+
+void orc_omegas(int collision, double nu, double magic, double* op, double* om) {
+  double tau = 3.0 * nu + 0.5;
+  *op = 1.0 / tau;
+  if (collision == 1) {
+    double lam_p = tau - 0.5;
+    *om = 1.0 / (magic / lam_p + 0.5);
+  } else {
+    *om = *op;
+  }
+}
+
+void orc_cm_transform(int Q, double* raw_from_pop, double* pop_from_raw, uint8_t* basis) {
+  const MomentTransform& t = transform_for(Q);
+  for (int m = 0; m < Q; ++m) {
+    for (int q = 0; q < Q; ++q) {
+      raw_from_pop[m * Q + q] = t.raw_from_pop[m][q];
+      pop_from_raw[m * Q + q] = t.pop_from_raw[m][q];
+    }
+    for (int a = 0; a < 3; ++a) basis[3 * m + a] = t.basis[m][a];
+  }
+}
+
+void* orc_new(int prec, int Q, const int dims[3], const int periodic[3], int collision, double nu, double magic,
+              double omega_shear, const double force[3], const uint8_t* solid, const double* wall_u) {
+  Handle* h = new Handle{prec, nullptr, nullptr};
+  if (prec == 4) {
+    h->s32 = new Solver<float>();
+    h->s32->init(Q, dims, periodic, collision, nu, magic, omega_shear, force);
+    h->s32->set_masks(solid, wall_u);
+  } else {
+    h->s64 = new Solver<double>();
+    h->s64->init(Q, dims, periodic, collision, nu, magic, omega_shear, force);
+    h->s64->set_masks(solid, wall_u);
+  }
+  return h;
+}
+
+void orc_free(void* p) {
+  Handle* h = (Handle*)p;
+  delete h->s32;
+  delete h->s64;
+  delete h;
+}
+
+void orc_step(void* p, int n) {
+  Handle* h = (Handle*)p;
+  DISPATCH(h, for (int k = 0; k < n; ++k) S->step());
+}
+
+void orc_collide_only(void* p) {
+  Handle* h = (Handle*)p;
+  DISPATCH(h, S->collide());
+}
+
+void orc_update_moments(void* p) {
+  Handle* h = (Handle*)p;
+  DISPATCH(h, S->update_moments());
+}
+
+// Populations, q-major compact core planes (deviation form).
+void orc_get_f(void* p, double* out) {
+  Handle* h = (Handle*)p;
+  DISPATCH(h, {
+    for (int q = 0; q < S->L.Q; ++q)
+      for (long z = 0; z < S->nz; ++z)
+        for (long y = 0; y < S->ny; ++y)
+          for (long x = 0; x < S->nx; ++x)
+            out[q * S->nc + S->cidx(x, y, z)] = (double)S->f[q * S->np + S->pidx(x, y, z)];
+  });
+}
+
+void orc_set_f(void* p, const double* in) {
+  Handle* h = (Handle*)p;
+  DISPATCH(h, {
+    using TT = std::remove_reference_t<decltype(S->f[0])>;
+    for (int q = 0; q < S->L.Q; ++q)
+      for (long z = 0; z < S->nz; ++z)
+        for (long y = 0; y < S->ny; ++y)
+          for (long x = 0; x < S->nx; ++x)
+            S->f[q * S->np + S->pidx(x, y, z)] = (TT)in[q * S->nc + S->cidx(x, y, z)];
+  });
+}
+
+void orc_get_moments(void* p, double* rho, double* ux, double* uy, double* uz) {
+  Handle* h = (Handle*)p;
+  DISPATCH(h, {
+    for (size_t c = 0; c < S->nc; ++c) {
+      rho[c] = (double)S->rho[c];
+      ux[c] = (double)S->ux[c];
+      uy[c] = (double)S->uy[c];
+      uz[c] = (double)S->uz[c];
+    }
+  });
+}
+
+void orc_set_moments(void* p, const double* rho, const double* ux, const double* uy, const double* uz) {
+  Handle* h = (Handle*)p;
+  DISPATCH(h, {
+    using TT = std::remove_reference_t<decltype(S->f[0])>;
+    for (size_t c = 0; c < S->nc; ++c) {
+      S->rho[c] = (TT)rho[c];
+      S->ux[c] = (TT)ux[c];
+      S->uy[c] = (TT)uy[c];
+      S->uz[c] = (TT)uz[c];
+    }
+  });
+}
+
+double orc_reduce(void* p, int kind, int axis) {
+  Handle* h = (Handle*)p;
+  double r = 0;
+  DISPATCH(h, r = S->reduce(kind, axis));
+  return r;
+}
+
+void orc_init_with(void* p, const double* rho, const double* u) {
+  Handle* h = (Handle*)p;
+  DISPATCH(h, S->init_with(rho, u));
+}
+
+// equilibrium() for one (rho, u); out has Q entries.
+void orc_equilibrium(void* p, double r, const double u[3], double* out) {
+  Handle* h = (Handle*)p;
+  DISPATCH(h, {
+    using TT = std::remove_reference_t<decltype(S->f[0])>;
+    TT uu[3] = {(TT)u[0], (TT)u[1], (TT)u[2]};
+    TT feq[27];
+    S->equilibrium((TT)r, uu, feq);
+    for (int q = 0; q < S->L.Q; ++q) out[q] = (double)feq[q];
+  });
+}
+
+void orc_set_force_field(void* p, const double* ff) {
+  Handle* h = (Handle*)p;
+  DISPATCH(h, S->set_force_field(ff));
+}
+
+void orc_clear_force_field(void* p) {
+  Handle* h = (Handle*)p;
+  DISPATCH(h, S->clear_force_field());
+}
+
+int orc_get_force_field(void* p, double* out) {
+  Handle* h = (Handle*)p;
+  int has = 0;
+  DISPATCH(h, {
+    has = S->has_ff ? 1 : 0;
+    for (size_t k = 0; k < S->nc * 3; ++k) out[k] = S->has_ff ? (double)S->ff[k] : 0.0;
+  });
+  return has;
+}
+
+void orc_set_omega_field(void* p, const double* om) {
+  Handle* h = (Handle*)p;
+  DISPATCH(h, {
+    using TT = std::remove_reference_t<decltype(S->f[0])>;
+    if (!om) {
+      S->has_omega = false;
+      S->omega_field.clear();
+    } else {
+      S->has_omega = true;
+      S->omega_field.resize(S->nc);
+      for (size_t c = 0; c < S->nc; ++c) S->omega_field[c] = (TT)om[c];
+    }
+  });
+}
+
+int orc_apply_ibm(void* p, int nm, const double* pos, const double* weight, const double center[3],
+                  const double omega[3], int max_iter, double tol, int radius, double relax, double* diag) {
+  Handle* h = (Handle*)p;
+  int r = 0;
+  DISPATCH(h, r = S->apply_ibm(nm, pos, weight, center, omega, max_iter, tol, radius, relax, diag));
+  return r;
+}
+
+int orc_scalar_enable(void* p, double D, int d3q7) {
+  Handle* h = (Handle*)p;
+  int r = 0;
+  DISPATCH(h, r = S->scalar_enable(D, d3q7 != 0));
+  return r;
+}
+
+void orc_scalar_set(void* p, int k, const double* vals) {
+  Handle* h = (Handle*)p;
+  DISPATCH(h, S->scalar_set(k, vals));
+}
+
+void orc_scalar_get(void* p, int k, double* out) {
+  Handle* h = (Handle*)p;
+  DISPATCH(h, {
+    for (size_t c = 0; c < S->nc; ++c) out[c] = (double)S->scalars[k].conc[c];
+  });
+}
+
+void orc_scalar_get_h(void* p, int k, double* out) {
+  Handle* h = (Handle*)p;
+  DISPATCH(h, {
+    auto& s = S->scalars[k];
+    for (int q = 0; q < s.L.Q; ++q)
+      for (long z = 0; z < S->nz; ++z)
+        for (long y = 0; y < S->ny; ++y)
+          for (long x = 0; x < S->nx; ++x)
+            out[q * S->nc + S->cidx(x, y, z)] = (double)s.h[q * S->np + S->pidx(x, y, z)];
+  });
+}
+
+void orc_build_wall_rims(const int dims[3], const int is_wall[6], const double* u6x3, uint8_t* solid,
+                         double* wall_u) {
+  double u[6][3];
+  for (int f = 0; f < 6; ++f)
+    for (int a = 0; a < 3; ++a) u[f][a] = u6x3[3 * f + a];
+  build_wall_rims(dims, is_wall, u, solid, wall_u);
+}
+
+void orc_tank_mask(const int dims[3], double dx, double diameter, double height, uint8_t* solid) {
+  tank_mask(dims, dx, diameter, height, solid);
+}
+
+void orc_baffle_mask(const int dims[3], double dx, double diameter, double height, double width, double thickness,
+                     uint8_t* solid, uint8_t* baffle) {
+  baffle_mask(dims, dx, diameter, height, width, thickness, solid, baffle);
+}
+
+// Returns marker count; if pos == NULL only counts.
+int orc_radial_impeller(const int dims[3], double dx, double vessel_diameter, double liquid_height, double diameter,
+                        double clearance, int blades, double blade_angle_deg, double omega_z,
+                        double thickness_fraction, int inner_kind, double* pos, double* weight, double* vel,
+                        double* center) {
+  std::vector<double> P, W, V;
+  double c[3];
+  int n = radial_impeller(dims, dx, vessel_diameter, liquid_height, diameter, clearance, blades, blade_angle_deg,
+                          omega_z, thickness_fraction, inner_kind, P, W, V, c);
+  if (pos) {
+    std::memcpy(pos, P.data(), P.size() * 8);
+    std::memcpy(weight, W.data(), W.size() * 8);
+    std::memcpy(vel, V.data(), V.size() * 8);
+    std::memcpy(center, c, 24);
+  }
+  return n;
+}
+
+// stamp_impeller_wall_velocity, geometry.rs:908-920
+void orc_stamp_wall_velocity(const int dims[3], int nm, const double* pos, const double* vel, double* wall_u) {
+  for (int m = 0; m < nm; ++m) {
+    long ix[3];
+    for (int a = 0; a < 3; ++a) {
+      double v = std::round(pos[3 * m + a]);
+      v = std::min(std::max(v, 0.0), (double)(dims[a] - 1));
+      ix[a] = (long)v;
+    }
+    size_t i = ((size_t)ix[2] * dims[1] + ix[1]) * dims[0] + ix[0];
+    for (int a = 0; a < 3; ++a) wall_u[3 * i + a] = vel[3 * m + a];
+  }
+}
+
+// marker_stencil for one marker: writes up to 64 (x,y,z) triples and weights.
+int orc_marker_stencil(const double p[3], const int dims[3], int radius, long* cells, double* w) {
+  std::vector<Solver<double>::SP> st;
+  Solver<double>::marker_stencil(p, dims, radius, st);
+  for (size_t k = 0; k < st.size(); ++k) {
+    cells[3 * k + 0] = st[k].x;
+    cells[3 * k + 1] = st[k].y;
+    cells[3 * k + 2] = st[k].z;
+    w[k] = st[k].w;
+  }
+  return (int)st.size();
+}
+
+// ---- QOI reducers, qoi.rs:477-556, 699-739 ----------------------------------
+// power_qois: out = [power, n_hz, np, p_over_v]
+int orc_power_qois(double torque, double omega, double rho, double d, double volume, double* out) {
+  if (!(std::isfinite(omega) && std::isfinite(rho) && std::isfinite(d) && std::isfinite(volume) && rho > 0.0 &&
+        d > 0.0 && volume > 0.0))
+    return -1;
+  double n_hz = omega / 6.283185307179586;
+  if (!(n_hz > 0.0)) return -2;
+  double power = omega * torque;
+  double denom = rho * (n_hz * n_hz * n_hz) * (d * d * d * d * d);
+  out[0] = power;
+  out[1] = n_hz;
+  out[2] = power / denom;
+  out[3] = power / volume;
+  return 0;
+}
+
+// scalar_cv: returns 1 and writes *cv if defined, else 0.
+int orc_scalar_cv(long n, const double* values, const uint8_t* include, double* cv) {
+  long cnt = 0;
+  double sum = 0.0;
+  for (long i = 0; i < n; ++i)
+    if (include[i]) {
+      cnt++;
+      sum += values[i];
+    }
+  if (cnt == 0) return 0;
+  double mean = sum / (double)cnt;
+  if (mean == 0.0) return 0;
+  double var = 0.0;
+  for (long i = 0; i < n; ++i)
+    if (include[i]) {
+      double d = values[i] - mean;
+      var += d * d;
+    }
+  *cv = std::sqrt(var / (double)cnt) / std::fabs(mean);
+  return 1;
+}
+
+// mixing_time_from_cv: out = [cv0, t95 (NaN if none), t99 (NaN if none)]
+int orc_mixing_time(long n, const double* t, const double* cv, double* out) {
+  long first = -1;
+  for (long i = 0; i < n; ++i)
+    if (std::isfinite(cv[i]) && cv[i] > 0.0) {
+      first = i;
+      break;
+    }
+  if (first < 0) return 0;
+  double cv0 = cv[first];
+  double t95 = NAN, t99 = NAN;
+  bool h95 = false, h99 = false;
+  for (long i = 0; i < n; ++i) {
+    if (!std::isfinite(cv[i])) continue;
+    if (!h95 && cv[i] <= 0.05 * cv0) {
+      t95 = t[i];
+      h95 = true;
+    }
+    if (!h99 && cv[i] <= 0.01 * cv0) {
+      t99 = t[i];
+      h99 = true;
+    }
+  }
+  out[0] = cv0;
+  out[1] = t95;
+  out[2] = t99;
+  return 1;
+}
+
+// compartment_cv: out = [cv_top, cv_bulk, cv_near] (NaN if undefined),
+// counts = [n_top, n_bulk, n_near]
+void orc_compartment_cv(const int dims[3], const double* values, const uint8_t* solid, int has_imp, double imp_z,
+                        double* out, long* counts) {
+  size_t n = (size_t)dims[0] * dims[1] * dims[2];
+  std::vector<uint8_t> mask[3];
+  for (auto& m : mask) m.assign(n, 0);
+  for (long z = 0; z < dims[2]; ++z)
+    for (long y = 0; y < dims[1]; ++y)
+      for (long x = 0; x < dims[0]; ++x) {
+        size_t i = ((size_t)z * dims[1] + y) * dims[0] + x;
+        if (solid[i]) continue;
+        double zn = ((double)z + 0.5) / (double)dims[2];
+        if (zn >= 2.0 / 3.0)
+          mask[0][i] = 1;
+        else if (has_imp && std::fabs(((double)z + 0.5) - imp_z) <= (double)dims[2] / 6.0)
+          mask[2][i] = 1;
+        else
+          mask[1][i] = 1;
+      }
+  for (int k = 0; k < 3; ++k) {
+    double cv;
+    out[k] = orc_scalar_cv((long)n, values, mask[k].data(), &cv) ? cv : NAN;
+    long c = 0;
+    for (size_t i = 0; i < n; ++i) c += mask[k][i];
+    counts[k] = c;
+  }
+}
+
+}  // extern "C"
