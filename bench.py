@@ -140,6 +140,8 @@ def main():
     cells_main = fluid - n_list
     achieved = cells_main * B / (k_avg_ms * 1e-3) / 1e9
 
+    copy_gbps = s.ctx.stream_copy_gbps(4 << 30, 20)
+
     cpu = None
     if not args.no_cpu_baseline:
         threads = os.cpu_count() or 1
@@ -174,7 +176,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": "k_hydro (fused AA collide-stream)", "kernel_avg_ms": round(k_avg_ms, 4),
-                     "bytes_per_cell": B, "cells_per_launch": int(cells_main)},
+                     "bytes_per_cell": B, "cells_per_launch": int(cells_main),
+                     "stream_copy_gbs": round(copy_gbps, 1), "frac_of_copy": round(achieved / copy_gbps, 4)},
         "cpu_baseline": cpu,
         "checks": {"mass_finite": bool(np.isfinite(mass)), "mass_total": mass,
                    "torque_z": float(diag["torque"][2]), "ibm_iterations": diag["iterations"]},

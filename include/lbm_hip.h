@@ -116,6 +116,10 @@ const char* lbm_last_error(const lbm_ctx* ctx);
 int lbm_capabilities(const lbm_ctx* ctx, uint32_t* flags);
 /* Host-visible description of the compiled kernels (for diagnostics). */
 const char* lbm_build_info(void);
+/* Diagnostic with no reference counterpart: STREAM copy of `bytes` between
+ * two device buffers, `iterations` times; *gbps counts read + write bytes.
+ * The measured HBM ceiling that the collide-stream roofline is read against. */
+int lbm_stream_copy(lbm_ctx* ctx, size_t bytes, int iterations, double* gbps);
 
 /* ---- fields ------------------------------------------------------------- */
 /* Backend::alloc, backend.rs:138-139: quiescent fields (f = 0, rho = 1,

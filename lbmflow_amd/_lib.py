@@ -9,7 +9,7 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liblbm_hip.so")
+LIB_PATH = os.environ.get("LBMFLOW_LIB") or os.path.join(HERE, "liblbm_hip.so")
 
 LBM_OK = 0
 ERRORS = {-1: "invalid argument", -2: "unsupported", -3: "device error", -4: "out of device memory",
@@ -61,6 +61,7 @@ SIGNATURES = {
     "lbm_last_error": (C.c_char_p, [_P]),
     "lbm_capabilities": (_I, [_P, _pU32]),
     "lbm_build_info": (C.c_char_p, []),
+    "lbm_stream_copy": (_I, [_P, C.c_size_t, _I, _pD]),
     "lbm_fields_alloc": (_I, [_P, _I, _I, _pI, _pI, C.POINTER(_P)]),
     "lbm_fields_free": (None, [_P]),
     "lbm_fields_device_bytes": (C.c_size_t, [_P]),

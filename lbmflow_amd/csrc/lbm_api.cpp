@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -36,7 +37,9 @@ struct lbm_fields {
   Dom d{};
   Tiling t{};
   size_t elem = 4;
-  void* f = nullptr;
+  void* f = nullptr;        // first element of plane 0
+  void* f_alloc = nullptr;  // allocation start (guard band before plane 0)
+  size_t f_bytes = 0;
   uint32_t* mask = nullptr;
   uint8_t* solid = nullptr;
   float* wall_u = nullptr;
@@ -54,6 +57,10 @@ struct lbm_fields {
   unsigned int open_face_cells = 0;
   bool field_uniform = false;
   std::vector<uint32_t> flist_host;
+  // active wave tiles of the main kernel
+  uint32_t* tiles = nullptr;
+  size_t tiles_cap = 0;
+  uint32_t ntiles = 0;
   uint32_t* flist = nullptr;
   size_t flist_cap = 0;
   // reduction scratch
@@ -107,6 +114,13 @@ void dev_free(lbm_fields* fl, void* p, size_t bytes) {
 
 size_t N3(const lbm_fields* fl) { return (size_t)fl->d.N * 3; }
 
+// Plane padding in elements; LBM_PLANE_PAD overrides the default (tuning).
+uint64_t plane_pad_elems() {
+  const char* e = getenv("LBM_PLANE_PAD");
+  if (!e || !*e) return UINT64_MAX;
+  return strtoull(e, nullptr, 10);
+}
+
 HydroParams build_hp(const lbm_step_params& p) {
   // KParams::new, params.rs:350-374: rates rounded to fp32, Guo prefactors
   // computed in f64 then rounded.
@@ -132,6 +146,8 @@ int ensure_force_arrays(lbm_fields* fl) {
   return LBM_OK;
 }
 
+int rebuild_tiles(lbm_fields* fl);
+
 // Upload the sorted union of the current list and `extra` cells, and flag
 // them in the mask so the main kernel leaves them to the list kernel.
 int grow_force_list(lbm_fields* fl, const std::vector<uint32_t>& extra) {
@@ -153,6 +169,38 @@ int grow_force_list(lbm_fields* fl, const std::vector<uint32_t>& extra) {
   HIPCHK(fl->ctx, launch_mark_forced(fl->flist, (uint32_t)fl->flist_host.size(), fl->mask, fl->ctx->stream));
   // the host vector must outlive the async copy
   HIPCHK(fl->ctx, hipStreamSynchronize(fl->ctx->stream));
+  return rebuild_tiles(fl);
+}
+
+// Rebuild the list of 64-cell tiles the main kernel must visit.
+int rebuild_tiles(lbm_fields* fl) {
+  const uint32_t nseg = (fl->d.nx + 63) / 64;
+  const size_t nt = (size_t)nseg * fl->d.ny * fl->d.nz;
+  uint8_t* dflags = nullptr;
+  int rc = dev_alloc(fl, (void**)&dflags, nt);
+  if (rc) return rc;
+  hipStream_t s = fl->ctx->stream;
+  std::vector<uint8_t> flags(nt);
+  HIPCHK(fl->ctx, launch_tile_flags(fl->d, fl->mask, dflags, s));
+  HIPCHK(fl->ctx, hipMemcpyAsync(flags.data(), dflags, nt, hipMemcpyDeviceToHost, s));
+  HIPCHK(fl->ctx, hipStreamSynchronize(s));
+  dev_free(fl, dflags, nt);
+  std::vector<uint32_t> list;
+  list.reserve(nt);
+  for (size_t t = 0; t < nt; ++t)
+    if (flags[t]) list.push_back((uint32_t)t);
+  if (list.size() > fl->tiles_cap) {
+    dev_free(fl, fl->tiles, fl->tiles_cap * 4);
+    fl->tiles = nullptr;
+    fl->tiles_cap = std::max<size_t>(list.size(), 1);
+    rc = dev_alloc(fl, (void**)&fl->tiles, fl->tiles_cap * 4);
+    if (rc) return rc;
+  }
+  if (!list.empty()) {
+    HIPCHK(fl->ctx, hipMemcpyAsync(fl->tiles, list.data(), list.size() * 4, hipMemcpyHostToDevice, s));
+    HIPCHK(fl->ctx, hipStreamSynchronize(s));
+  }
+  fl->ntiles = (uint32_t)list.size();
   return LBM_OK;
 }
 
@@ -172,13 +220,16 @@ int rebuild_mask(lbm_fields* fl) {
   // Fluid on a non-periodic face needs an open-face BC, which is not on this
   // path; stepping such a domain is refused (lbm_run_span).
   fl->open_face_cells = nerr;
-  return LBM_OK;
+  return rebuild_tiles(fl);
 }
 
 HydroArgs hydro_args(lbm_fields* fl) {
   HydroArgs a{};
   a.d = fl->d;
   a.t = fl->t;
+  a.tiles = fl->tiles;
+  a.ntiles = fl->ntiles;
+  a.nseg = (fl->d.nx + 63) / 64;
   a.f = fl->f;
   a.mask = fl->mask;
   a.wall_u = fl->wall_u;
@@ -254,6 +305,37 @@ void lbm_ctx_destroy(lbm_ctx* ctx) {
 
 const char* lbm_last_error(const lbm_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
+int lbm_stream_copy(lbm_ctx* ctx, size_t bytes, int iterations, double* gbps) {
+  if (!ctx || !gbps || iterations <= 0 || bytes < 16) return LBM_ERR_INVALID;
+  bytes &= ~(size_t)15;
+  void *a = nullptr, *b = nullptr;
+  if (hipMalloc(&a, bytes) != hipSuccess) return fail(ctx, LBM_ERR_NOMEM, "stream copy: hipMalloc failed");
+  if (hipMalloc(&b, bytes) != hipSuccess) {
+    (void)hipFree(a);
+    return fail(ctx, LBM_ERR_NOMEM, "stream copy: hipMalloc failed");
+  }
+  hipStream_t s = ctx->stream;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = LBM_OK;
+  float ms = 0.0f;
+  if (hipMemsetAsync(a, 0, bytes, s) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+      hipEventCreate(&e1) != hipSuccess || launch_stream_copy(a, b, bytes, s) != hipSuccess ||
+      hipEventRecord(e0, s) != hipSuccess)
+    rc = LBM_ERR_DEVICE;
+  for (int k = 0; rc == LBM_OK && k < iterations; ++k)
+    if (launch_stream_copy(a, b, bytes, s) != hipSuccess) rc = LBM_ERR_DEVICE;
+  if (rc == LBM_OK && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                       hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+    rc = LBM_ERR_DEVICE;
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  if (rc) return fail(ctx, rc, "stream copy failed");
+  *gbps = 2.0 * (double)bytes * iterations / (ms * 1e-3) / 1e9;
+  return LBM_OK;
+}
+
 int lbm_capabilities(const lbm_ctx* ctx, uint32_t* flags) {
   if (!ctx || !flags) return LBM_ERR_INVALID;
   *flags = LBM_CAP_TRT | LBM_CAP_CENTRAL_MOMENT | LBM_CAP_F16_STORAGE | LBM_CAP_BF16_STORAGE | LBM_CAP_FORCE_FIELD |
@@ -270,7 +352,8 @@ int lbm_fields_alloc(lbm_ctx* ctx, int Q, int storage, const int dims[3], const 
   for (int a = 0; a < 3; ++a)
     if (dims[a] < 1) return fail(ctx, LBM_ERR_INVALID, "dims must be positive");
   const uint64_t n = (uint64_t)dims[0] * dims[1] * dims[2];
-  if (n >= (1ull << 31)) return fail(ctx, LBM_ERR_INVALID, "domain of %llu cells exceeds 2^31", (unsigned long long)n);
+  // byte offsets inside one population plane are 32-bit in the kernels
+  if (n >= (1ull << 30)) return fail(ctx, LBM_ERR_INVALID, "domain of %llu cells exceeds 2^30", (unsigned long long)n);
   lbm_fields* fl = new (std::nothrow) lbm_fields();
   if (!fl) return LBM_ERR_NOMEM;
   fl->ctx = ctx;
@@ -281,9 +364,27 @@ int lbm_fields_alloc(lbm_ctx* ctx, int Q, int storage, const int dims[3], const 
   fl->d.nz = dims[2];
   fl->d.N = (uint32_t)n;
   for (int a = 0; a < 3; ++a) fl->d.per[a] = periodic ? (periodic[a] != 0) : 0;
+  // Plane stride: the cell count rounded up to 256 elements, plus an odd
+  // multiple of 256 elements so the Q concurrent plane streams do not share
+  // the same low address bits (a 512^3 fp32 plane is exactly 2^29 bytes).
+  {
+    uint64_t ps = (n + 255) / 256 * 256;
+    uint64_t pad = plane_pad_elems();
+    if (pad == UINT64_MAX) pad = 256 * 9;
+    ps += pad;
+    if (ps * 4 >= (1ull << 32)) ps = n;
+    fl->d.ps = (uint32_t)ps;
+  }
   fl->t = make_tiling(fl->d, 0);
   fl->elem = storage == kF32 ? 4 : 2;
-  int rc = dev_alloc(fl, &fl->f, (size_t)Q * n * fl->elem);
+  // Guard bands of one z-slab (+ row + cell) before plane 0 and after the
+  // last plane: the collide-stream kernel issues its neighbour loads before
+  // it knows whether a cell is solid, and a solid cell on the domain face
+  // would otherwise address just outside the buffer.
+  const size_t guard = ((size_t)fl->d.nx * fl->d.ny + fl->d.nx + 1 + 255) / 256 * 256;
+  fl->f_bytes = ((size_t)Q * fl->d.ps + 2 * guard) * fl->elem;
+  int rc = dev_alloc(fl, &fl->f_alloc, fl->f_bytes);
+  if (!rc) fl->f = (char*)fl->f_alloc + guard * fl->elem;
   if (!rc) rc = dev_alloc(fl, (void**)&fl->mask, n * 4);
   if (!rc) rc = dev_alloc(fl, (void**)&fl->solid, n);
   fl->npartials = 1024;
@@ -293,7 +394,7 @@ int lbm_fields_alloc(lbm_ctx* ctx, int Q, int storage, const int dims[3], const 
     return rc;
   }
   hipStream_t s = ctx->stream;
-  if (hipMemsetAsync(fl->f, 0, (size_t)Q * n * fl->elem, s) != hipSuccess ||
+  if (hipMemsetAsync(fl->f_alloc, 0, fl->f_bytes, s) != hipSuccess ||
       hipMemsetAsync(fl->solid, 0, n, s) != hipSuccess) {
     lbm_fields_free(fl);
     return fail(ctx, LBM_ERR_DEVICE, "memset failed");
@@ -324,8 +425,8 @@ void lbm_fields_free(lbm_fields* fl) {
   if (fl->ibm) ibm_destroy(fl->ibm);
   for (ScalarState* s : fl->scalars) scalar_destroy(s);
   for (hipEvent_t e : fl->ev_pool) (void)hipEventDestroy(e);
-  for (void* p : {(void*)fl->f, (void*)fl->mask, (void*)fl->solid, (void*)fl->wall_u, (void*)fl->omega,
-                  (void*)fl->F_cur, (void*)fl->F_mom, (void*)fl->flist, (void*)fl->partials})
+  for (void* p : {(void*)fl->f_alloc, (void*)fl->mask, (void*)fl->solid, (void*)fl->wall_u, (void*)fl->omega,
+                  (void*)fl->F_cur, (void*)fl->F_mom, (void*)fl->flist, (void*)fl->partials, (void*)fl->tiles})
     if (p) (void)hipFree(p);
   delete fl;
 }

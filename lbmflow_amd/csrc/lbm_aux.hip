@@ -101,7 +101,7 @@ __global__ void k_ibm_gather(Dom d, const S* f, int odd, const uint32_t* uc_cell
   double mom[3] = {0.0, 0.0, 0.0};
   sfor<0, Q>([&](auto qc) {
     constexpr int q = decltype(qc)::value;
-    const float v = read_pop<Q, S, q>(f, d.N, nb, i, odd != 0);
+    const float v = read_pop<Q, S, q>(f, d.ps, nb, i, odd != 0);
     dr = dr + v;
     if constexpr (lat_c(q, 0) != 0) mom[0] += (double)lat_c(q, 0) * (double)v;
     if constexpr (lat_c(q, 1) != 0) mom[1] += (double)lat_c(q, 1) * (double)v;

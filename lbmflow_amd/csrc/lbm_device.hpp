@@ -168,11 +168,11 @@ __device__ __forceinline__ void moments(const float* fv, const float Fm[3], bool
 // read a population of cell i in the reference sense (post-stream f_q)
 // ---------------------------------------------------------------------------
 template <int Q, typename S, int q>
-__device__ __forceinline__ float read_pop(const S* base, size_t N, const Nbr& nb, uint32_t i, bool odd) {
+__device__ __forceinline__ float read_pop(const S* base, size_t ps, const Nbr& nb, uint32_t i, bool odd) {
   constexpr int o = lat_opp(q);
-  if (!odd) return pop_ld<S, Q, q>(base + (size_t)q * N, i);
+  if (!odd) return pop_ld<S, Q, q>(base + (size_t)q * ps, i);
   const uint32_t s = i + (uint32_t)nb.template off<-lat_c(q, 0), -lat_c(q, 1), -lat_c(q, 2)>();
-  return pop_ld<S, Q, q>(base + (size_t)o * N, s);
+  return pop_ld<S, Q, q>(base + (size_t)o * ps, s);
 }
 
 __device__ __forceinline__ void cell_xyz(const Dom& d, uint32_t i, uint32_t& x, uint32_t& y, uint32_t& z) {

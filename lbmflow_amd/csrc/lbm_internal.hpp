@@ -20,7 +20,8 @@ enum Storage : int { kF32 = 0, kF16 = 1, kBF16 = 2 };
 
 struct Dom {
   uint32_t nx, ny, nz;
-  uint32_t N;        // nx*ny*nz (< 2^31)
+  uint32_t N;        // nx*ny*nz cells
+  uint32_t ps;       // population plane stride in elements (>= N)
   uint32_t per[3];   // periodic axes
 };
 
@@ -42,6 +43,11 @@ struct HydroParams {
 struct HydroArgs {
   Dom d;
   Tiling t;
+  // active wave tiles: 64-cell x-segments holding at least one cell of the
+  // main kernel; tile id = (z*ny + y)*nseg + segment
+  const uint32_t* tiles;
+  uint32_t ntiles;
+  uint32_t nseg;
   void* f;                  // Q planes of N storage elements
   const uint32_t* mask;     // N
   const float* wall_u;      // N x 3 or null
@@ -60,6 +66,7 @@ hipError_t launch_hydro_list(int Q, int storage, int collision, int odd, const H
 hipError_t launch_build_mask(int Q, const Dom& d, const uint8_t* solid, const float* wall_u, uint32_t* mask,
                              unsigned int* err_count, hipStream_t s);
 hipError_t launch_mark_forced(const uint32_t* flist, uint32_t n, uint32_t* mask, hipStream_t s);
+hipError_t launch_tile_flags(const Dom& d, const uint32_t* mask, uint8_t* flags, hipStream_t s);
 hipError_t launch_stage(int Q, int storage, int dir, int odd, const Dom& d, void* f, float* host_layout,
                         const uint32_t* mask, hipStream_t s);
 hipError_t launch_init_with(int Q, int storage, const Dom& d, void* f, const uint32_t* mask, const float* rho,
@@ -75,6 +82,7 @@ hipError_t launch_tank(const Dom& d, double dx, double diameter, double height, 
 hipError_t launch_rims(const Dom& d, const int is_wall[6], const double u[18], uint8_t* solid, float* wall_u,
                        double* best, hipStream_t s);
 hipError_t launch_fill_f32(float* p, size_t n, float v, hipStream_t s);
+hipError_t launch_stream_copy(const void* a, void* b, size_t bytes, hipStream_t s);
 hipError_t launch_copy_force_list(const uint32_t* flist, uint32_t n, float* F_dst, const float* F_src,
                                   hipStream_t s);
 

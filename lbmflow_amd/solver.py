@@ -71,6 +71,12 @@ class Context:
         self.check(self.lib.lbm_capabilities(self.h, C.byref(v)))
         return v.value
 
+    def stream_copy_gbps(self, nbytes: int = 2 << 30, iterations: int = 20) -> float:
+        """Measured copy bandwidth (read + write bytes) on this device."""
+        v = C.c_double()
+        self.check(self.lib.lbm_stream_copy(self.h, int(nbytes), int(iterations), C.byref(v)))
+        return v.value
+
     def close(self):
         if getattr(self, "h", None):
             self.lib.lbm_ctx_destroy(self.h)

@@ -80,3 +80,72 @@ def tank_geometry(n, *, T=None, H=None, baffle_width=None, baffle_thickness=None
     solid = O.tank_mask((n, n, n), 1.0, T, H)
     solid, baffle = O.baffle_mask((n, n, n), 1.0, T, H, bw, bt, solid)
     return solid, (T, H, bw, bt)
+
+
+def tank_case(n, *, nu=0.01, tip=0.05, collision="cm", Q=27, sigma=None):
+    """Small stirred tank (same ratios as the benchmark): geometry, markers
+    and a Gaussian tracer near the surface."""
+    T = 15.0 * n / 16.0
+    H = T
+    solid, _ = tank_geometry(n, T=T, H=H, baffle_width=T / 10.0, baffle_thickness=T / 48.0)
+    D = T / 3.0
+    omega = tip / (0.5 * D)
+    pos, w, vel, c = O.radial_impeller((n, n, n), 1.0, T, H, D, T / 3.0, 6, 0.0, omega)
+    x, y, z = coords((n, n, n))
+    sig = (8.0 * n / 512.0 * 8.0) if sigma is None else sigma
+    cz = 0.8 * H
+    blob = np.exp(-((x + 0.5 - n / 2) ** 2 + (y + 0.5 - n / 2) ** 2 + (z + 0.5 - cz) ** 2) / (2 * sig ** 2))
+    blob = np.where(solid == 0, blob, 0.0).astype(np.float32).astype(np.float64)
+    return {"n": n, "T": T, "H": H, "D": D, "omega": omega, "nu": nu, "solid": solid, "pos": pos, "w": w,
+            "c": c, "blob": blob, "collision": collision, "Q": Q}
+
+
+def run_tank_oracle(tc, steps, prec=4, every=1, scalar_D=None, d3q7=True):
+    s = O.OracleSolver(tc["Q"], (tc["n"],) * 3, prec=prec, collision=COLL[tc["collision"]], nu=tc["nu"],
+                       solid=tc["solid"])
+    k = None
+    if scalar_D is not None:
+        k = s.scalar_enable(scalar_D, d3q7)
+        s.scalar_set(k, tc["blob"])
+    torque, cv = [], []
+    fl = (tc["solid"] == 0).astype(np.uint8)
+    for step in range(1, steps + 1):
+        s.clear_force_field()
+        d = s.apply_ibm(tc["pos"], tc["w"], tc["c"], (0.0, 0.0, tc["omega"]))
+        s.step()
+        if step % every == 0:
+            torque.append(-d["torque"][2])
+            if k is not None:
+                cv.append(O.scalar_cv(s.scalar_get(k), fl))
+    return np.array(torque), np.array(cv), s
+
+
+def run_tank_gpu(tc, steps, storage="f32", every=1, scalar_D=None, d3q7=True):
+    import lbmflow_amd as L
+
+    n = tc["n"]
+    s = L.Solver(tc["Q"], (n, n, n), nu=tc["nu"], collision=tc["collision"], storage=storage, solid=tc["solid"])
+    s.set_rotating_body(tc["pos"], tc["w"], tc["c"], (0.0, 0.0, tc["omega"]))
+    k = None
+    if scalar_D is not None:
+        k = s.enable_scalar_distribution(scalar_D, "d3q7" if d3q7 else "hydro")
+        s.set_scalar_concentration(k, tc["blob"])
+    torque, cv = [], []
+    for step in range(1, steps + 1):
+        s.clear_body_force_field()
+        d = s.apply_rotating_ibm()
+        s.step()
+        if step % every == 0:
+            torque.append(-d["torque"][2])
+            if k is not None:
+                cv.append(s.scalar_cv(k))
+    return np.array(torque), np.array(cv), s
+
+
+def np_from_torque(torque, tc, steps, every=1):
+    """Last-half mean shaft torque (runner.rs:1514-1530) -> Np (qoi.rs:477-505)."""
+    steps_rec = np.arange(1, len(torque) + 1) * every
+    win = torque[steps_rec >= steps // 2]
+    tq = win.mean() if len(win) else torque.mean()
+    q = O.power_qois(tq, tc["omega"], 1.0, tc["D"], 1.0)
+    return q["np"], tq
