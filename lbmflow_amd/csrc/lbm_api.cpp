@@ -204,9 +204,29 @@ int rebuild_tiles(lbm_fields* fl) {
   return LBM_OK;
 }
 
+// The odd-state layout depends on the wall links (a bounced population sits
+// in the fluid cell's own slot), so before the links change the populations
+// are brought to the even state (every cell's own slots) under the old mask.
+static int normalize_parity(lbm_fields* fl) {
+  if (!fl->parity) return LBM_OK;
+  const size_t bytes = (size_t)fl->Q * fl->d.N * 4;
+  float* tmp = nullptr;
+  int rc = dev_alloc(fl, (void**)&tmp, bytes);
+  if (rc) return rc;
+  hipStream_t s = fl->ctx->stream;
+  HIPCHK(fl->ctx, launch_stage(fl->Q, fl->storage, 1, 1, fl->d, fl->f, tmp, fl->mask, s));
+  HIPCHK(fl->ctx, launch_stage(fl->Q, fl->storage, 0, 0, fl->d, fl->f, tmp, fl->mask, s));
+  HIPCHK(fl->ctx, hipStreamSynchronize(s));
+  dev_free(fl, tmp, bytes);
+  fl->parity = 0;
+  return LBM_OK;
+}
+
 int rebuild_mask(lbm_fields* fl) {
+  int rc = normalize_parity(fl);
+  if (rc) return rc;
   unsigned int* err = nullptr;
-  int rc = dev_alloc(fl, (void**)&err, sizeof(unsigned int));
+  rc = dev_alloc(fl, (void**)&err, sizeof(unsigned int));
   if (rc) return rc;
   hipStream_t s = fl->ctx->stream;
   HIPCHK(fl->ctx, hipMemsetAsync(err, 0, sizeof(unsigned int), s));

@@ -88,20 +88,21 @@ struct IbmState {
 // (solver.rs:3040-3084: f64 momentum of the populations, force without the
 // just-cleared field).
 template <int Q, typename S>
-__global__ void k_ibm_gather(Dom d, const S* f, int odd, const uint32_t* uc_cell, uint32_t U, const float* F_cur,
-                             int field_present, HydroParams hp, double* unow, double* rho, double* du,
-                             double* spread) {
+__global__ void k_ibm_gather(Dom d, const S* f, const uint32_t* mask, int odd, const uint32_t* uc_cell, uint32_t U,
+                             const float* F_cur, int field_present, HydroParams hp, double* unow, double* rho,
+                             double* du, double* spread) {
   const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
   if (u >= U) return;
   const uint32_t i = uc_cell[u];
   uint32_t x, y, z;
   cell_xyz(d, i, x, y, z);
   const Nbr nb(d, x, y, z);
+  const uint32_t m = mask[i];
   float dr = 0.0f;
   double mom[3] = {0.0, 0.0, 0.0};
   sfor<0, Q>([&](auto qc) {
     constexpr int q = decltype(qc)::value;
-    const float v = read_pop<Q, S, q>(f, d.ps, nb, i, odd != 0);
+    const float v = read_pop<Q, S, q>(f, d.ps, nb, i, m, odd != 0);
     dr = dr + v;
     if constexpr (lat_c(q, 0) != 0) mom[0] += (double)lat_c(q, 0) * (double)v;
     if constexpr (lat_c(q, 1) != 0) mom[1] += (double)lat_c(q, 1) * (double)v;
@@ -429,7 +430,7 @@ IbmState* ibm_create(const FieldsView& v, int64_t n, const double* pos, const do
 
 template <int Q, typename S>
 static void gather_q(const FieldsView& v, IbmState* s) {
-  hipLaunchKernelGGL((k_ibm_gather<Q, S>), dim3(nblk(s->U)), dim3(256), 0, v.stream, v.d, (const S*)v.f, v.parity,
+  hipLaunchKernelGGL((k_ibm_gather<Q, S>), dim3(nblk(s->U)), dim3(256), 0, v.stream, v.d, (const S*)v.f, v.mask, v.parity,
                      s->uc_cell, s->U, v.F_cur, v.field_present ? 1 : 0, v.hp, s->unow, s->rho, s->du, s->spread);
 }
 

@@ -167,10 +167,15 @@ __device__ __forceinline__ void moments(const float* fv, const float Fm[3], bool
 // ---------------------------------------------------------------------------
 // read a population of cell i in the reference sense (post-stream f_q)
 // ---------------------------------------------------------------------------
+// Even state: f_q(i) sits in the cell's own slot q.  Odd state (after an
+// even step): in the upstream cell's slot opp(q), except across a wall link
+// (cell i - c_q solid), where the bounced population stays in the cell's own
+// slot q.  `m` is the cell's mask word.
 template <int Q, typename S, int q>
-__device__ __forceinline__ float read_pop(const S* base, size_t ps, const Nbr& nb, uint32_t i, bool odd) {
+__device__ __forceinline__ float read_pop(const S* base, size_t ps, const Nbr& nb, uint32_t i, uint32_t m,
+                                          bool odd) {
   constexpr int o = lat_opp(q);
-  if (!odd) return pop_ld<S, Q, q>(base + (size_t)q * ps, i);
+  if (!odd || ((m >> q) & 1u)) return pop_ld<S, Q, q>(base + (size_t)q * ps, i);
   const uint32_t s = i + (uint32_t)nb.template off<-lat_c(q, 0), -lat_c(q, 1), -lat_c(q, 2)>();
   return pop_ld<S, Q, q>(base + (size_t)o * ps, s);
 }

@@ -451,7 +451,17 @@ __device__ __forceinline__ void slot_st(S* p, float v) {
   Store<S>::st(p, 0, v);
 }
 
-template <int Q, typename S, int KIND, bool ODD, bool FON, bool WRAP>
+// Bounce-back layout.  A population reflected off the wall at i + c_q is
+// stored where the plain AA store of direction q would put it in the cell's
+// own slots: even steps write slot opp(q) of cell i (the plain target), odd
+// steps write slot opp(q) of cell i instead of slot q of i + c_q.  The odd
+// step then reads a wall-link direction q (i - c_q solid) from the cell's
+// own slot q, not from the solid upstream cell.  So no fluid cell ever reads
+// or writes a solid cell's slots, and a solid cell inside an active tile can
+// rewrite its own slots freely: even steps do, which turns every store of an
+// active tile into whole cache lines (partial lines cost the memory a
+// read-modify-write).
+template <int Q, typename S, int KIND, bool ODD, bool FON, bool WRAP, bool FILL>
 __device__ __forceinline__ void cell_step(const HydroArgs& a, uint32_t i, uint32_t x, uint32_t y, uint32_t z,
                                           const uint32_t* mask_ptr, uint32_t skip, const float Fm[3],
                                           const float Fc[3]) {
@@ -471,7 +481,29 @@ __device__ __forceinline__ void cell_step(const HydroArgs& a, uint32_t i, uint32
     else
       fv[q] = slot_ld<S, Q, q>(sl.template at<o, -lat_c(q, 0), -lat_c(q, 1), -lat_c(q, 2)>());
   });
+  // Consume every loaded value before the solid test, so the compiler cannot
+  // sink the population loads behind the mask branch: one memory latency per
+  // cell instead of two.
+#pragma unroll
+  for (int q = 0; q < Q; ++q) asm volatile("" : "+v"(fv[q]));
+  if constexpr (FILL && !ODD) {
+    if (m & kMaskSolid) {  // dead slots: rewrite them so the tile's lines are whole
+      sfor<0, Q>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        slot_st<S, Q, q>(sl.template at<lat_opp(q), 0, 0, 0>(), fv[q]);
+      });
+      return;
+    }
+  }
   if (m & skip) return;
+  if constexpr (ODD) {
+    if (m & kMaskLinks) {  // wall links: the bounced population is in the own slot
+      sfor<1, Q>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        if ((m >> q) & 1u) fv[q] = slot_ld<S, Q, q>(sl.template at<q, 0, 0, 0>());
+      });
+    }
+  }
   float op = a.p.omega_p, cp = a.p.cp, os = a.p.omega_shear;
   if (a.omega) {  // per-cell omega_plus (kernels.rs:151-156)
     op = a.omega[i];
@@ -493,6 +525,22 @@ __device__ __forceinline__ void cell_step(const HydroArgs& a, uint32_t i, uint32
       r = collide_cm27_free_direct(fv, os);
     }
   }
+  // half-way bounce-back off a moving wall at i + c_q (stream_row,
+  // kernels.rs:591-603): the reflected population enters i along c_o with
+  // fin = fout + 6 w_o rho (c_o . u_wall)
+  if (m & kMaskMoving) {
+    sfor<1, Q>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      constexpr int o = lat_opp(q);
+      if ((m >> o) & 1u) {
+        const uint32_t fwd = i + (uint32_t)sl.nb.template off<lat_c(q, 0), lat_c(q, 1), lat_c(q, 2)>();
+        const float* wu = a.wall_u + 3 * (size_t)fwd;
+        const float cu = cdot<o>(wu[0], wu[1], wu[2]);
+        constexpr float six_w = 6.0f * lat_wf(Q, o);
+        fv[q] = fv[q] + six_w * r * cu;
+      }
+    });
+  }
   // plain streaming store of direction q
   auto store_plain = [&](auto qc) {
     constexpr int q = decltype(qc)::value;
@@ -502,36 +550,18 @@ __device__ __forceinline__ void cell_step(const HydroArgs& a, uint32_t i, uint32
     else
       slot_st<S, Q, q>(sl.template at<q, lat_c(q, 0), lat_c(q, 1), lat_c(q, 2)>(), fv[q]);
   };
-  if ((m & kMaskLinks) == 0) {  // no wall next to this cell: uniform in the bulk
+  // even steps: the bounced population goes to the plain target as well
+  if (ODD == false || (m & kMaskLinks) == 0) {
     sfor<0, Q>(store_plain);
     return;
   }
-  const bool moving = (m & kMaskMoving) != 0;
   sfor<0, Q>([&](auto qc) {
     constexpr int q = decltype(qc)::value;
     constexpr int o = lat_opp(q);
-    constexpr int cx = lat_c(q, 0), cy = lat_c(q, 1), cz = lat_c(q, 2);
-    if constexpr (q == 0) {
+    if ((m >> o) & 1u)
+      slot_st<S, Q, q>(sl.template at<o, 0, 0, 0>(), fv[q]);
+    else
       store_plain(qc);
-    } else if ((m >> o) & 1u) {
-      // half-way bounce-back off the solid at i + c_q (stream_row,
-      // kernels.rs:591-603): the reflected population enters i along c_o
-      // with fin = fout + 6 w_o rho (c_o . u_wall)
-      float v = fv[q];
-      if (moving) {
-        const uint32_t fwd = i + (uint32_t)sl.nb.template off<cx, cy, cz>();
-        const float* wu = a.wall_u + 3 * (size_t)fwd;
-        const float cu = cdot<o>(wu[0], wu[1], wu[2]);
-        constexpr float six_w = 6.0f * lat_wf(Q, o);
-        v = v + six_w * r * cu;
-      }
-      if constexpr (!ODD)
-        slot_st<S, Q, q>(sl.template at<q, cx, cy, cz>(), v);
-      else
-        slot_st<S, Q, q>(sl.template at<o, 0, 0, 0>(), v);
-    } else {
-      store_plain(qc);
-    }
   });
 }
 
@@ -548,7 +578,8 @@ __global__ __launch_bounds__(256) void k_hydro(HydroArgs a) {
   if (x >= a.d.nx) return;
   const uint32_t y = row % a.d.ny, z = row / a.d.ny;
   const uint32_t i = row * a.d.nx + x;
-  cell_step<Q, S, KIND, ODD, FON, WRAP>(a, i, x, y, z, a.mask + i, kMaskSolid | kMaskForced, a.p.F, a.p.F);
+  cell_step<Q, S, KIND, ODD, FON, WRAP, true>(a, i, x, y, z, a.mask + i, kMaskSolid | kMaskForced, a.p.F,
+                                              a.p.F);
 }
 
 // Cells carrying a per-cell force: the velocity uses the force of the cached
@@ -571,7 +602,7 @@ __global__ __launch_bounds__(256) void k_hydro_list(HydroArgs a) {
     Fm[c] = a.p.F[c] + a.F_mom[3 * (size_t)i + c];
     Fc[c] = a.p.F[c] + a.F_cur[3 * (size_t)i + c];
   }
-  cell_step<Q, S, KIND, ODD, true, true>(a, i, x, y, z, a.mask + i, kMaskSolid, Fm, Fc);
+  cell_step<Q, S, KIND, ODD, true, true, false>(a, i, x, y, z, a.mask + i, kMaskSolid, Fm, Fc);
 #pragma unroll
   for (int c = 0; c < 3; ++c) a.F_mom[3 * (size_t)i + c] = a.F_cur[3 * (size_t)i + c];
 }
@@ -582,7 +613,8 @@ __global__ void k_moments(Dom d, const S* f, const uint32_t* mask, const float* 
                           float* rho, float* ux, float* uy, float* uz) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= d.N) return;
-  if (mask[i] & kMaskSolid) {
+  const uint32_t m = mask[i];
+  if (m & kMaskSolid) {
     rho[i] = 1.0f;
     ux[i] = uy[i] = uz[i] = 0.0f;
     return;
@@ -593,7 +625,7 @@ __global__ void k_moments(Dom d, const S* f, const uint32_t* mask, const float* 
   float fv[Q];
   sfor<0, Q>([&](auto qc) {
     constexpr int q = decltype(qc)::value;
-    fv[q] = read_pop<Q, S, q>(f, d.ps, nb, i, odd != 0);
+    fv[q] = read_pop<Q, S, q>(f, d.ps, nb, i, m, odd != 0);
   });
   float Fm[3] = {p.F[0], p.F[1], p.F[2]};
   bool fon = p.force_on != 0;
@@ -622,13 +654,14 @@ __global__ void k_stage(Dom d, S* f, float* h, const uint32_t* mask, int dir, in
     });
     return;
   }
-  const bool solid = (mask[i] & kMaskSolid) != 0;
+  const uint32_t m = mask[i];
+  const bool solid = (m & kMaskSolid) != 0;
   uint32_t x, y, z;
   cell_xyz(d, i, x, y, z);
   const Nbr nb(d, x, y, z);
   sfor<0, Q>([&](auto qc) {
     constexpr int q = decltype(qc)::value;
-    h[(size_t)q * d.N + i] = solid ? 0.0f : read_pop<Q, S, q>(f, d.ps, nb, i, odd != 0);
+    h[(size_t)q * d.N + i] = solid ? 0.0f : read_pop<Q, S, q>(f, d.ps, nb, i, m, odd != 0);
   });
 }
 
@@ -726,7 +759,8 @@ __global__ __launch_bounds__(256) void k_reduce(Dom d, const S* f, const uint32_
   __shared__ double sm[256];
   double acc = 0.0;
   for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < d.N; i += gridDim.x * 256) {
-    if (mask[i] & kMaskSolid) continue;
+    const uint32_t m = mask[i];
+    if (m & kMaskSolid) continue;
     if (kind == 0) {
       acc += 1.0;
       continue;
@@ -737,7 +771,7 @@ __global__ __launch_bounds__(256) void k_reduce(Dom d, const S* f, const uint32_
     double s = 0.0, mom = 0.0;
     sfor<0, Q>([&](auto qc) {
       constexpr int q = decltype(qc)::value;
-      const double v = (double)read_pop<Q, S, q>(f, d.ps, nb, i, odd != 0);
+      const double v = (double)read_pop<Q, S, q>(f, d.ps, nb, i, m, odd != 0);
       s += v;
       if (kind == 2) {
         const int c = axis == 0 ? lat_c(q, 0) : (axis == 1 ? lat_c(q, 1) : lat_c(q, 2));
