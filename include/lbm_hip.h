@@ -218,6 +218,17 @@ int lbm_ibm_set_body(lbm_fields* fl, int64_t n, const double* pos, const double*
 /* One direct-forcing update: adds the spread impulses to the force field
  * (the caller clears it first, as the runner does, runner.rs:1080-1084). */
 int lbm_ibm_apply(lbm_fields* fl, lbm_ibm_diag* diag);
+/* `steps` runner steps (runner.rs:1079-1129: clear the force field, one
+ * rotating-IBM update, one collide-stream step, the scalar distributions)
+ * with no host synchronisation inside the span: the per-step diagnostics
+ * are kept on the device and read back at the end.  torque (nullable)
+ * receives steps x 3 values (IbmDiagnostics::torque of each step); last
+ * (nullable) the final step's diagnostics.  use_graph != 0 replays pairs of
+ * steps from one captured HIP graph (recaptured whenever masks, parameters,
+ * force list, IBM body or scalars change); the results are identical to the
+ * eager launches.  The reference's equivalent seam is Backend::run_chunk_size
+ * / finish_run_chunk (backend.rs:340-349). */
+int lbm_run_ibm_span(lbm_fields* fl, int steps, int use_graph, double* torque, lbm_ibm_diag* last);
 
 /* ---- passive scalar ADE (solver.rs:2113-2205, scalar.rs:6-15) ------------ */
 int lbm_scalar_enable(lbm_fields* fl, double diffusivity, int lattice, int* index);

@@ -91,6 +91,7 @@ SIGNATURES = {
                                       _pI64]),
     "lbm_ibm_set_body": (_I, [_P, C.c_int64, _pD, _pD, _pD, _pD, C.POINTER(IbmConfig)]),
     "lbm_ibm_apply": (_I, [_P, C.POINTER(IbmDiag)]),
+    "lbm_run_ibm_span": (_I, [_P, _I, _I, _pD, C.POINTER(IbmDiag)]),
     "lbm_scalar_enable": (_I, [_P, _D, _I, _pI]),
     "lbm_scalar_set": (_I, [_P, _I, _pF]),
     "lbm_scalar_gather": (_I, [_P, _I, _pF]),

@@ -73,6 +73,18 @@ struct lbm_fields {
   // rotating IBM and scalar distributions
   IbmState* ibm = nullptr;
   std::vector<ScalarState*> scalars;
+  // device-resident step loop: per-step IBM diagnostics ring, and a HIP
+  // graph of two consecutive steps (even + odd) with the key it was
+  // captured under (every argument the launches bake in)
+  void* ring = nullptr;
+  uint32_t ring_cap = 0;
+  uint32_t* ring_pos = nullptr;
+  uint64_t gen = 0;  // bumped by every change a captured step would not see
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  HydroArgs g_args{};
+  int g_parity = -1;
+  uint64_t g_gen = 0;
   size_t bytes = 0;
 };
 
@@ -185,10 +197,25 @@ int rebuild_tiles(lbm_fields* fl) {
   HIPCHK(fl->ctx, hipMemcpyAsync(flags.data(), dflags, nt, hipMemcpyDeviceToHost, s));
   HIPCHK(fl->ctx, hipStreamSynchronize(s));
   dev_free(fl, dflags, nt);
+  // Visit order: slabs of zb planes, y-rows within a slab, then the planes of
+  // the slab, then the x-segments.  Odd steps write a row's slots from the
+  // row itself (bounce-back) and from its upstream rows; keeping z-neighbour
+  // rows a few tiles apart lets one XCD's L2 merge those writes into whole
+  // lines before they reach memory.  That pays for 16-bit storage (a 64-cell
+  // tile is one 128-byte line per plane); with fp32 the plain row order
+  // (zb = 1) streams each plane more sequentially and measures faster.
+  uint32_t zb = fl->storage == kF32 ? 1 : 16;
+  if (const char* e = getenv("LBM_TILE_ZBLOCK")) zb = (uint32_t)std::max(1, atoi(e));
   std::vector<uint32_t> list;
   list.reserve(nt);
-  for (size_t t = 0; t < nt; ++t)
-    if (flags[t]) list.push_back((uint32_t)t);
+  const uint32_t ny = fl->d.ny, nz = fl->d.nz;
+  for (uint32_t z0 = 0; z0 < nz; z0 += zb)
+    for (uint32_t y = 0; y < ny; ++y)
+      for (uint32_t z = z0; z < std::min(nz, z0 + zb); ++z)
+        for (uint32_t sg = 0; sg < nseg; ++sg) {
+          const size_t t = ((size_t)z * ny + y) * nseg + sg;
+          if (flags[t]) list.push_back((uint32_t)t);
+        }
   if (list.size() > fl->tiles_cap) {
     dev_free(fl, fl->tiles, fl->tiles_cap * 4);
     fl->tiles = nullptr;
@@ -201,6 +228,7 @@ int rebuild_tiles(lbm_fields* fl) {
     HIPCHK(fl->ctx, hipStreamSynchronize(s));
   }
   fl->ntiles = (uint32_t)list.size();
+  fl->gen++;
   return LBM_OK;
 }
 
@@ -445,8 +473,11 @@ void lbm_fields_free(lbm_fields* fl) {
   if (fl->ibm) ibm_destroy(fl->ibm);
   for (ScalarState* s : fl->scalars) scalar_destroy(s);
   for (hipEvent_t e : fl->ev_pool) (void)hipEventDestroy(e);
+  if (fl->gexec) (void)hipGraphExecDestroy(fl->gexec);
+  if (fl->graph) (void)hipGraphDestroy(fl->graph);
   for (void* p : {(void*)fl->f_alloc, (void*)fl->mask, (void*)fl->solid, (void*)fl->wall_u, (void*)fl->omega,
-                  (void*)fl->F_cur, (void*)fl->F_mom, (void*)fl->flist, (void*)fl->partials, (void*)fl->tiles})
+                  (void*)fl->F_cur, (void*)fl->F_mom, (void*)fl->flist, (void*)fl->partials, (void*)fl->tiles,
+                  (void*)fl->ring, (void*)fl->ring_pos})
     if (p) (void)hipFree(p);
   delete fl;
 }
@@ -522,6 +553,7 @@ int lbm_set_params(lbm_fields* fl, const lbm_step_params* p) {
   fl->hp = build_hp(*p);
   fl->collision = p->collision;
   fl->params_set = true;
+  fl->gen++;
   return LBM_OK;
 }
 
@@ -647,39 +679,147 @@ int lbm_read_force_field(lbm_fields* fl, float* ff, int* present) {
   return LBM_OK;
 }
 
-int lbm_run_span(lbm_fields* fl, int steps) {
-  if (!fl || steps < 0) return LBM_ERR_INVALID;
+static int check_steppable(lbm_fields* fl) {
   if (fl->open_face_cells)
     return fail(fl->ctx, LBM_ERR_UNSUPPORTED,
                 "%u fluid cells lie on a non-periodic domain face; open faces are not supported "
                 "(close the domain with solid rims or make the axis periodic)",
                 fl->open_face_cells);
+  return LBM_OK;
+}
+
+// One collide-stream step (main kernel, forced-cell list kernel) plus the
+// scalar distributions; launches only.
+static int hydro_step(lbm_fields* fl) {
   hipStream_t s = fl->ctx->stream;
-  for (int k = 0; k < steps; ++k) {
-    HydroArgs a = hydro_args(fl);
-    if (fl->timing) {
-      if (fl->ev_used + 2 > fl->ev_pool.size()) {
-        for (int k2 = 0; k2 < 2; ++k2) {
-          hipEvent_t e;
-          HIPCHK(fl->ctx, hipEventCreate(&e));
-          fl->ev_pool.push_back(e);
-        }
+  HydroArgs a = hydro_args(fl);
+  if (fl->timing) {
+    if (fl->ev_used + 2 > fl->ev_pool.size()) {
+      for (int k2 = 0; k2 < 2; ++k2) {
+        hipEvent_t e;
+        HIPCHK(fl->ctx, hipEventCreate(&e));
+        fl->ev_pool.push_back(e);
       }
-      HIPCHK(fl->ctx, hipEventRecord(fl->ev_pool[fl->ev_used], s));
     }
-    HIPCHK(fl->ctx, launch_hydro(fl->Q, fl->storage, fl->collision, fl->parity, a, s));
-    if (fl->timing) {
-      HIPCHK(fl->ctx, hipEventRecord(fl->ev_pool[fl->ev_used + 1], s));
-      fl->ev_used += 2;
-    }
-    if (a.nlist) HIPCHK(fl->ctx, launch_hydro_list(fl->Q, fl->storage, fl->collision, fl->parity, a, s));
-    fl->parity ^= 1;
-    fl->time += 1;
-    for (ScalarState* sc : fl->scalars) {
-      int rc = scalar_step(view_of(fl), sc);
-      if (rc) return rc;
-    }
+    HIPCHK(fl->ctx, hipEventRecord(fl->ev_pool[fl->ev_used], s));
   }
+  HIPCHK(fl->ctx, launch_hydro(fl->Q, fl->storage, fl->collision, fl->parity, a, s));
+  if (fl->timing) {
+    HIPCHK(fl->ctx, hipEventRecord(fl->ev_pool[fl->ev_used + 1], s));
+    fl->ev_used += 2;
+  }
+  if (a.nlist) HIPCHK(fl->ctx, launch_hydro_list(fl->Q, fl->storage, fl->collision, fl->parity, a, s));
+  fl->parity ^= 1;
+  fl->time += 1;
+  for (ScalarState* sc : fl->scalars) {
+    int rc = scalar_step(view_of(fl), sc);
+    if (rc) return rc;
+  }
+  return LBM_OK;
+}
+
+int lbm_run_span(lbm_fields* fl, int steps) {
+  if (!fl || steps < 0) return LBM_ERR_INVALID;
+  int rc = check_steppable(fl);
+  if (rc) return rc;
+  for (int k = 0; k < steps; ++k) {
+    rc = hydro_step(fl);
+    if (rc) return rc;
+  }
+  return LBM_OK;
+}
+
+// The runner's step (runner.rs:1079-1129): clear the force field, rotating
+// IBM, collide-stream, scalars.  Launches only; the IBM diagnostics go to the
+// device ring.
+static int ibm_step(lbm_fields* fl) {
+  if (!fl->flist_host.empty())
+    HIPCHK(fl->ctx, launch_copy_force_list(fl->flist, (uint32_t)fl->flist_host.size(), fl->F_cur, nullptr,
+                                           fl->ctx->stream));
+  fl->field_present = false;
+  fl->field_uniform = false;
+  int rc = ibm_enqueue(view_of(fl), fl->ibm, fl->ring, fl->ring_cap, fl->ring_pos);
+  if (rc) return rc;
+  fl->field_present = true;
+  return hydro_step(fl);
+}
+
+int lbm_run_ibm_span(lbm_fields* fl, int steps, int use_graph, double* torque, lbm_ibm_diag* last) {
+  if (!fl || steps < 0) return LBM_ERR_INVALID;
+  if (!fl->ibm) return fail(fl->ctx, LBM_ERR_STATE, "no IBM body registered");
+  int rc = check_steppable(fl);
+  if (rc) return rc;
+  if (steps == 0) return LBM_OK;
+  if (fl->field_present && fl->field_uniform) {  // the clear must refresh the cached moments
+    rc = lbm_clear_force_field(fl);
+    if (rc) return rc;
+  }
+  hipStream_t s = fl->ctx->stream;
+  if (fl->ring_cap < (uint32_t)steps) {
+    if (fl->ring) dev_free(fl, fl->ring, (size_t)fl->ring_cap * ibm_ring_entry_bytes());
+    fl->ring = nullptr;
+    fl->ring_cap = 0;
+    rc = dev_alloc(fl, &fl->ring, (size_t)steps * ibm_ring_entry_bytes());
+    if (rc) return rc;
+    fl->ring_cap = (uint32_t)steps;
+    fl->gen++;
+  }
+  if (!fl->ring_pos) {
+    rc = dev_alloc(fl, (void**)&fl->ring_pos, 256);
+    if (rc) return rc;
+  }
+  // a fill kernel, not a memset, on the stream the graph runs on
+  HIPCHK(fl->ctx, launch_fill_f32((float*)fl->ring_pos, 1, 0.0f, s));
+  int k = 0;
+  if (use_graph && !fl->timing && steps >= 2) {
+    const HydroArgs a = hydro_args(fl);
+    const bool stale = !fl->gexec || fl->g_parity != fl->parity || fl->g_gen != fl->gen ||
+                       memcmp(&a, &fl->g_args, sizeof a) != 0;
+    if (stale) {
+      if (fl->gexec) (void)hipGraphExecDestroy(fl->gexec);
+      if (fl->graph) (void)hipGraphDestroy(fl->graph);
+      fl->gexec = nullptr;
+      fl->graph = nullptr;
+      const int parity0 = fl->parity;
+      const int64_t time0 = fl->time;
+      HIPCHK(fl->ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      int rc1 = ibm_step(fl);
+      if (!rc1) rc1 = ibm_step(fl);
+      hipGraph_t g = nullptr;
+      const hipError_t e = hipStreamEndCapture(s, &g);
+      // capture records the launches without running them: two steps flip
+      // the parity (and the scalar buffers) back, the clock does not move
+      fl->time = time0;
+      if (rc1) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc1;
+      }
+      HIPCHK(fl->ctx, e);
+      if (fl->parity != parity0) return fail(fl->ctx, LBM_ERR_STATE, "captured step pair changed parity");
+      fl->graph = g;
+      HIPCHK(fl->ctx, hipGraphInstantiateWithFlags(&fl->gexec, g, 0));
+      fl->g_args = a;
+      fl->g_parity = fl->parity;
+      fl->g_gen = fl->gen;
+    }
+    for (; k + 2 <= steps; k += 2) {
+      HIPCHK(fl->ctx, hipGraphLaunch(fl->gexec, s));
+      fl->time += 2;
+    }
+    fl->field_present = true;
+    fl->field_uniform = false;
+  }
+  for (; k < steps; ++k) {
+    rc = ibm_step(fl);
+    if (rc) return rc;
+  }
+  std::vector<lbm_ibm_diag> d(steps);
+  rc = ibm_ring_read(view_of(fl), fl->ring, (uint32_t)steps, d.data());
+  if (rc) return rc;
+  if (torque)
+    for (int j = 0; j < steps; ++j)
+      for (int c = 0; c < 3; ++c) torque[3 * (size_t)j + c] = d[j].torque[c];
+  if (last) *last = d[steps - 1];
   return LBM_OK;
 }
 
@@ -876,6 +1016,7 @@ int lbm_ibm_set_body(lbm_fields* fl, int64_t n, const double* pos, const double*
   IbmState* st = ibm_create(view_of(fl), n, pos, weight, center, omega, c, &cells, &err);
   if (!st) return fail(fl->ctx, LBM_ERR_INVALID, "%s", err.c_str());
   fl->ibm = st;
+  fl->gen++;
   int rc = ensure_force_arrays(fl);
   if (rc) return rc;
   return grow_force_list(fl, cells);
@@ -902,6 +1043,7 @@ int lbm_scalar_enable(lbm_fields* fl, double diffusivity, int lattice, int* inde
   ScalarState* s = scalar_create(view_of(fl), diffusivity, lattice == LBM_SCALAR_D3Q7 ? 7 : fl->Q, &err);
   if (!s) return fail(fl->ctx, LBM_ERR_NOMEM, "%s", err.c_str());
   fl->scalars.push_back(s);
+  fl->gen++;
   *index = (int)fl->scalars.size() - 1;
   return LBM_OK;
 }

@@ -430,11 +430,35 @@ IbmState* ibm_create(const FieldsView& v, int64_t n, const double* pos, const do
 
 template <int Q, typename S>
 static void gather_q(const FieldsView& v, IbmState* s) {
-  hipLaunchKernelGGL((k_ibm_gather<Q, S>), dim3(nblk(s->U)), dim3(256), 0, v.stream, v.d, (const S*)v.f, v.mask, v.parity,
-                     s->uc_cell, s->U, v.F_cur, v.field_present ? 1 : 0, v.hp, s->unow, s->rho, s->du, s->spread);
+  hipLaunchKernelGGL((k_ibm_gather<Q, S>), dim3(nblk(s->U)), dim3(256), 0, v.stream, v.d, (const S*)v.f, v.mask,
+                     v.parity, s->uc_cell, s->U, v.F_cur, v.field_present ? 1 : 0, v.hp, s->unow, s->rho, s->du,
+                     s->spread);
 }
 
-int ibm_apply(const FieldsView& v, IbmState* s, lbm_ibm_diag* out) {
+// one thread: append the current diagnostics to a device ring (read back
+// after a span of steps, so the step loop needs no host synchronisation)
+__global__ void k_ibm_record(const IbmDevDiag* diag, IbmDevDiag* ring, uint32_t cap, uint32_t* pos) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const uint32_t k = *pos;
+  if (k < cap) ring[k] = *diag;
+  *pos = k + 1;
+}
+
+static void diag_to_abi(const IbmDevDiag& h, lbm_ibm_diag* out) {
+  out->iterations = h.iterations;
+  out->slip_max = h.slip_max;
+  out->slip_rms = h.slip_rms;
+  out->slip_max_rel = h.slip_max_rel;
+  out->slip_rms_rel = h.slip_rms_rel;
+  for (int a = 0; a < 3; ++a) {
+    out->torque[a] = h.torque[a];
+    out->fluid_force[a] = h.fluid_force[a];
+    out->marker_force[a] = h.marker_force[a];
+  }
+  out->momentum_error_rel = h.mom_err;
+}
+
+int ibm_enqueue(const FieldsView& v, IbmState* s, void* ring, uint32_t cap, uint32_t* pos) {
   if (!v.F_cur) {
     if (v.ctx_err) *v.ctx_err = "force arrays missing";
     return LBM_ERR_STATE;
@@ -472,20 +496,32 @@ int ibm_apply(const FieldsView& v, IbmState* s, lbm_ibm_diag* out) {
     hipLaunchKernelGGL(k_ibm_commit, dim3(gu), dim3(256), 0, st, s->U, s->uc_cell, s->spread, v.F_cur);
     AUXCHK(v, hipGetLastError());
   }
-  IbmDevDiag h{};
-  AUXCHK(v, hipMemcpyAsync(&h, s->diag, sizeof h, hipMemcpyDeviceToHost, st));
-  AUXCHK(v, hipStreamSynchronize(st));
-  out->iterations = h.iterations;
-  out->slip_max = h.slip_max;
-  out->slip_rms = h.slip_rms;
-  out->slip_max_rel = h.slip_max_rel;
-  out->slip_rms_rel = h.slip_rms_rel;
-  for (int a = 0; a < 3; ++a) {
-    out->torque[a] = h.torque[a];
-    out->fluid_force[a] = h.fluid_force[a];
-    out->marker_force[a] = h.marker_force[a];
+  if (ring) {
+    hipLaunchKernelGGL(k_ibm_record, dim3(1), dim3(64), 0, st, s->diag, (IbmDevDiag*)ring, cap, pos);
+    AUXCHK(v, hipGetLastError());
   }
-  out->momentum_error_rel = h.mom_err;
+  return LBM_OK;
+}
+
+int ibm_apply(const FieldsView& v, IbmState* s, lbm_ibm_diag* out) {
+  int rc = ibm_enqueue(v, s, nullptr, 0, nullptr);
+  if (rc) return rc;
+  IbmDevDiag h{};
+  AUXCHK(v, hipMemcpyAsync(&h, s->diag, sizeof h, hipMemcpyDeviceToHost, v.stream));
+  AUXCHK(v, hipStreamSynchronize(v.stream));
+  diag_to_abi(h, out);
+  return LBM_OK;
+}
+
+size_t ibm_ring_entry_bytes() { return sizeof(IbmDevDiag); }
+
+int ibm_ring_read(const FieldsView& v, const void* ring, uint32_t n, lbm_ibm_diag* out) {
+  std::vector<IbmDevDiag> h(n);
+  if (n) {
+    AUXCHK(v, hipMemcpyAsync(h.data(), ring, n * sizeof(IbmDevDiag), hipMemcpyDeviceToHost, v.stream));
+    AUXCHK(v, hipStreamSynchronize(v.stream));
+  }
+  for (uint32_t k = 0; k < n; ++k) diag_to_abi(h[k], out + k);
   return LBM_OK;
 }
 

@@ -36,6 +36,12 @@ IbmState* ibm_create(const FieldsView& v, int64_t n, const double* pos, const do
                      std::string* err);
 void ibm_destroy(IbmState* s);
 int ibm_apply(const FieldsView& v, IbmState* s, lbm_ibm_diag* diag);
+// The same update without the host read-back: the diagnostics go to entry
+// *pos of a device ring of `cap` entries (ibm_ring_entry_bytes each) and
+// *pos advances.  Launches kernels only, so a stream capture can record it.
+int ibm_enqueue(const FieldsView& v, IbmState* s, void* ring, uint32_t cap, uint32_t* pos);
+size_t ibm_ring_entry_bytes();
+int ibm_ring_read(const FieldsView& v, const void* ring, uint32_t n, lbm_ibm_diag* out);
 
 // Passive scalar ADE (Solver::step_scalar_distributions, solver.rs:2113-2205).
 ScalarState* scalar_create(const FieldsView& v, double diffusivity, int Qs, std::string* err);

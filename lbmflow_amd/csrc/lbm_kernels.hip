@@ -486,16 +486,11 @@ __device__ __forceinline__ void cell_step(const HydroArgs& a, uint32_t i, uint32
   // cell instead of two.
 #pragma unroll
   for (int q = 0; q < Q; ++q) asm volatile("" : "+v"(fv[q]));
-  if constexpr (FILL && !ODD) {
-    if (m & kMaskSolid) {  // dead slots: rewrite them so the tile's lines are whole
-      sfor<0, Q>([&](auto qc) {
-        constexpr int q = decltype(qc)::value;
-        slot_st<S, Q, q>(sl.template at<lat_opp(q), 0, 0, 0>(), fv[q]);
-      });
-      return;
-    }
-  }
-  if (m & skip) return;
+  // Even steps: a solid cell of an active tile runs along with its wave and
+  // stores zeros into its own (dead) slots, so the tile's stores are whole
+  // cache lines issued by the same instructions.
+  const bool dead = FILL && !ODD && (m & kMaskSolid) != 0;
+  if (m & skip & (dead ? ~kMaskSolid : ~0u)) return;
   if constexpr (ODD) {
     if (m & kMaskLinks) {  // wall links: the bounced population is in the own slot
       sfor<1, Q>([&](auto qc) {
@@ -546,7 +541,7 @@ __device__ __forceinline__ void cell_step(const HydroArgs& a, uint32_t i, uint32
     constexpr int q = decltype(qc)::value;
     constexpr int o = lat_opp(q);
     if constexpr (!ODD)
-      slot_st<S, Q, q>(sl.template at<o, 0, 0, 0>(), fv[q]);
+      slot_st<S, Q, q>(sl.template at<o, 0, 0, 0>(), dead ? 0.0f : fv[q]);
     else
       slot_st<S, Q, q>(sl.template at<q, lat_c(q, 0), lat_c(q, 1), lat_c(q, 2)>(), fv[q]);
   };

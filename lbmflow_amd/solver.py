@@ -106,6 +106,15 @@ class DirectForcingConfig:
     relaxation: float = 1.0
 
 
+def _diag_dict(d):
+    return {
+        "iterations": d.iterations, "slip_max": d.slip_max, "slip_rms": d.slip_rms,
+        "slip_max_rel": d.slip_max_rel, "slip_rms_rel": d.slip_rms_rel, "torque": np.array(d.torque[:]),
+        "fluid_force": np.array(d.fluid_force[:]), "marker_force": np.array(d.marker_force[:]),
+        "momentum_error_rel": d.momentum_error_rel,
+    }
+
+
 class Solver:
     """A monolithic domain on one GPU (``Solver<L, f32, HipBackend, LocalPeriodic>``)."""
 
@@ -299,12 +308,18 @@ class Solver:
         """``Solver::apply_rotating_ibm`` (solver.rs:3016-3282) for the registered body."""
         d = _L.IbmDiag()
         self._chk(self.lib.lbm_ibm_apply(self.h, C.byref(d)))
-        return {
-            "iterations": d.iterations, "slip_max": d.slip_max, "slip_rms": d.slip_rms,
-            "slip_max_rel": d.slip_max_rel, "slip_rms_rel": d.slip_rms_rel, "torque": np.array(d.torque[:]),
-            "fluid_force": np.array(d.fluid_force[:]), "marker_force": np.array(d.marker_force[:]),
-            "momentum_error_rel": d.momentum_error_rel,
-        }
+        return _diag_dict(d)
+
+    def run_with_ibm(self, steps: int, graph: bool = True):
+        """``steps`` runner steps (runner.rs:1079-1129: clear the force field,
+        rotating IBM, collide-stream, scalars) without host synchronisation
+        inside the span.  Returns (torque[steps, 3], diagnostics of the last
+        step).  ``graph`` replays pairs of steps from one captured HIP graph."""
+        steps = int(steps)
+        tq = np.zeros((max(steps, 0), 3))
+        d = _L.IbmDiag()
+        self._chk(self.lib.lbm_run_ibm_span(self.h, steps, int(bool(graph)), _dp(tq), C.byref(d)))
+        return tq, (_diag_dict(d) if steps > 0 else None)
 
     # ---- scalar distributions ----------------------------------------------
     def enable_scalar_distribution(self, diffusivity: float, lattice: str = "hydro") -> int:

@@ -100,7 +100,7 @@ def tank_case(n, *, nu=0.01, tip=0.05, collision="cm", Q=27, sigma=None):
             "c": c, "blob": blob, "collision": collision, "Q": Q}
 
 
-def run_tank_oracle(tc, steps, prec=4, every=1, scalar_D=None, d3q7=True):
+def run_tank_oracle(tc, steps, prec=4, every=1, scalar_D=None, d3q7=True, cv_every=None):
     s = O.OracleSolver(tc["Q"], (tc["n"],) * 3, prec=prec, collision=COLL[tc["collision"]], nu=tc["nu"],
                        solid=tc["solid"])
     k = None
@@ -115,12 +115,12 @@ def run_tank_oracle(tc, steps, prec=4, every=1, scalar_D=None, d3q7=True):
         s.step()
         if step % every == 0:
             torque.append(-d["torque"][2])
-            if k is not None:
-                cv.append(O.scalar_cv(s.scalar_get(k), fl))
+        if k is not None and step % (cv_every or every) == 0:
+            cv.append(O.scalar_cv(s.scalar_get(k), fl))
     return np.array(torque), np.array(cv), s
 
 
-def run_tank_gpu(tc, steps, storage="f32", every=1, scalar_D=None, d3q7=True):
+def run_tank_gpu(tc, steps, storage="f32", every=1, scalar_D=None, d3q7=True, cv_every=None):
     import lbmflow_amd as L
 
     n = tc["n"]
@@ -137,8 +137,8 @@ def run_tank_gpu(tc, steps, storage="f32", every=1, scalar_D=None, d3q7=True):
         s.step()
         if step % every == 0:
             torque.append(-d["torque"][2])
-            if k is not None:
-                cv.append(s.scalar_cv(k))
+        if k is not None and step % (cv_every or every) == 0:
+            cv.append(s.scalar_cv(k))
     return np.array(torque), np.array(cv), s
 
 

@@ -1,0 +1,53 @@
+"""Generate the FP64 reference series for the storage-precision test.
+
+Runs the oracle (the CPU restatement of the reference path, in f64) on the
+128^3 stirred tank at the benchmark's ratios (T = H = 120 lu, 4 baffles
+T/10 x T/48, 6-blade Rushton D = C = T/3, tip speed 0.05 lu, Re_imp = 5000)
+with a D3Q7 tracer (Gaussian sigma = 2 lu at 0.8 H on the axis, Sc = 1),
+and stores the shaft torque of every step and the tracer CV every 10 steps.
+Restatement-generated: the reference ships no such series.
+
+usage: python tests/golden/make_tank128_f64.py [steps]   (about 0.5 s/step on 8 cores)
+"""
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+sys.path.insert(0, os.path.dirname(HERE))
+
+import cases  # noqa: E402
+
+N = 128
+EVERY_CV = 10
+
+
+def tank_params():
+    T = 120.0
+    D = T / 3.0
+    omega = 0.05 / (0.5 * D)
+    nu = (omega / (2.0 * np.pi)) * D * D / 5000.0
+    return nu
+
+
+def main():
+    steps = int(sys.argv[1]) if len(sys.argv) > 1 else 3000
+    nu = tank_params()
+    tc = cases.tank_case(N, nu=nu, collision="cm", Q=27, sigma=2.0)
+    t0 = time.time()
+    torque, cv, s = cases.run_tank_oracle(tc, steps, prec=8, every=1, scalar_D=nu, d3q7=True,
+                                          cv_every=EVERY_CV)
+    meta = {"n": N, "steps": steps, "nu": nu, "omega": tc["omega"], "D": tc["D"], "T": tc["T"], "sigma": 2.0,
+            "scalar": "d3q7", "cv_every": EVERY_CV, "collision": "cm", "Q": 27, "precision": "f64",
+            "seconds": round(time.time() - t0, 1)}
+    out = os.path.join(HERE, "tank128_cm_f64.npz")
+    np.savez_compressed(out, torque=torque, cv=cv, meta=np.frombuffer(json.dumps(meta).encode(), np.uint8))
+    print(out, meta)
+
+
+if __name__ == "__main__":
+    main()
