@@ -230,6 +230,30 @@ int lbm_ibm_apply(lbm_fields* fl, lbm_ibm_diag* diag);
  * / finish_run_chunk (backend.rs:340-349). */
 int lbm_run_ibm_span(lbm_fields* fl, int steps, int use_graph, double* torque, lbm_ibm_diag* last);
 
+/* ---- batched domains ------------------------------------------------------
+ * No single reference counterpart: lbm-cli's sweep runs its cases one after
+ * another through the runner (crates/lbm-cli/src/sweep.rs:61-106, grid
+ * expansion crates/lbm-scenario/src/sweep.rs:52-71).  A batch steps several
+ * independent closed domains (same lattice, storage and collision kind; each
+ * with its own grid, masks, rates and rotating body) together: one persistent
+ * collide-stream launch over their concatenated tile lists, one forced-cell
+ * launch, one set of IBM launches for all bodies.  Members must not be
+ * changed while a batch over them exists; the batch checks this. */
+typedef struct lbm_batch lbm_batch;
+int lbm_batch_create(lbm_ctx* ctx, lbm_fields* const* members, int n, lbm_batch** out);
+void lbm_batch_destroy(lbm_batch* b);
+int lbm_batch_info(const lbm_batch* b, int* members, uint32_t* tiles, uint32_t* forced_cells, uint32_t* grid);
+/* Persistent grid (workgroups of 256) of the collide-stream launch. */
+int lbm_batch_set_grid(lbm_batch* b, uint32_t grid);
+/* `steps` runner steps of every member (as lbm_run_ibm_span, or plain
+ * collide-stream steps when the members have no rotating body).  torque
+ * (nullable) receives members x steps x 3 values, member-major. */
+int lbm_batch_run_span(lbm_batch* b, int steps, int use_graph, double* torque);
+/* Times the collide-stream launch over a set of persistent grid sizes on
+ * scratch copies of the populations (the members' state is untouched) and
+ * keeps the fastest. */
+int lbm_batch_autotune(lbm_batch* b, int trial_steps, uint32_t* best_grid, double* best_ms);
+
 /* ---- passive scalar ADE (solver.rs:2113-2205, scalar.rs:6-15) ------------ */
 int lbm_scalar_enable(lbm_fields* fl, double diffusivity, int lattice, int* index);
 /* set_scalar_concentration, solver.rs:2910-2965 (h = h_eq(C, u)). */

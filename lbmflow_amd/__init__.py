@@ -6,6 +6,7 @@ reference's Solver/Backend interface over that ABI.
 """
 from .solver import (  # noqa: F401
     MAGIC_STD,
+    Batch,
     Context,
     DirectForcingConfig,
     Solver,
@@ -17,5 +18,5 @@ from .solver import (  # noqa: F401
 )
 from ._lib import LbmError, lib  # noqa: F401
 
-__all__ = ["Solver", "Context", "DirectForcingConfig", "omegas", "radial_impeller", "power_qois",
+__all__ = ["Solver", "Batch", "Context", "DirectForcingConfig", "omegas", "radial_impeller", "power_qois",
            "mixing_time_from_cv", "MAGIC_STD", "LbmError", "lib", "default_context"]

@@ -92,6 +92,12 @@ SIGNATURES = {
     "lbm_ibm_set_body": (_I, [_P, C.c_int64, _pD, _pD, _pD, _pD, C.POINTER(IbmConfig)]),
     "lbm_ibm_apply": (_I, [_P, C.POINTER(IbmDiag)]),
     "lbm_run_ibm_span": (_I, [_P, _I, _I, _pD, C.POINTER(IbmDiag)]),
+    "lbm_batch_create": (_I, [_P, C.POINTER(_P), _I, C.POINTER(_P)]),
+    "lbm_batch_destroy": (None, [_P]),
+    "lbm_batch_info": (_I, [_P, _pI, _pU32, _pU32, _pU32]),
+    "lbm_batch_set_grid": (_I, [_P, C.c_uint32]),
+    "lbm_batch_run_span": (_I, [_P, _I, _I, _pD]),
+    "lbm_batch_autotune": (_I, [_P, _I, _pU32, _pD]),
     "lbm_scalar_enable": (_I, [_P, _D, _I, _pI]),
     "lbm_scalar_set": (_I, [_P, _I, _pF]),
     "lbm_scalar_gather": (_I, [_P, _I, _pF]),

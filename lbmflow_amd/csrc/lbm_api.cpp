@@ -61,6 +61,7 @@ struct lbm_fields {
   uint32_t* tiles = nullptr;
   size_t tiles_cap = 0;
   uint32_t ntiles = 0;
+  std::vector<uint32_t> tiles_host;
   uint32_t* flist = nullptr;
   size_t flist_cap = 0;
   // reduction scratch
@@ -214,7 +215,7 @@ int rebuild_tiles(lbm_fields* fl) {
       for (uint32_t z = z0; z < std::min(nz, z0 + zb); ++z)
         for (uint32_t sg = 0; sg < nseg; ++sg) {
           const size_t t = ((size_t)z * ny + y) * nseg + sg;
-          if (flags[t]) list.push_back((uint32_t)t);
+          if (flags[t]) list.push_back((uint32_t)t | (flags[t] == 2 ? kTilePlain : 0u));
         }
   if (list.size() > fl->tiles_cap) {
     dev_free(fl, fl->tiles, fl->tiles_cap * 4);
@@ -228,6 +229,7 @@ int rebuild_tiles(lbm_fields* fl) {
     HIPCHK(fl->ctx, hipStreamSynchronize(s));
   }
   fl->ntiles = (uint32_t)list.size();
+  fl->tiles_host.swap(list);
   fl->gen++;
   return LBM_OK;
 }
@@ -754,6 +756,9 @@ int lbm_run_ibm_span(lbm_fields* fl, int steps, int use_graph, double* torque, l
     rc = lbm_clear_force_field(fl);
     if (rc) return rc;
   }
+  const IbmDomain D = ibm_domain_of(view_of(fl));
+  rc = ibm_set_domains(view_of(fl), fl->ibm, &D);
+  if (rc) return rc;
   hipStream_t s = fl->ctx->stream;
   if (fl->ring_cap < (uint32_t)steps) {
     if (fl->ring) dev_free(fl, fl->ring, (size_t)fl->ring_cap * ibm_ring_entry_bytes());
@@ -1025,7 +1030,10 @@ int lbm_ibm_set_body(lbm_fields* fl, int64_t n, const double* pos, const double*
 int lbm_ibm_apply(lbm_fields* fl, lbm_ibm_diag* diag) {
   if (!fl || !diag) return LBM_ERR_INVALID;
   if (!fl->ibm) return fail(fl->ctx, LBM_ERR_STATE, "no IBM body registered");
-  int rc = ibm_apply(view_of(fl), fl->ibm, diag);
+  const IbmDomain D = ibm_domain_of(view_of(fl));
+  int rc = ibm_set_domains(view_of(fl), fl->ibm, &D);
+  if (rc) return rc;
+  rc = ibm_apply(view_of(fl), fl->ibm, diag);
   if (rc) return rc;
   fl->field_present = true;
   fl->field_uniform = false;
@@ -1108,6 +1116,359 @@ int lbm_qoi_mixing_time(int64_t n, const double* t, const double* cv, double out
   out[0] = cv0;
   out[1] = t95;
   out[2] = t99;
+  return LBM_OK;
+}
+
+}  // extern "C"
+
+// ---- batched domains ---------------------------------------------------------
+// K independent stirred tanks stepped together (lbm-cli's sweep runs its
+// cases one after another through the runner, crates/lbm-cli/src/sweep.rs:
+// 61-106; here they share every launch).  The members keep their own
+// populations, masks, parameters and IBM bodies; the batch owns only the
+// concatenated tile and force lists, the device argument table, the joined
+// IBM state and the captured step graph.
+struct lbm_batch {
+  lbm_ctx* ctx = nullptr;
+  std::vector<lbm_fields*> mem;
+  std::vector<uint64_t> mem_gen;
+  int Q = 19, storage = kF32, collision = LBM_COLLISION_TRT;
+  HydroArgs* tab = nullptr;
+  uint32_t* ent = nullptr;  // 2 x nent: (tile entry, member)
+  uint32_t nent = 0;
+  uint32_t* cells = nullptr;  // 2 x ncells: (forced cell, member)
+  uint32_t ncells = 0;
+  IbmState* ibm = nullptr;
+  void* ring = nullptr;
+  uint32_t ring_cap = 0;
+  uint32_t* ring_pos = nullptr;
+  uint32_t grid = 0;  // persistent blocks of the collide-stream launch
+  int ncu = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  int g_parity = -1;
+  uint32_t g_grid = 0;
+};
+
+namespace {
+
+FieldsView batch_view(lbm_batch* b) {
+  FieldsView v = view_of(b->mem[0]);
+  v.ctx_err = &b->ctx->err;
+  v.field_present = false;
+  return v;
+}
+
+template <typename T>
+int batch_upload(lbm_batch* b, T** dst, const std::vector<T>& src) {
+  const size_t bytes = std::max<size_t>(1, src.size()) * sizeof(T);
+  HIPCHK(b->ctx, hipMalloc((void**)dst, bytes));
+  if (!src.empty())
+    HIPCHK(b->ctx, hipMemcpyAsync(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice, b->ctx->stream));
+  HIPCHK(b->ctx, hipStreamSynchronize(b->ctx->stream));
+  return LBM_OK;
+}
+
+int batch_check(lbm_batch* b) {
+  for (size_t k = 0; k < b->mem.size(); ++k)
+    if (b->mem[k]->gen != b->mem_gen[k] || b->mem[k]->parity != b->mem[0]->parity)
+      return fail(b->ctx, LBM_ERR_STATE, "batch member %zu changed after the batch was made; make a new batch", k);
+  return LBM_OK;
+}
+
+// one collide-stream step of every member (launches only)
+int batch_hydro(lbm_batch* b, const HydroArgs* tab, uint32_t grid) {
+  hipStream_t s = b->ctx->stream;
+  const int odd = b->mem[0]->parity;
+  HIPCHK(b->ctx, launch_hydro_batch(b->Q, b->storage, b->collision, odd, tab, b->ent, b->nent, grid, s));
+  HIPCHK(b->ctx, launch_hydro_list_batch(b->Q, b->storage, b->collision, odd, tab, b->cells, b->ncells, s));
+  for (lbm_fields* fl : b->mem) {
+    fl->parity ^= 1;
+    fl->time += 1;
+  }
+  return LBM_OK;
+}
+
+// the runner's step for every member: clear, rotating IBM, collide-stream
+int batch_step(lbm_batch* b) {
+  hipStream_t s = b->ctx->stream;
+  if (b->ibm) {
+    HIPCHK(b->ctx, launch_clear_batch(b->tab, b->cells, b->ncells, s));
+    int rc = ibm_enqueue(batch_view(b), b->ibm, b->ring, b->ring_cap, b->ring_pos);
+    if (rc) return rc;
+  }
+  return batch_hydro(b, b->tab, b->grid);
+}
+
+}  // namespace
+
+extern "C" {
+
+int lbm_batch_create(lbm_ctx* ctx, lbm_fields* const* members, int n, lbm_batch** out) {
+  if (!ctx || !members || n <= 0 || !out) return LBM_ERR_INVALID;
+  *out = nullptr;
+  lbm_fields* f0 = members[0];
+  if (!f0) return LBM_ERR_INVALID;
+  int with_ibm = -1;
+  for (int k = 0; k < n; ++k) {
+    lbm_fields* fl = members[k];
+    if (!fl || fl->ctx != ctx) return fail(ctx, LBM_ERR_INVALID, "batch member %d is null or on another context", k);
+    for (int j = 0; j < k; ++j)
+      if (members[j] == fl) return fail(ctx, LBM_ERR_INVALID, "batch member %d appears twice", k);
+    if (fl->Q != f0->Q || fl->storage != f0->storage || fl->collision != f0->collision)
+      return fail(ctx, LBM_ERR_INVALID, "batch members must share lattice, storage and collision kind");
+    if (fl->d.per[0] || fl->d.per[1] || fl->d.per[2])
+      return fail(ctx, LBM_ERR_UNSUPPORTED, "batch member %d has a periodic axis; batches hold closed domains", k);
+    if (fl->open_face_cells) return fail(ctx, LBM_ERR_UNSUPPORTED, "batch member %d has fluid on a face", k);
+    if (!fl->scalars.empty())
+      return fail(ctx, LBM_ERR_UNSUPPORTED, "batch member %d carries scalar distributions", k);
+    if (fl->parity != f0->parity) return fail(ctx, LBM_ERR_STATE, "batch members must be at the same step parity");
+    if (fl->hp.force_on || fl->omega)
+      return fail(ctx, LBM_ERR_UNSUPPORTED, "batch member %d has a body force or an omega field", k);
+    if (fl->field_present && fl->field_uniform && !fl->ibm)
+      return fail(ctx, LBM_ERR_UNSUPPORTED, "batch member %d has a uniform force field", k);
+    const int has = fl->ibm ? 1 : 0;
+    if (with_ibm >= 0 && has != with_ibm)
+      return fail(ctx, LBM_ERR_INVALID, "either every batch member has a rotating body or none");
+    with_ibm = has;
+  }
+  lbm_batch* b = new (std::nothrow) lbm_batch();
+  if (!b) return LBM_ERR_NOMEM;
+  b->ctx = ctx;
+  b->Q = f0->Q;
+  b->storage = f0->storage;
+  b->collision = f0->collision;
+  int rc = LBM_OK;
+  for (int k = 0; k < n && !rc; ++k) {
+    lbm_fields* fl = members[k];
+    b->mem.push_back(fl);
+    // a uniform clear must refresh moments; do it before the members freeze
+    if (fl->field_present && fl->field_uniform) rc = lbm_clear_force_field(fl);
+  }
+  std::vector<HydroArgs> tab;
+  std::vector<uint32_t> ent, cells;
+  for (int k = 0; k < n && !rc; ++k) {
+    lbm_fields* fl = members[k];
+    tab.push_back(hydro_args(fl));
+    for (uint32_t te : fl->tiles_host) {
+      ent.push_back(te);
+      ent.push_back((uint32_t)k);
+    }
+    for (uint32_t c : fl->flist_host) {
+      cells.push_back(c);
+      cells.push_back((uint32_t)k);
+    }
+  }
+  if (!rc) rc = batch_upload(b, &b->tab, tab);
+  if (!rc) rc = batch_upload(b, &b->ent, ent);
+  if (!rc) rc = batch_upload(b, &b->cells, cells);
+  b->nent = (uint32_t)(ent.size() / 2);
+  b->ncells = (uint32_t)(cells.size() / 2);
+  if (!rc && with_ibm == 1) {
+    std::vector<const IbmState*> parts;
+    std::vector<IbmDomain> doms;
+    for (lbm_fields* fl : b->mem) {
+      parts.push_back(fl->ibm);
+      doms.push_back(ibm_domain_of(view_of(fl)));
+    }
+    std::string err;
+    b->ibm = ibm_concat(parts, &err);
+    if (!b->ibm) rc = fail(ctx, LBM_ERR_INVALID, "%s", err.c_str());
+    if (!rc) rc = ibm_set_domains(batch_view(b), b->ibm, doms.data());
+  }
+  if (!rc) {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, ctx->device) != hipSuccess) prop.multiProcessorCount = 256;
+    b->ncu = prop.multiProcessorCount;
+    // default: every entry's wave resident at once, capped at 8 blocks per CU
+    b->grid = std::max<uint32_t>(1, std::min<uint32_t>((b->nent + 3) / 4, (uint32_t)b->ncu * 8));
+  }
+  for (lbm_fields* fl : b->mem) b->mem_gen.push_back(fl->gen);
+  if (rc) {
+    lbm_batch_destroy(b);
+    return rc;
+  }
+  *out = b;
+  return LBM_OK;
+}
+
+void lbm_batch_destroy(lbm_batch* b) {
+  if (!b) return;
+  (void)hipStreamSynchronize(b->ctx->stream);
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->ibm) ibm_destroy(b->ibm);
+  for (void* p : {(void*)b->tab, (void*)b->ent, (void*)b->cells, b->ring, (void*)b->ring_pos})
+    if (p) (void)hipFree(p);
+  delete b;
+}
+
+int lbm_batch_info(const lbm_batch* b, int* members, uint32_t* tiles, uint32_t* forced_cells, uint32_t* grid) {
+  if (!b) return LBM_ERR_INVALID;
+  if (members) *members = (int)b->mem.size();
+  if (tiles) *tiles = b->nent;
+  if (forced_cells) *forced_cells = b->ncells;
+  if (grid) *grid = b->grid;
+  return LBM_OK;
+}
+
+int lbm_batch_set_grid(lbm_batch* b, uint32_t grid) {
+  if (!b || grid == 0) return LBM_ERR_INVALID;
+  b->grid = grid;
+  return LBM_OK;
+}
+
+int lbm_batch_run_span(lbm_batch* b, int steps, int use_graph, double* torque) {
+  if (!b || steps < 0) return LBM_ERR_INVALID;
+  int rc = batch_check(b);
+  if (rc) return rc;
+  if (steps == 0) return LBM_OK;
+  const size_t K = b->mem.size();
+  hipStream_t s = b->ctx->stream;
+  if (b->ibm && b->ring_cap < (uint32_t)steps) {
+    if (b->ring) (void)hipFree(b->ring);
+    b->ring = nullptr;
+    b->ring_cap = 0;
+    HIPCHK(b->ctx, hipMalloc(&b->ring, (size_t)steps * K * ibm_ring_entry_bytes()));
+    b->ring_cap = (uint32_t)steps;
+    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->gexec = nullptr;
+    b->graph = nullptr;
+  }
+  if (b->ibm && !b->ring_pos) HIPCHK(b->ctx, hipMalloc((void**)&b->ring_pos, 256));
+  if (b->ibm) HIPCHK(b->ctx, launch_fill_f32((float*)b->ring_pos, 1, 0.0f, s));
+  int k = 0;
+  if (use_graph && steps >= 2) {
+    const int parity = b->mem[0]->parity;
+    if (!b->gexec || b->g_parity != parity || b->g_grid != b->grid) {
+      if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+      if (b->graph) (void)hipGraphDestroy(b->graph);
+      b->gexec = nullptr;
+      b->graph = nullptr;
+      std::vector<int64_t> t0;
+      for (lbm_fields* fl : b->mem) t0.push_back(fl->time);
+      HIPCHK(b->ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      int rc1 = batch_step(b);
+      if (!rc1) rc1 = batch_step(b);
+      hipGraph_t g = nullptr;
+      const hipError_t e = hipStreamEndCapture(s, &g);
+      for (size_t j = 0; j < K; ++j) b->mem[j]->time = t0[j];  // capture ran nothing
+      if (rc1) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc1;
+      }
+      HIPCHK(b->ctx, e);
+      b->graph = g;
+      HIPCHK(b->ctx, hipGraphInstantiateWithFlags(&b->gexec, g, 0));
+      b->g_parity = parity;
+      b->g_grid = b->grid;
+    }
+    for (; k + 2 <= steps; k += 2) {
+      HIPCHK(b->ctx, hipGraphLaunch(b->gexec, s));
+      for (lbm_fields* fl : b->mem) fl->time += 2;
+    }
+  }
+  for (; k < steps; ++k) {
+    rc = batch_step(b);
+    if (rc) return rc;
+  }
+  for (lbm_fields* fl : b->mem) {
+    if (b->ibm) {
+      fl->field_present = true;
+      fl->field_uniform = false;
+    }
+  }
+  if (b->ibm) {
+    std::vector<lbm_ibm_diag> d((size_t)steps * K);
+    rc = ibm_ring_read(batch_view(b), b->ring, (uint32_t)(steps * K), d.data());
+    if (rc) return rc;
+    if (torque)
+      for (size_t j = 0; j < K; ++j)
+        for (int t = 0; t < steps; ++t)
+          for (int c = 0; c < 3; ++c) torque[(j * steps + t) * 3 + c] = d[(size_t)t * K + j].torque[c];
+  } else {
+    HIPCHK(b->ctx, hipStreamSynchronize(s));
+  }
+  return LBM_OK;
+}
+
+int lbm_batch_autotune(lbm_batch* b, int trial_steps, uint32_t* best_grid, double* best_ms) {
+  if (!b || trial_steps < 2) return LBM_ERR_INVALID;
+  int rc = batch_check(b);
+  if (rc) return rc;
+  hipStream_t s = b->ctx->stream;
+  const size_t K = b->mem.size();
+  // trial copies of the populations: the members' state is left untouched
+  std::vector<void*> copies(K, nullptr);
+  std::vector<HydroArgs> tab;
+  auto release = [&]() {
+    for (void* p : copies)
+      if (p) (void)hipFree(p);
+  };
+  for (size_t j = 0; j < K; ++j) {
+    lbm_fields* fl = b->mem[j];
+    if (hipMalloc(&copies[j], fl->f_bytes) != hipSuccess) {
+      release();
+      return fail(b->ctx, LBM_ERR_NOMEM, "autotune copy of member %zu", j);
+    }
+    if (hipMemcpyAsync(copies[j], fl->f_alloc, fl->f_bytes, hipMemcpyDeviceToDevice, s) != hipSuccess) {
+      release();
+      return fail(b->ctx, LBM_ERR_DEVICE, "autotune copy of member %zu", j);
+    }
+    HydroArgs a = hydro_args(fl);
+    a.f = (char*)copies[j] + ((char*)fl->f - (char*)fl->f_alloc);
+    tab.push_back(a);
+  }
+  HydroArgs* dtab = nullptr;
+  if (hipMalloc((void**)&dtab, K * sizeof(HydroArgs)) != hipSuccess ||
+      hipMemcpyAsync(dtab, tab.data(), K * sizeof(HydroArgs), hipMemcpyHostToDevice, s) != hipSuccess) {
+    release();
+    if (dtab) (void)hipFree(dtab);
+    return fail(b->ctx, LBM_ERR_NOMEM, "autotune table");
+  }
+  hipEvent_t e0, e1;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  const uint32_t full = std::max<uint32_t>(1, (b->nent + 3) / 4);
+  std::vector<uint32_t> cand;
+  for (uint32_t per_cu : {1u, 2u, 4u, 8u, 16u}) cand.push_back(std::min(full, (uint32_t)b->ncu * per_cu));
+  cand.push_back(full);
+  std::sort(cand.begin(), cand.end());
+  cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+  const int parity0 = b->mem[0]->parity;
+  double best = 1e300;
+  uint32_t bg = b->grid;
+  for (uint32_t g : cand) {
+    hipError_t err = hipSuccess;
+    for (int w = 0; w < 2 && err == hipSuccess; ++w)
+      err = launch_hydro_batch(b->Q, b->storage, b->collision, w & 1, dtab, b->ent, b->nent, g, s);
+    if (err == hipSuccess) err = hipEventRecord(e0, s);
+    for (int t = 0; t < trial_steps && err == hipSuccess; ++t)
+      err = launch_hydro_batch(b->Q, b->storage, b->collision, (parity0 + t) & 1, dtab, b->ent, b->nent, g, s);
+    if (err == hipSuccess) err = hipEventRecord(e1, s);
+    if (err == hipSuccess) err = hipEventSynchronize(e1);
+    float ms = 0.0f;
+    if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+    if (err != hipSuccess) {
+      rc = fail(b->ctx, LBM_ERR_DEVICE, "autotune trial: %s", hipGetErrorString(err));
+      break;
+    }
+    const double per = ms / trial_steps;
+    if (per < best) {
+      best = per;
+      bg = g;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipStreamSynchronize(s);
+  (void)hipFree(dtab);
+  release();
+  if (rc) return rc;
+  b->grid = bg;
+  if (best_grid) *best_grid = bg;
+  if (best_ms) *best_ms = best;
   return LBM_OK;
 }
 

@@ -45,6 +45,10 @@ inline uint32_t nblk(size_t n, uint32_t bs = 256) { return (uint32_t)((n + bs - 
 // ===========================================================================
 // rotating IBM
 // ===========================================================================
+// A device IBM state holds K bodies, one per domain ("instance"), with every
+// array concatenated instance after instance; a single domain is K = 1.
+// Markers and stencil cells carry their instance, the per-instance
+// reductions run one block per instance, and the early exit is per instance.
 struct IbmDevDiag {
   int iterations;
   int converged;
@@ -53,21 +57,40 @@ struct IbmDevDiag {
   double mom_err;
 };
 
+// host description of one body (stencils built once: markers are frozen)
+struct IbmHost {
+  int64_t M = 0;
+  double center[3], omega[3], scale = 1.0;
+  std::vector<double> pos, weight, target;
+  std::vector<uint32_t> mk_off, pt_u, pt_m, uc_cell, uc_off, uc_ent;
+  std::vector<double> pt_w, ksum;
+};
+
 struct IbmState {
+  int K = 0;
   int64_t M = 0;
   uint32_t P = 0, U = 0;
   lbm_ibm_config cfg{};
-  double center[3], omega[3], scale = 1.0;
+  IbmHost host;  // K == 1: the body this state was built from
+  // per instance
+  uint32_t* moff = nullptr;  // K + 1 marker offsets
+  double* cen = nullptr;     // K x 3
+  double* scale = nullptr;   // K
+  IbmDomain* dom = nullptr;  // K
+  std::vector<IbmDomain> dom_h;
+  bool dom_valid = false;
   // marker-major
-  double* pos = nullptr;     // M x 3
-  double* weight = nullptr;  // M
-  double* target = nullptr;  // M x 3
+  double* pos = nullptr;       // M x 3
+  double* weight = nullptr;    // M
+  double* target = nullptr;    // M x 3
   uint32_t* mk_off = nullptr;  // M + 1
+  uint32_t* mk_inst = nullptr; // M
   uint32_t* pt_u = nullptr;    // P: unique-cell index
   uint32_t* pt_m = nullptr;    // P: marker index
   double* pt_w = nullptr;      // P
   // cell-major
-  uint32_t* uc_cell = nullptr;  // U
+  uint32_t* uc_cell = nullptr;  // U: cell index within its instance
+  uint32_t* uc_inst = nullptr;  // U
   uint32_t* uc_off = nullptr;   // U + 1
   uint32_t* uc_ent = nullptr;   // P, marker order within a cell
   double* ksum = nullptr;       // U
@@ -81,23 +104,25 @@ struct IbmState {
   double* mtq = nullptr;     // M x 3
   double* mslip = nullptr;   // M
   uint8_t* mact = nullptr;   // M
-  IbmDevDiag* diag = nullptr;
+  IbmDevDiag* diag = nullptr;  // K
 };
 
 // u_now = (m + F/2) / rho at the stencil cells, rho the cached density
 // (solver.rs:3040-3084: f64 momentum of the populations, force without the
 // just-cleared field).
 template <int Q, typename S>
-__global__ void k_ibm_gather(Dom d, const S* f, const uint32_t* mask, int odd, const uint32_t* uc_cell, uint32_t U,
-                             const float* F_cur, int field_present, HydroParams hp, double* unow, double* rho,
-                             double* du, double* spread) {
+__global__ void k_ibm_gather(const IbmDomain* dom, int odd, const uint32_t* uc_cell, const uint32_t* uc_inst,
+                             uint32_t U, int field_present, double* unow, double* rho, double* du, double* spread) {
   const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
   if (u >= U) return;
+  const IbmDomain& D = dom[uc_inst[u]];
+  const Dom d = D.d;
   const uint32_t i = uc_cell[u];
   uint32_t x, y, z;
   cell_xyz(d, i, x, y, z);
   const Nbr nb(d, x, y, z);
-  const uint32_t m = mask[i];
+  const uint32_t m = D.mask[i];
+  const S* f = (const S*)D.f;
   float dr = 0.0f;
   double mom[3] = {0.0, 0.0, 0.0};
   sfor<0, Q>([&](auto qc) {
@@ -109,9 +134,9 @@ __global__ void k_ibm_gather(Dom d, const S* f, const uint32_t* mask, int odd, c
     if constexpr (lat_c(q, 2) != 0) mom[2] += (double)lat_c(q, 2) * (double)v;
   });
   const float r = 1.0f + dr;
-  double force[3] = {(double)hp.F[0], (double)hp.F[1], (double)hp.F[2]};
-  if (field_present && F_cur)
-    for (int a = 0; a < 3; ++a) force[a] += (double)F_cur[3 * (size_t)i + a];
+  double force[3] = {(double)D.F[0], (double)D.F[1], (double)D.F[2]};
+  if (field_present && D.F_cur)
+    for (int a = 0; a < 3; ++a) force[a] += (double)D.F_cur[3 * (size_t)i + a];
   const double inv_rho = 1.0 / fmax((double)r, 1.0e-30);
   for (int a = 0; a < 3; ++a) {
     unow[3 * (size_t)u + a] = (mom[a] + 0.5 * force[a]) * inv_rho;
@@ -121,18 +146,18 @@ __global__ void k_ibm_gather(Dom d, const S* f, const uint32_t* mask, int odd, c
   rho[u] = (double)r;
 }
 
-__global__ void k_ibm_reset(IbmDevDiag* diag) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    memset(diag, 0, sizeof(IbmDevDiag));
-  }
+__global__ void k_ibm_reset(IbmDevDiag* diag, int K) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < K) memset(diag + k, 0, sizeof(IbmDevDiag));
 }
 
 // per marker: interpolated velocity, mobility row sum, impulse
-__global__ void k_ibm_impulse(int64_t M, const uint32_t* mk_off, const uint32_t* pt_u, const double* pt_w,
-                              const double* target, const double* unow, const double* du, const double* rho,
-                              const double* ksum, double relax, double* imp, const IbmDevDiag* diag) {
+__global__ void k_ibm_impulse(int64_t M, const uint32_t* mk_off, const uint32_t* mk_inst, const uint32_t* pt_u,
+                              const double* pt_w, const double* target, const double* unow, const double* du,
+                              const double* rho, const double* ksum, double relax, double* imp,
+                              const IbmDevDiag* diag) {
   const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= M || diag->converged) return;
+  if (m >= M || diag[mk_inst[m]].converged) return;
   double um[3] = {0.0, 0.0, 0.0}, mob = 0.0;
   for (uint32_t p = mk_off[m]; p < mk_off[m + 1]; ++p) {
     const uint32_t u = pt_u[p];
@@ -147,11 +172,11 @@ __global__ void k_ibm_impulse(int64_t M, const uint32_t* mk_off, const uint32_t*
 }
 
 // per cell: spread and du accumulated over the cell's markers in marker order
-__global__ void k_ibm_spread(uint32_t U, const uint32_t* uc_off, const uint32_t* uc_ent, const uint32_t* pt_m,
-                             const double* pt_w, const double* imp, const double* rho, double* spread, double* du,
-                             const IbmDevDiag* diag) {
+__global__ void k_ibm_spread(uint32_t U, const uint32_t* uc_off, const uint32_t* uc_inst, const uint32_t* uc_ent,
+                             const uint32_t* pt_m, const double* pt_w, const double* imp, const double* rho,
+                             double* spread, double* du, const IbmDevDiag* diag) {
   const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
-  if (u >= U || diag->converged) return;
+  if (u >= U || diag[uc_inst[u]].converged) return;
   const double inv_rho = 1.0 / fmax(rho[u], 1.0e-30);
   double s[3], dd[3];
   for (int a = 0; a < 3; ++a) {
@@ -175,12 +200,14 @@ __global__ void k_ibm_spread(uint32_t U, const uint32_t* uc_off, const uint32_t*
 }
 
 // per marker: represented force, torque contribution, post-sweep slip
-__global__ void k_ibm_marker_diag(int64_t M, const uint32_t* mk_off, const uint32_t* pt_u, const double* pt_w,
-                                  const double* pos, const double* target, const double* unow, const double* du,
-                                  const double* imp, double c0, double c1, double c2, double* mrep, double* mtq,
+__global__ void k_ibm_marker_diag(int64_t M, const uint32_t* mk_off, const uint32_t* mk_inst, const uint32_t* pt_u,
+                                  const double* pt_w, const double* pos, const double* target, const double* unow,
+                                  const double* du, const double* imp, const double* cen, double* mrep, double* mtq,
                                   double* mslip, uint8_t* mact, const IbmDevDiag* diag) {
   const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= M || diag->converged) return;
+  if (m >= M) return;
+  const uint32_t k = mk_inst[m];
+  if (diag[k].converged) return;
   double rep[3] = {0.0, 0.0, 0.0}, um[3] = {0.0, 0.0, 0.0};
   for (uint32_t p = mk_off[m]; p < mk_off[m + 1]; ++p) {
     const double w = pt_w[p];
@@ -191,7 +218,7 @@ __global__ void k_ibm_marker_diag(int64_t M, const uint32_t* mk_off, const uint3
     const double w = pt_w[p];
     for (int a = 0; a < 3; ++a) um[a] += w * (unow[3 * (size_t)u + a] + du[3 * (size_t)u + a]);
   }
-  const double r[3] = {pos[3 * m] - c0, pos[3 * m + 1] - c1, pos[3 * m + 2] - c2};
+  const double r[3] = {pos[3 * m] - cen[3 * k], pos[3 * m + 1] - cen[3 * k + 1], pos[3 * m + 2] - cen[3 * k + 2]};
   mtq[3 * m + 0] = r[1] * rep[2] - r[2] * rep[1];
   mtq[3 * m + 1] = r[2] * rep[0] - r[0] * rep[2];
   mtq[3 * m + 2] = r[0] * rep[1] - r[1] * rep[0];
@@ -201,17 +228,20 @@ __global__ void k_ibm_marker_diag(int64_t M, const uint32_t* mk_off, const uint3
   mact[m] = mk_off[m + 1] > mk_off[m] ? 1 : 0;
 }
 
-// one block: fixed-tree reduction over markers and the convergence test
-__global__ __launch_bounds__(256) void k_ibm_reduce(int64_t M, const double* mrep, const double* mtq,
+// one block per instance: fixed-tree reduction over its markers and the
+// convergence test
+__global__ __launch_bounds__(256) void k_ibm_reduce(const uint32_t* moff, const double* mrep, const double* mtq,
                                                     const double* mslip, const uint8_t* mact, const double* weight,
-                                                    int iter, double scale, double tol, IbmDevDiag* diag) {
+                                                    int iter, const double* scale, double tol, IbmDevDiag* diags) {
   __shared__ double sh[9][256];
   __shared__ int conv;
+  const int k = blockIdx.x;
+  IbmDevDiag* diag = diags + k;
   if (threadIdx.x == 0) conv = diag->converged;
   __syncthreads();
   if (conv) return;
   double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // tq[3], rep[3], slipmax, ssq, wsum
-  for (int64_t m = threadIdx.x; m < M; m += 256) {
+  for (int64_t m = (int64_t)moff[k] + threadIdx.x; m < (int64_t)moff[k + 1]; m += 256) {
     for (int a = 0; a < 3; ++a) {
       acc[a] += mtq[3 * m + a];
       acc[3 + a] += mrep[3 * m + a];
@@ -222,13 +252,13 @@ __global__ __launch_bounds__(256) void k_ibm_reduce(int64_t M, const double* mre
       acc[8] += weight[m];
     }
   }
-  for (int k = 0; k < 9; ++k) sh[k][threadIdx.x] = acc[k];
+  for (int j = 0; j < 9; ++j) sh[j][threadIdx.x] = acc[j];
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w)
-      for (int k = 0; k < 9; ++k)
-        sh[k][threadIdx.x] = k == 6 ? fmax(sh[k][threadIdx.x], sh[k][threadIdx.x + w])
-                                    : sh[k][threadIdx.x] + sh[k][threadIdx.x + w];
+      for (int j = 0; j < 9; ++j)
+        sh[j][threadIdx.x] = j == 6 ? fmax(sh[j][threadIdx.x], sh[j][threadIdx.x + w])
+                                    : sh[j][threadIdx.x] + sh[j][threadIdx.x + w];
     __syncthreads();
   }
   if (threadIdx.x != 0) return;
@@ -240,8 +270,8 @@ __global__ __launch_bounds__(256) void k_ibm_reduce(int64_t M, const double* mre
   diag->iterations = iter + 1;
   diag->slip_max = sh[6][0];
   diag->slip_rms = sh[8][0] > 0.0 ? sqrt(sh[7][0] / sh[8][0]) : 0.0;
-  diag->slip_max_rel = diag->slip_max / scale;
-  diag->slip_rms_rel = diag->slip_rms / scale;
+  diag->slip_max_rel = diag->slip_max / scale[k];
+  diag->slip_rms_rel = diag->slip_rms / scale[k];
   const double* ff = diag->fluid_force;
   const double* mf = diag->marker_force;
   const double e[3] = {ff[0] - mf[0], ff[1] - mf[1], ff[2] - mf[2]};
@@ -252,11 +282,26 @@ __global__ __launch_bounds__(256) void k_ibm_reduce(int64_t M, const double* mre
 }
 
 // force_field += T(spread)
-__global__ void k_ibm_commit(uint32_t U, const uint32_t* uc_cell, const double* spread, float* F_cur) {
+__global__ void k_ibm_commit(uint32_t U, const uint32_t* uc_cell, const uint32_t* uc_inst, const IbmDomain* dom,
+                             const double* spread) {
   const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
   if (u >= U) return;
+  float* F = dom[uc_inst[u]].F_cur;
   const size_t i = uc_cell[u];
-  for (int a = 0; a < 3; ++a) F_cur[3 * i + a] = F_cur[3 * i + a] + (float)spread[3 * (size_t)u + a];
+  for (int a = 0; a < 3; ++a) F[3 * i + a] = F[3 * i + a] + (float)spread[3 * (size_t)u + a];
+}
+
+// one block: append the K diagnostics of this update to a device ring (read
+// back after a span of steps, so the step loop needs no host
+// synchronisation), then advance the ring position
+__global__ void k_ibm_record(const IbmDevDiag* diag, int K, IbmDevDiag* ring, uint32_t cap, uint32_t* pos) {
+  __shared__ uint32_t slot;
+  if (threadIdx.x == 0) slot = *pos;
+  __syncthreads();
+  for (int k = threadIdx.x; k < K; k += blockDim.x)
+    if (slot < cap) ring[(size_t)slot * K + k] = diag[k];
+  __syncthreads();
+  if (threadIdx.x == 0) *pos = slot + 1;
 }
 
 // marker_stencil, rotating_ibm.rs:163-224 (host, f64)
@@ -308,12 +353,92 @@ static hipError_t scratch(T** dst, size_t n) {
 
 void ibm_destroy(IbmState* s) {
   if (!s) return;
-  void* ptrs[] = {s->pos,  s->weight, s->target, s->mk_off, s->pt_u,  s->pt_m,  s->pt_w,  s->uc_cell,
-                  s->uc_off, s->uc_ent, s->ksum, s->unow,   s->rho,   s->du,    s->spread, s->imp,
-                  s->mrep, s->mtq,    s->mslip,  s->mact,   s->diag};
+  void* ptrs[] = {s->moff,    s->cen,    s->scale,   s->dom,    s->pos,    s->weight, s->target, s->mk_off,
+                  s->mk_inst, s->pt_u,   s->pt_m,    s->pt_w,   s->uc_cell, s->uc_inst, s->uc_off, s->uc_ent,
+                  s->ksum,    s->unow,   s->rho,     s->du,     s->spread, s->imp,    s->mrep,   s->mtq,
+                  s->mslip,   s->mact,   s->diag};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete s;
+}
+
+// Concatenate bodies into one device state (instance k = parts[k]).
+static IbmState* ibm_build(const std::vector<const IbmHost*>& parts, const lbm_ibm_config& cfg, std::string* err) {
+  IbmState* s = new IbmState();
+  s->K = (int)parts.size();
+  s->cfg = cfg;
+  std::vector<double> P, W, T, cen, scale;
+  std::vector<uint32_t> moff{0}, mk_off{0}, mk_inst, pt_u, pt_m, uc_cell, uc_inst, uc_off, uc_ent;
+  std::vector<double> pt_w, ksum;
+  for (int k = 0; k < s->K; ++k) {
+    const IbmHost& h = *parts[k];
+    const uint32_t m0 = (uint32_t)mk_inst.size(), p0 = (uint32_t)pt_u.size(), u0 = (uint32_t)uc_cell.size();
+    P.insert(P.end(), h.pos.begin(), h.pos.end());
+    W.insert(W.end(), h.weight.begin(), h.weight.end());
+    T.insert(T.end(), h.target.begin(), h.target.end());
+    for (int a = 0; a < 3; ++a) cen.push_back(h.center[a]);
+    scale.push_back(h.scale);
+    for (int64_t m = 0; m < h.M; ++m) {
+      mk_off.push_back(p0 + h.mk_off[m + 1]);
+      mk_inst.push_back((uint32_t)k);
+    }
+    moff.push_back((uint32_t)mk_inst.size());
+    for (size_t p = 0; p < h.pt_u.size(); ++p) {
+      pt_u.push_back(u0 + h.pt_u[p]);
+      pt_m.push_back(m0 + h.pt_m[p]);
+      pt_w.push_back(h.pt_w[p]);
+      uc_ent.push_back(p0 + h.uc_ent[p]);
+    }
+    for (size_t u = 0; u < h.uc_cell.size(); ++u) {
+      uc_cell.push_back(h.uc_cell[u]);
+      uc_inst.push_back((uint32_t)k);
+      uc_off.push_back(p0 + h.uc_off[u]);
+      ksum.push_back(h.ksum[u]);
+    }
+  }
+  uc_off.push_back((uint32_t)pt_u.size());
+  s->M = (int64_t)mk_inst.size();
+  s->P = (uint32_t)pt_u.size();
+  s->U = (uint32_t)uc_cell.size();
+  if (s->K == 1) s->host = *parts[0];
+  s->dom_h.assign(s->K, IbmDomain{});
+  hipError_t e = hipSuccess;
+  auto chain = [&](hipError_t r) {
+    if (e == hipSuccess) e = r;
+  };
+  chain(upload(&s->moff, moff));
+  chain(upload(&s->cen, cen));
+  chain(upload(&s->scale, scale));
+  chain(scratch(&s->dom, s->K));
+  chain(upload(&s->pos, P));
+  chain(upload(&s->weight, W));
+  chain(upload(&s->target, T));
+  chain(upload(&s->mk_off, mk_off));
+  chain(upload(&s->mk_inst, mk_inst));
+  chain(upload(&s->pt_u, pt_u));
+  chain(upload(&s->pt_m, pt_m));
+  chain(upload(&s->pt_w, pt_w));
+  chain(upload(&s->uc_cell, uc_cell));
+  chain(upload(&s->uc_inst, uc_inst));
+  chain(upload(&s->uc_off, uc_off));
+  chain(upload(&s->uc_ent, uc_ent));
+  chain(upload(&s->ksum, ksum));
+  chain(scratch(&s->unow, 3 * (size_t)s->U));
+  chain(scratch(&s->rho, s->U));
+  chain(scratch(&s->du, 3 * (size_t)s->U));
+  chain(scratch(&s->spread, 3 * (size_t)s->U));
+  chain(scratch(&s->imp, 3 * (size_t)s->M));
+  chain(scratch(&s->mrep, 3 * (size_t)s->M));
+  chain(scratch(&s->mtq, 3 * (size_t)s->M));
+  chain(scratch(&s->mslip, s->M));
+  chain(scratch(&s->mact, s->M));
+  chain(scratch(&s->diag, s->K));
+  if (e != hipSuccess) {
+    *err = std::string("IBM device setup failed: ") + hipGetErrorString(e);
+    ibm_destroy(s);
+    return nullptr;
+  }
+  return s;
 }
 
 IbmState* ibm_create(const FieldsView& v, int64_t n, const double* pos, const double* weight, const double center[3],
@@ -334,114 +459,120 @@ IbmState* ibm_create(const FieldsView& v, int64_t n, const double* pos, const do
     *err = "mask readback failed";
     return nullptr;
   }
-  IbmState* s = new IbmState();
-  s->M = n;
-  s->cfg = cfg;
+  IbmHost h;
+  h.M = n;
   for (int a = 0; a < 3; ++a) {
-    s->center[a] = center[a];
-    s->omega[a] = omega[a];
+    h.center[a] = center[a];
+    h.omega[a] = omega[a];
   }
   // targets (RotatingBody::target_velocity, rotating_ibm.rs:89-101) and scale
-  std::vector<double> tgt(3 * n), P(pos, pos + 3 * n), W(weight, weight + n);
+  h.target.assign(3 * n, 0.0);
+  h.pos.assign(pos, pos + 3 * n);
+  h.weight.assign(weight, weight + n);
   double max_target = 0.0;
   for (int64_t m = 0; m < n; ++m) {
     const double r[3] = {pos[3 * m] - center[0], pos[3 * m + 1] - center[1], pos[3 * m + 2] - center[2]};
-    tgt[3 * m + 0] = omega[1] * r[2] - omega[2] * r[1];
-    tgt[3 * m + 1] = omega[2] * r[0] - omega[0] * r[2];
-    tgt[3 * m + 2] = omega[0] * r[1] - omega[1] * r[0];
-    const double* t = &tgt[3 * m];
+    h.target[3 * m + 0] = omega[1] * r[2] - omega[2] * r[1];
+    h.target[3 * m + 1] = omega[2] * r[0] - omega[0] * r[2];
+    h.target[3 * m + 2] = omega[0] * r[1] - omega[1] * r[0];
+    const double* t = &h.target[3 * m];
     max_target = std::max(max_target, std::sqrt(0.0 + t[0] * t[0] + t[1] * t[1] + t[2] * t[2]));
   }
-  s->scale = std::max(max_target, 1.0);
+  h.scale = std::max(max_target, 1.0);
   // stencils, solid cells dropped
-  std::vector<uint32_t> mk_off(n + 1, 0), pt_cell, pt_m;
-  std::vector<double> pt_w;
+  h.mk_off.assign(n + 1, 0);
+  std::vector<uint32_t> pt_cell;
   std::vector<int64_t> sc;
   std::vector<double> sw;
   for (int64_t m = 0; m < n; ++m) {
     marker_stencil(&pos[3 * m], dims, cfg.kernel_radius, sc, sw);
     if (sc.empty()) {
       *err = "IBM marker stencil is outside the domain";
-      ibm_destroy(s);
       return nullptr;
     }
     for (size_t k = 0; k < sc.size(); ++k) {
       if (mask[sc[k]] & kMaskSolid) continue;
       pt_cell.push_back((uint32_t)sc[k]);
-      pt_m.push_back((uint32_t)m);
-      pt_w.push_back(sw[k]);
+      h.pt_m.push_back((uint32_t)m);
+      h.pt_w.push_back(sw[k]);
     }
-    mk_off[m + 1] = (uint32_t)pt_cell.size();
+    h.mk_off[m + 1] = (uint32_t)pt_cell.size();
   }
   const uint32_t Pn = (uint32_t)pt_cell.size();
   // unique cells, and per cell the points in marker order (stable sort)
-  std::vector<uint32_t> order(Pn);
-  for (uint32_t k = 0; k < Pn; ++k) order[k] = k;
-  std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return pt_cell[a] < pt_cell[b]; });
-  std::vector<uint32_t> uc_cell, uc_off, pt_u(Pn);
-  std::vector<double> ksum;
+  h.uc_ent.resize(Pn);
+  for (uint32_t k = 0; k < Pn; ++k) h.uc_ent[k] = k;
+  std::stable_sort(h.uc_ent.begin(), h.uc_ent.end(),
+                   [&](uint32_t a, uint32_t b) { return pt_cell[a] < pt_cell[b]; });
+  h.pt_u.assign(Pn, 0);
   for (uint32_t k = 0; k < Pn; ++k) {
-    const uint32_t p = order[k];
-    if (uc_cell.empty() || uc_cell.back() != pt_cell[p]) {
-      uc_cell.push_back(pt_cell[p]);
-      uc_off.push_back(k);
-      ksum.push_back(0.0);
+    const uint32_t p = h.uc_ent[k];
+    if (h.uc_cell.empty() || h.uc_cell.back() != pt_cell[p]) {
+      h.uc_cell.push_back(pt_cell[p]);
+      h.uc_off.push_back(k);
+      h.ksum.push_back(0.0);
     }
-    pt_u[p] = (uint32_t)uc_cell.size() - 1;
+    h.pt_u[p] = (uint32_t)h.uc_cell.size() - 1;
   }
-  uc_off.push_back(Pn);
+  h.uc_off.push_back(Pn);
   // cell_kernel_sum in marker order (solver.rs:3107-3116)
-  for (uint32_t p = 0; p < Pn; ++p) ksum[pt_u[p]] += pt_w[p];
-  s->P = Pn;
-  s->U = (uint32_t)uc_cell.size();
-  hipError_t e = hipSuccess;
-  auto chain = [&](hipError_t r) {
-    if (e == hipSuccess) e = r;
-  };
-  chain(upload(&s->pos, P));
-  chain(upload(&s->weight, W));
-  chain(upload(&s->target, tgt));
-  chain(upload(&s->mk_off, mk_off));
-  chain(upload(&s->pt_u, pt_u));
-  chain(upload(&s->pt_m, pt_m));
-  chain(upload(&s->pt_w, pt_w));
-  chain(upload(&s->uc_cell, uc_cell));
-  chain(upload(&s->uc_off, uc_off));
-  chain(upload(&s->uc_ent, order));
-  chain(upload(&s->ksum, ksum));
-  chain(scratch(&s->unow, 3 * (size_t)s->U));
-  chain(scratch(&s->rho, s->U));
-  chain(scratch(&s->du, 3 * (size_t)s->U));
-  chain(scratch(&s->spread, 3 * (size_t)s->U));
-  chain(scratch(&s->imp, 3 * (size_t)n));
-  chain(scratch(&s->mrep, 3 * (size_t)n));
-  chain(scratch(&s->mtq, 3 * (size_t)n));
-  chain(scratch(&s->mslip, n));
-  chain(scratch(&s->mact, n));
-  chain(scratch(&s->diag, 1));
-  if (e != hipSuccess) {
-    *err = std::string("IBM device setup failed: ") + hipGetErrorString(e);
-    ibm_destroy(s);
+  for (uint32_t p = 0; p < Pn; ++p) h.ksum[h.pt_u[p]] += h.pt_w[p];
+  IbmState* s = ibm_build({&h}, cfg, err);
+  if (!s) return nullptr;
+  *cells = h.uc_cell;
+  return s;
+}
+
+IbmState* ibm_concat(const std::vector<const IbmState*>& parts, std::string* err) {
+  std::vector<const IbmHost*> hosts;
+  for (const IbmState* p : parts) {
+    if (!p || p->K != 1) {
+      *err = "batched IBM needs one single-body state per instance";
+      return nullptr;
+    }
+    const lbm_ibm_config& a = p->cfg;
+    const lbm_ibm_config& b = parts[0]->cfg;
+    if (a.max_iterations != b.max_iterations || a.kernel_radius != b.kernel_radius ||
+        a.relaxation != b.relaxation || a.slip_tolerance != b.slip_tolerance) {
+      *err = "batched IBM bodies must share one IBM configuration";
+      return nullptr;
+    }
+    hosts.push_back(&p->host);
+  }
+  if (hosts.empty()) {
+    *err = "empty IBM batch";
     return nullptr;
   }
-  *cells = uc_cell;
-  return s;
+  return ibm_build(hosts, parts[0]->cfg, err);
+}
+
+int ibm_instances(const IbmState* s) { return s ? s->K : 0; }
+
+int ibm_set_domains(const FieldsView& v, IbmState* s, const IbmDomain* doms) {
+  bool same = s->dom_valid;
+  for (int k = 0; k < s->K && same; ++k) same = memcmp(&doms[k], &s->dom_h[k], sizeof(IbmDomain)) == 0;
+  if (same) return LBM_OK;
+  for (int k = 0; k < s->K; ++k) s->dom_h[k] = doms[k];
+  AUXCHK(v, hipMemcpyAsync(s->dom, s->dom_h.data(), s->K * sizeof(IbmDomain), hipMemcpyHostToDevice, v.stream));
+  AUXCHK(v, hipStreamSynchronize(v.stream));
+  s->dom_valid = true;
+  return LBM_OK;
+}
+
+IbmDomain ibm_domain_of(const FieldsView& v) {
+  IbmDomain D{};
+  D.d = v.d;
+  D.f = v.f;
+  D.mask = v.mask;
+  D.F_cur = v.F_cur;
+  for (int a = 0; a < 3; ++a) D.F[a] = v.hp.F[a];
+  return D;
 }
 
 template <int Q, typename S>
 static void gather_q(const FieldsView& v, IbmState* s) {
-  hipLaunchKernelGGL((k_ibm_gather<Q, S>), dim3(nblk(s->U)), dim3(256), 0, v.stream, v.d, (const S*)v.f, v.mask,
-                     v.parity, s->uc_cell, s->U, v.F_cur, v.field_present ? 1 : 0, v.hp, s->unow, s->rho, s->du,
-                     s->spread);
-}
-
-// one thread: append the current diagnostics to a device ring (read back
-// after a span of steps, so the step loop needs no host synchronisation)
-__global__ void k_ibm_record(const IbmDevDiag* diag, IbmDevDiag* ring, uint32_t cap, uint32_t* pos) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  const uint32_t k = *pos;
-  if (k < cap) ring[k] = *diag;
-  *pos = k + 1;
+  hipLaunchKernelGGL((k_ibm_gather<Q, S>), dim3(nblk(s->U)), dim3(256), 0, v.stream, s->dom, v.parity, s->uc_cell,
+                     s->uc_inst, s->U, v.field_present ? 1 : 0, s->unow, s->rho, s->du, s->spread);
 }
 
 static void diag_to_abi(const IbmDevDiag& h, lbm_ibm_diag* out) {
@@ -459,10 +590,15 @@ static void diag_to_abi(const IbmDevDiag& h, lbm_ibm_diag* out) {
 }
 
 int ibm_enqueue(const FieldsView& v, IbmState* s, void* ring, uint32_t cap, uint32_t* pos) {
-  if (!v.F_cur) {
-    if (v.ctx_err) *v.ctx_err = "force arrays missing";
+  if (!s->dom_valid) {
+    if (v.ctx_err) *v.ctx_err = "IBM domain table not set";
     return LBM_ERR_STATE;
   }
+  for (int k = 0; k < s->K; ++k)
+    if (!s->dom_h[k].F_cur) {
+      if (v.ctx_err) *v.ctx_err = "force arrays missing";
+      return LBM_ERR_STATE;
+    }
   hipStream_t st = v.stream;
   if (s->U) {
     if (v.Q == 19) {
@@ -476,28 +612,28 @@ int ibm_enqueue(const FieldsView& v, IbmState* s, void* ring, uint32_t cap, uint
     }
     AUXCHK(v, hipGetLastError());
   }
-  hipLaunchKernelGGL(k_ibm_reset, dim3(1), dim3(64), 0, st, s->diag);
+  hipLaunchKernelGGL(k_ibm_reset, dim3(nblk(s->K, 64)), dim3(64), 0, st, s->diag, s->K);
   AUXCHK(v, hipGetLastError());
   const uint32_t gm = nblk((size_t)s->M), gu = nblk(s->U);
   for (int it = 0; it < s->cfg.max_iterations; ++it) {
-    hipLaunchKernelGGL(k_ibm_impulse, dim3(gm), dim3(256), 0, st, s->M, s->mk_off, s->pt_u, s->pt_w, s->target,
-                       s->unow, s->du, s->rho, s->ksum, s->cfg.relaxation, s->imp, s->diag);
+    hipLaunchKernelGGL(k_ibm_impulse, dim3(gm), dim3(256), 0, st, s->M, s->mk_off, s->mk_inst, s->pt_u, s->pt_w,
+                       s->target, s->unow, s->du, s->rho, s->ksum, s->cfg.relaxation, s->imp, s->diag);
     if (s->U)
-      hipLaunchKernelGGL(k_ibm_spread, dim3(gu), dim3(256), 0, st, s->U, s->uc_off, s->uc_ent, s->pt_m, s->pt_w,
-                         s->imp, s->rho, s->spread, s->du, s->diag);
-    hipLaunchKernelGGL(k_ibm_marker_diag, dim3(gm), dim3(256), 0, st, s->M, s->mk_off, s->pt_u, s->pt_w, s->pos,
-                       s->target, s->unow, s->du, s->imp, s->center[0], s->center[1], s->center[2], s->mrep, s->mtq,
-                       s->mslip, s->mact, s->diag);
-    hipLaunchKernelGGL(k_ibm_reduce, dim3(1), dim3(256), 0, st, s->M, s->mrep, s->mtq, s->mslip, s->mact, s->weight,
-                       it, s->scale, s->cfg.slip_tolerance, s->diag);
+      hipLaunchKernelGGL(k_ibm_spread, dim3(gu), dim3(256), 0, st, s->U, s->uc_off, s->uc_inst, s->uc_ent, s->pt_m,
+                         s->pt_w, s->imp, s->rho, s->spread, s->du, s->diag);
+    hipLaunchKernelGGL(k_ibm_marker_diag, dim3(gm), dim3(256), 0, st, s->M, s->mk_off, s->mk_inst, s->pt_u, s->pt_w,
+                       s->pos, s->target, s->unow, s->du, s->imp, s->cen, s->mrep, s->mtq, s->mslip, s->mact,
+                       s->diag);
+    hipLaunchKernelGGL(k_ibm_reduce, dim3(s->K), dim3(256), 0, st, s->moff, s->mrep, s->mtq, s->mslip, s->mact,
+                       s->weight, it, s->scale, s->cfg.slip_tolerance, s->diag);
     AUXCHK(v, hipGetLastError());
   }
   if (s->U) {
-    hipLaunchKernelGGL(k_ibm_commit, dim3(gu), dim3(256), 0, st, s->U, s->uc_cell, s->spread, v.F_cur);
+    hipLaunchKernelGGL(k_ibm_commit, dim3(gu), dim3(256), 0, st, s->U, s->uc_cell, s->uc_inst, s->dom, s->spread);
     AUXCHK(v, hipGetLastError());
   }
   if (ring) {
-    hipLaunchKernelGGL(k_ibm_record, dim3(1), dim3(64), 0, st, s->diag, (IbmDevDiag*)ring, cap, pos);
+    hipLaunchKernelGGL(k_ibm_record, dim3(1), dim3(64), 0, st, s->diag, s->K, (IbmDevDiag*)ring, cap, pos);
     AUXCHK(v, hipGetLastError());
   }
   return LBM_OK;
@@ -506,10 +642,10 @@ int ibm_enqueue(const FieldsView& v, IbmState* s, void* ring, uint32_t cap, uint
 int ibm_apply(const FieldsView& v, IbmState* s, lbm_ibm_diag* out) {
   int rc = ibm_enqueue(v, s, nullptr, 0, nullptr);
   if (rc) return rc;
-  IbmDevDiag h{};
-  AUXCHK(v, hipMemcpyAsync(&h, s->diag, sizeof h, hipMemcpyDeviceToHost, v.stream));
+  std::vector<IbmDevDiag> h(s->K);
+  AUXCHK(v, hipMemcpyAsync(h.data(), s->diag, s->K * sizeof(IbmDevDiag), hipMemcpyDeviceToHost, v.stream));
   AUXCHK(v, hipStreamSynchronize(v.stream));
-  diag_to_abi(h, out);
+  for (int k = 0; k < s->K; ++k) diag_to_abi(h[k], out + k);
   return LBM_OK;
 }
 

@@ -29,18 +29,39 @@ struct FieldsView {
 struct IbmState;
 struct ScalarState;
 
+// What the IBM kernels need of one domain (one entry per batched instance).
+struct IbmDomain {
+  Dom d;
+  const void* f;
+  const uint32_t* mask;
+  float* F_cur;
+  float F[3];  // uniform body force
+};
+
 // Rotating IBM (Solver::apply_rotating_ibm, solver.rs:3016-3282).  `cells`
 // receives the sorted fluid cells the body can force.
 IbmState* ibm_create(const FieldsView& v, int64_t n, const double* pos, const double* weight, const double center[3],
                      const double omega[3], const lbm_ibm_config& cfg, std::vector<uint32_t>* cells,
                      std::string* err);
 void ibm_destroy(IbmState* s);
+// One state may hold several bodies, one per domain ("instance"); a state
+// made by ibm_create holds one.  ibm_concat joins single-body states (same
+// IBM configuration) into one state whose instance k is parts[k].
+IbmState* ibm_concat(const std::vector<const IbmState*>& parts, std::string* err);
+int ibm_instances(const IbmState* s);
+// The per-instance domain table the kernels read (populations, mask, force
+// field, body force).  Uploads only when it changed; not inside a capture.
+int ibm_set_domains(const FieldsView& v, IbmState* s, const IbmDomain* doms);
+IbmDomain ibm_domain_of(const FieldsView& v);
+// One update of every instance; `diag` receives one entry per instance.
+// v supplies the stream, lattice, storage, parity and field flag.
 int ibm_apply(const FieldsView& v, IbmState* s, lbm_ibm_diag* diag);
-// The same update without the host read-back: the diagnostics go to entry
-// *pos of a device ring of `cap` entries (ibm_ring_entry_bytes each) and
+// The same update without the host read-back: the K diagnostics go to entry
+// *pos of a device ring of `cap` entries (K x ibm_ring_entry_bytes each) and
 // *pos advances.  Launches kernels only, so a stream capture can record it.
 int ibm_enqueue(const FieldsView& v, IbmState* s, void* ring, uint32_t cap, uint32_t* pos);
 size_t ibm_ring_entry_bytes();
+// n ring records (entry-major, instance-minor) to ABI diagnostics
 int ibm_ring_read(const FieldsView& v, const void* ring, uint32_t n, lbm_ibm_diag* out);
 
 // Passive scalar ADE (Solver::step_scalar_distributions, solver.rs:2113-2205).

@@ -15,6 +15,8 @@ constexpr uint32_t kMaskSolid = 1u << 31;
 constexpr uint32_t kMaskForced = 1u << 30;  // handled by the force-list kernel
 constexpr uint32_t kMaskMoving = 1u << 29;  // some linked solid has a nonzero wall velocity
 constexpr uint32_t kMaskLinks = (1u << 27) - 1u;
+// tile-list entry flag: every cell of the 64-cell tile has mask word 0
+constexpr uint32_t kTilePlain = 1u << 31;
 
 enum Storage : int { kF32 = 0, kF16 = 1, kBF16 = 2 };
 
@@ -63,6 +65,13 @@ struct HydroArgs {
 // Host-side launchers (lbm_kernels.hip).  All enqueue on `s`.
 hipError_t launch_hydro(int Q, int storage, int collision, int odd, const HydroArgs& a, hipStream_t s);
 hipError_t launch_hydro_list(int Q, int storage, int collision, int odd, const HydroArgs& a, hipStream_t s);
+// Batched domains: `tab` holds K HydroArgs on the device, `ent` 2 x nent
+// (tile entry, domain) pairs, `cells` 2 x n (cell, domain) pairs.
+hipError_t launch_hydro_batch(int Q, int storage, int collision, int odd, const HydroArgs* tab, const uint32_t* ent,
+                              uint32_t nent, uint32_t grid, hipStream_t s);
+hipError_t launch_hydro_list_batch(int Q, int storage, int collision, int odd, const HydroArgs* tab,
+                                   const uint32_t* cells, uint32_t n, hipStream_t s);
+hipError_t launch_clear_batch(const HydroArgs* tab, const uint32_t* cells, uint32_t n, hipStream_t s);
 hipError_t launch_build_mask(int Q, const Dom& d, const uint8_t* solid, const float* wall_u, uint32_t* mask,
                              unsigned int* err_count, hipStream_t s);
 hipError_t launch_mark_forced(const uint32_t* flist, uint32_t n, uint32_t* mask, hipStream_t s);

@@ -9,7 +9,9 @@
 // fp64.  The whole file is compiled with -ffp-contract=off so the TRT path
 // keeps the reference's separately rounded multiply/add sequence
 // (collision.rs:123-202) and reproduces the CpuScalar f32 trajectory
-// bit for bit; the central-moment path opts into contraction locally.
+// bit for bit; the central-moment path opts into contraction locally, per
+// source expression (contract(on)), so every kernel that inlines it rounds
+// identically (the batched launch reproduces the single-domain one).
 //
 // AA pattern (in-place streaming, one buffer):
 //   even step  cell i reads f_q from slot (i, q), writes the post-collision
@@ -135,7 +137,7 @@ constexpr int gidx(int cx, int cy, int cz) { return (cx + 1) * 9 + (cy + 1) * 3 
 
 // forward 1-D central line: (g-, g0, g+) -> (k0, k1, k2)
 __device__ __forceinline__ void fwd_line(float& a0, float& a1, float& a2, float u) {
-#pragma clang fp contract(fast)
+#pragma clang fp contract(on)
   const float gm = a0, g0 = a1, gp = a2;
   const float s = gp + gm, t = gp - gm;
   const float k0 = s + g0;
@@ -147,7 +149,7 @@ __device__ __forceinline__ void fwd_line(float& a0, float& a1, float& a2, float 
 }
 // inverse 1-D central line: (k0, k1, k2) -> (g-, g0, g+)
 __device__ __forceinline__ void inv_line(float& a0, float& a1, float& a2, float u) {
-#pragma clang fp contract(fast)
+#pragma clang fp contract(on)
   const float k0 = a0, k1 = a1, k2 = a2;
   const float m1 = k1 + u * k0;
   const float m2 = (k2 + 2.0f * u * k1) + u * u * k0;
@@ -181,7 +183,7 @@ __device__ __forceinline__ void inv27(float* G, float ux, float uy, float uz) {
 // deviation equilibrium of all 27 directions (same formula as the TRT
 // path, here with contraction allowed)
 __device__ __forceinline__ void feq27(float* feq, float r, float ux, float uy, float uz) {
-#pragma clang fp contract(fast)
+#pragma clang fp contract(on)
   const float usq = ux * ux + uy * uy + uz * uz;
   const float drho = r - 1.0f;
   const float b = drho - r * 1.5f * usq;
@@ -193,7 +195,7 @@ __device__ __forceinline__ void feq27(float* feq, float r, float ux, float uy, f
 }
 
 __device__ __forceinline__ float cm_rate(float os_base, float ux, float uy, float uz) {
-#pragma clang fp contract(fast)
+#pragma clang fp contract(on)
   const float usq = ux * ux + uy * uy + uz * uz;
   return fminf(os_base * (1.0f - 0.16f * usq), 2.0f);
 }
@@ -201,7 +203,7 @@ __device__ __forceinline__ float cm_rate(float os_base, float ux, float uy, floa
 // General form (any force): full transforms of f - feq and of the source.
 __device__ __forceinline__ void collide_cm27_general(float* fv, float r, float ux, float uy, float uz,
                                                      const float F[3], bool fon, float os_base) {
-#pragma clang fp contract(fast)
+#pragma clang fp contract(on)
   float feq[27];
   feq27(feq, r, ux, uy, uz);
   float K[27], S[27];
@@ -263,7 +265,7 @@ __device__ __forceinline__ void collide_cm27_general(float* fv, float r, float u
 // rounding of rho and u exactly as mass- and momentum-conserving as the
 // reference's f64 path.  Returns rho for the moving-wall bounce-back term.
 __device__ __forceinline__ float collide_cm27_free_direct(float* fv, float os_base) {
-#pragma clang fp contract(fast)
+#pragma clang fp contract(on)
   float Z0[9], Z1[9], Z2[9];
   sfor<0, 9>([&](auto abc) {
     constexpr int ab = decltype(abc)::value;
@@ -461,7 +463,10 @@ __device__ __forceinline__ void slot_st(S* p, float v) {
 // rewrite its own slots freely: even steps do, which turns every store of an
 // active tile into whole cache lines (partial lines cost the memory a
 // read-modify-write).
-template <int Q, typename S, int KIND, bool ODD, bool FON, bool WRAP, bool FILL>
+//
+// PLAIN: the whole tile is fluid with mask word 0 (no wall link, no force
+// list, no moving wall), known from the tile list; the mask is not read.
+template <int Q, typename S, int KIND, bool ODD, bool FON, bool WRAP, bool FILL, bool PLAIN = false>
 __device__ __forceinline__ void cell_step(const HydroArgs& a, uint32_t i, uint32_t x, uint32_t y, uint32_t z,
                                           const uint32_t* mask_ptr, uint32_t skip, const float Fm[3],
                                           const float Fc[3]) {
@@ -471,7 +476,7 @@ __device__ __forceinline__ void cell_step(const HydroArgs& a, uint32_t i, uint32
   // not wait for the solid test (the buffer's guard bands keep a solid face
   // cell's neighbour addresses inside the allocation); only the stores are
   // predicated on it.
-  const uint32_t m = *mask_ptr;
+  const uint32_t m = PLAIN ? 0u : *mask_ptr;
   float fv[Q];
   sfor<0, Q>([&](auto qc) {
     constexpr int q = decltype(qc)::value;
@@ -546,18 +551,19 @@ __device__ __forceinline__ void cell_step(const HydroArgs& a, uint32_t i, uint32
       slot_st<S, Q, q>(sl.template at<q, lat_c(q, 0), lat_c(q, 1), lat_c(q, 2)>(), fv[q]);
   };
   // even steps: the bounced population goes to the plain target as well
-  if (ODD == false || (m & kMaskLinks) == 0) {
+  if constexpr (!ODD || PLAIN) {
     sfor<0, Q>(store_plain);
-    return;
+  } else {
+    // one exec-masked pair of stores per direction for the whole wave
+    sfor<0, Q>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      constexpr int o = lat_opp(q);
+      if ((m >> o) & 1u)
+        slot_st<S, Q, q>(sl.template at<o, 0, 0, 0>(), fv[q]);
+      else
+        store_plain(qc);
+    });
   }
-  sfor<0, Q>([&](auto qc) {
-    constexpr int q = decltype(qc)::value;
-    constexpr int o = lat_opp(q);
-    if ((m >> o) & 1u)
-      slot_st<S, Q, q>(sl.template at<o, 0, 0, 0>(), fv[q]);
-    else
-      store_plain(qc);
-  });
 }
 
 template <int Q, typename S, int KIND, bool ODD, bool FON, bool WRAP>
@@ -567,14 +573,18 @@ __global__ __launch_bounds__(256) void k_hydro(HydroArgs a) {
   const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
   const uint32_t t = b * 4 + (threadIdx.x >> 6);
   if (t >= a.ntiles) return;
-  const uint32_t tile = __builtin_amdgcn_readfirstlane(a.tiles[t]);  // one tile per wave
+  const uint32_t te = __builtin_amdgcn_readfirstlane(a.tiles[t]);  // one tile per wave
+  const uint32_t tile = te & ~kTilePlain;
   const uint32_t row = tile / a.nseg, seg = tile - row * a.nseg;
   const uint32_t x = seg * 64 + (threadIdx.x & 63);
   if (x >= a.d.nx) return;
   const uint32_t y = row % a.d.ny, z = row / a.d.ny;
   const uint32_t i = row * a.d.nx + x;
-  cell_step<Q, S, KIND, ODD, FON, WRAP, true>(a, i, x, y, z, a.mask + i, kMaskSolid | kMaskForced, a.p.F,
-                                              a.p.F);
+  if (te & kTilePlain)
+    cell_step<Q, S, KIND, ODD, FON, WRAP, true, true>(a, i, x, y, z, nullptr, 0u, a.p.F, a.p.F);
+  else
+    cell_step<Q, S, KIND, ODD, FON, WRAP, true, false>(a, i, x, y, z, a.mask + i, kMaskSolid | kMaskForced,
+                                                       a.p.F, a.p.F);
 }
 
 // Cells carrying a per-cell force: the velocity uses the force of the cached
@@ -583,10 +593,7 @@ __global__ __launch_bounds__(256) void k_hydro(HydroArgs a) {
 // (solver.rs:2805-2822 keeps moments across a non-uniform clear).  After the
 // step the cached moments carry F_cur.
 template <int Q, typename S, int KIND, bool ODD>
-__global__ __launch_bounds__(256) void k_hydro_list(HydroArgs a) {
-  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= a.nlist) return;
-  const uint32_t i = a.flist[k];
+__device__ __forceinline__ void list_cell(const HydroArgs& a, uint32_t i) {
   const uint32_t m = a.mask[i];
   if (m & kMaskSolid) return;
   const uint32_t x = i % a.d.nx, t = i / a.d.nx;
@@ -600,6 +607,63 @@ __global__ __launch_bounds__(256) void k_hydro_list(HydroArgs a) {
   cell_step<Q, S, KIND, ODD, true, true, false>(a, i, x, y, z, a.mask + i, kMaskSolid, Fm, Fc);
 #pragma unroll
   for (int c = 0; c < 3; ++c) a.F_mom[3 * (size_t)i + c] = a.F_cur[3 * (size_t)i + c];
+}
+
+template <int Q, typename S, int KIND, bool ODD>
+__global__ __launch_bounds__(256) void k_hydro_list(HydroArgs a) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= a.nlist) return;
+  list_cell<Q, S, KIND, ODD>(a, a.flist[k]);
+}
+
+// ---------------------------------------------------------------------------
+// Batched domains: one persistent launch walks the concatenated tile lists
+// of K independent (non-periodic) domains; entry e = (tile, domain).  Each
+// wave strides over entries and reads its tile's domain arguments from the
+// table (wave-uniform, so scalar loads).  Consecutive entries share an XCD.
+// ---------------------------------------------------------------------------
+template <int Q, typename S, int KIND, bool ODD>
+__global__ __launch_bounds__(256) void k_hydro_batch(const HydroArgs* __restrict__ tab,
+                                                     const uint32_t* __restrict__ ent, uint32_t nent) {
+  const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
+  const uint32_t stride = gridDim.x * 4;
+  for (uint32_t e = b * 4 + (threadIdx.x >> 6); e < nent; e += stride) {
+    const uint32_t te = __builtin_amdgcn_readfirstlane(ent[2 * (size_t)e]);
+    const uint32_t k = __builtin_amdgcn_readfirstlane(ent[2 * (size_t)e + 1]);
+    const HydroArgs& a = tab[k];
+    const uint32_t tile = te & ~kTilePlain;
+    const uint32_t row = tile / a.nseg, seg = tile - row * a.nseg;
+    const uint32_t x = seg * 64 + (threadIdx.x & 63);
+    if (x < a.d.nx) {
+      const uint32_t y = row % a.d.ny, z = row / a.d.ny;
+      const uint32_t i = row * a.d.nx + x;
+      if (te & kTilePlain)
+        cell_step<Q, S, KIND, ODD, false, false, true, true>(a, i, x, y, z, nullptr, 0u, a.p.F, a.p.F);
+      else
+        cell_step<Q, S, KIND, ODD, false, false, true, false>(a, i, x, y, z, a.mask + i, kMaskSolid | kMaskForced,
+                                                              a.p.F, a.p.F);
+    }
+  }
+}
+
+// forced cells of all domains: entry = (cell, domain)
+template <int Q, typename S, int KIND, bool ODD>
+__global__ __launch_bounds__(256) void k_hydro_list_batch(const HydroArgs* __restrict__ tab,
+                                                          const uint32_t* __restrict__ cells, uint32_t n) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  list_cell<Q, S, KIND, ODD>(tab[cells[2 * (size_t)k + 1]], cells[2 * (size_t)k]);
+}
+
+// clear the force field at the listed cells of all domains
+__global__ void k_clear_batch(const HydroArgs* __restrict__ tab, const uint32_t* __restrict__ cells, uint32_t n) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  float* F = const_cast<float*>(tab[cells[2 * (size_t)k + 1]].F_cur);
+  const size_t i = cells[2 * (size_t)k];
+  F[3 * i] = 0.0f;
+  F[3 * i + 1] = 0.0f;
+  F[3 * i + 2] = 0.0f;
 }
 
 // moments readback (moments_row with the cached-moment force F_u + F_mom)
@@ -820,17 +884,21 @@ __global__ void k_build_mask(Dom d, const uint8_t* solid, const float* wall_u, u
   mask[i] = m;
 }
 
-// one flag per 64-cell x-segment: does it hold a cell for the main kernel?
+// one flag per 64-cell x-segment: 0 nothing for the main kernel, 1 some
+// cell for it, 2 all 64 cells plain fluid (mask word 0)
 __global__ void k_tile_flags(Dom d, const uint32_t* mask, uint8_t* flags) {
   const uint32_t nseg = (d.nx + 63) / 64;
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nseg * d.ny * d.nz) return;
   const uint32_t row = t / nseg, seg = t - row * nseg;
   const uint32_t x0 = seg * 64, x1 = min(x0 + 64, d.nx);
-  uint8_t any = 0;
-  for (uint32_t x = x0; x < x1; ++x)
-    if (!(mask[row * d.nx + x] & (kMaskSolid | kMaskForced))) any = 1;
-  flags[t] = any;
+  bool any = false, plain = x1 - x0 == 64;
+  for (uint32_t x = x0; x < x1; ++x) {
+    const uint32_t m = mask[row * d.nx + x];
+    if (!(m & (kMaskSolid | kMaskForced))) any = true;
+    if (m != 0u) plain = false;
+  }
+  flags[t] = any ? (plain ? 2 : 1) : 0;
 }
 
 __global__ void k_mark_forced(const uint32_t* flist, uint32_t n, uint32_t* mask) {
@@ -976,6 +1044,58 @@ hipError_t launch_hydro_list(int Q, int storage, int collision, int odd, const H
   if (storage == kF32) return hydro_s<float>(Q, collision, odd, a, s, true);
   if (storage == kF16) return hydro_s<_Float16>(Q, collision, odd, a, s, true);
   return hydro_s<bf16s>(Q, collision, odd, a, s, true);
+}
+
+template <int Q, typename S, int KIND>
+static hipError_t batch_q(int odd, const HydroArgs* tab, const uint32_t* ent, uint32_t nent, uint32_t grid,
+                          const uint32_t* cells, uint32_t ncells, hipStream_t s) {
+  if (ent) {
+    if (nent == 0) return hipSuccess;
+    if (odd)
+      hipLaunchKernelGGL((k_hydro_batch<Q, S, KIND, true>), dim3(grid), dim3(256), 0, s, tab, ent, nent);
+    else
+      hipLaunchKernelGGL((k_hydro_batch<Q, S, KIND, false>), dim3(grid), dim3(256), 0, s, tab, ent, nent);
+  } else {
+    if (ncells == 0) return hipSuccess;
+    if (odd)
+      hipLaunchKernelGGL((k_hydro_list_batch<Q, S, KIND, true>), dim3(blocks_for(ncells)), dim3(256), 0, s, tab,
+                         cells, ncells);
+    else
+      hipLaunchKernelGGL((k_hydro_list_batch<Q, S, KIND, false>), dim3(blocks_for(ncells)), dim3(256), 0, s, tab,
+                         cells, ncells);
+  }
+  return hipGetLastError();
+}
+
+template <typename S>
+static hipError_t batch_s(int Q, int collision, int odd, const HydroArgs* tab, const uint32_t* ent, uint32_t nent,
+                          uint32_t grid, const uint32_t* cells, uint32_t ncells, hipStream_t s) {
+  const int kind = collision == 2 ? 1 : 0;
+  if (Q == 19 && kind == 0) return batch_q<19, S, 0>(odd, tab, ent, nent, grid, cells, ncells, s);
+  if (Q == 27 && kind == 0) return batch_q<27, S, 0>(odd, tab, ent, nent, grid, cells, ncells, s);
+  if (Q == 27 && kind == 1) return batch_q<27, S, 1>(odd, tab, ent, nent, grid, cells, ncells, s);
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_hydro_batch(int Q, int storage, int collision, int odd, const HydroArgs* tab, const uint32_t* ent,
+                              uint32_t nent, uint32_t grid, hipStream_t s) {
+  if (grid == 0) return hipErrorInvalidValue;
+  if (storage == kF32) return batch_s<float>(Q, collision, odd, tab, ent, nent, grid, nullptr, 0, s);
+  if (storage == kF16) return batch_s<_Float16>(Q, collision, odd, tab, ent, nent, grid, nullptr, 0, s);
+  return batch_s<bf16s>(Q, collision, odd, tab, ent, nent, grid, nullptr, 0, s);
+}
+
+hipError_t launch_hydro_list_batch(int Q, int storage, int collision, int odd, const HydroArgs* tab,
+                                   const uint32_t* cells, uint32_t n, hipStream_t s) {
+  if (storage == kF32) return batch_s<float>(Q, collision, odd, tab, nullptr, 0, 0, cells, n, s);
+  if (storage == kF16) return batch_s<_Float16>(Q, collision, odd, tab, nullptr, 0, 0, cells, n, s);
+  return batch_s<bf16s>(Q, collision, odd, tab, nullptr, 0, 0, cells, n, s);
+}
+
+hipError_t launch_clear_batch(const HydroArgs* tab, const uint32_t* cells, uint32_t n, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_clear_batch, dim3(blocks_for(n)), dim3(256), 0, s, tab, cells, n);
+  return hipGetLastError();
 }
 
 hipError_t launch_build_mask(int Q, const Dom& d, const uint8_t* solid, const float* wall_u, uint32_t* mask,

@@ -392,3 +392,51 @@ def mixing_time_from_cv(t, cv):
         return None
     f = lambda v: None if np.isnan(v) else float(v)  # noqa: E731
     return {"cv0": float(out[0]), "t95": f(out[1]), "t99": f(out[2])}
+
+
+class Batch:
+    """Several closed domains stepped together: one persistent collide-stream
+    launch over their concatenated tile lists and one set of IBM launches for
+    all rotating bodies (lbm_batch_* in include/lbm_hip.h).  The member
+    solvers keep their state and stay readable; they must not be changed
+    while the batch exists."""
+
+    def __init__(self, solvers):
+        self.solvers = list(solvers)
+        if not self.solvers:
+            raise ValueError("a batch needs at least one solver")
+        self.ctx = self.solvers[0].ctx
+        self.lib = _L.lib()
+        arr = (C.c_void_p * len(self.solvers))(*[sv.h for sv in self.solvers])
+        h = C.c_void_p()
+        self.ctx.check(self.lib.lbm_batch_create(self.ctx.h, arr, len(self.solvers), C.byref(h)))
+        self.h = h
+
+    def info(self):
+        m, t, c, g = C.c_int(), C.c_uint32(), C.c_uint32(), C.c_uint32()
+        self.ctx.check(self.lib.lbm_batch_info(self.h, C.byref(m), C.byref(t), C.byref(c), C.byref(g)))
+        return {"members": m.value, "tiles": t.value, "forced_cells": c.value, "grid": g.value}
+
+    def set_grid(self, grid: int):
+        self.ctx.check(self.lib.lbm_batch_set_grid(self.h, int(grid)))
+
+    def autotune(self, trial_steps: int = 10):
+        g, ms = C.c_uint32(), C.c_double()
+        self.ctx.check(self.lib.lbm_batch_autotune(self.h, int(trial_steps), C.byref(g), C.byref(ms)))
+        return g.value, ms.value
+
+    def run(self, steps: int, graph: bool = True):
+        """``steps`` runner steps of every member; returns torque[members, steps, 3]
+        (zeros when the members carry no rotating body)."""
+        steps = int(steps)
+        tq = np.zeros((len(self.solvers), max(steps, 0), 3))
+        self.ctx.check(self.lib.lbm_batch_run_span(self.h, steps, int(bool(graph)), _dp(tq)))
+        return tq
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.lbm_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
