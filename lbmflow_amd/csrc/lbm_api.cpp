@@ -215,7 +215,15 @@ int rebuild_tiles(lbm_fields* fl) {
       for (uint32_t z = z0; z < std::min(nz, z0 + zb); ++z)
         for (uint32_t sg = 0; sg < nseg; ++sg) {
           const size_t t = ((size_t)z * ny + y) * nseg + sg;
-          if (flags[t]) list.push_back((uint32_t)t | (flags[t] == 2 ? kTilePlain : 0u));
+          const uint8_t fg = flags[t];
+          if (!(fg & 1)) continue;
+          uint32_t e = (uint32_t)t | ((fg & 2) ? kTilePlain : 0u);
+          // an all-solid half of an fp32 tile is one whole line per plane
+          // nobody needs; with 16-bit storage it is half a line, and
+          // leaving it out (a 64-byte partial write) measured slower than
+          // keeping its zero stores (2.83 vs 2.75 ms on the 512^3 tank)
+          if (fl->storage == kF32) e |= ((fg & 4) ? kTileDeadLo : 0u) | ((fg & 8) ? kTileDeadHi : 0u);
+          list.push_back(e);
         }
   if (list.size() > fl->tiles_cap) {
     dev_free(fl, fl->tiles, fl->tiles_cap * 4);

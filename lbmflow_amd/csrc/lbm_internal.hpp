@@ -15,8 +15,12 @@ constexpr uint32_t kMaskSolid = 1u << 31;
 constexpr uint32_t kMaskForced = 1u << 30;  // handled by the force-list kernel
 constexpr uint32_t kMaskMoving = 1u << 29;  // some linked solid has a nonzero wall velocity
 constexpr uint32_t kMaskLinks = (1u << 27) - 1u;
-// tile-list entry flag: every cell of the 64-cell tile has mask word 0
+// tile-list entry flags: every cell of the 64-cell tile has mask word 0;
+// lanes 0-31 / 32-63 are all solid (their slots are dead: skip them)
 constexpr uint32_t kTilePlain = 1u << 31;
+constexpr uint32_t kTileDeadLo = 1u << 30;
+constexpr uint32_t kTileDeadHi = 1u << 29;
+constexpr uint32_t kTileId = kTileDeadHi - 1u;
 
 enum Storage : int { kF32 = 0, kF16 = 1, kBF16 = 2 };
 

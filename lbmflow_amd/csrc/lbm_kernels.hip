@@ -574,10 +574,10 @@ __global__ __launch_bounds__(256) void k_hydro(HydroArgs a) {
   const uint32_t t = b * 4 + (threadIdx.x >> 6);
   if (t >= a.ntiles) return;
   const uint32_t te = __builtin_amdgcn_readfirstlane(a.tiles[t]);  // one tile per wave
-  const uint32_t tile = te & ~kTilePlain;
+  const uint32_t tile = te & kTileId;
   const uint32_t row = tile / a.nseg, seg = tile - row * a.nseg;
   const uint32_t x = seg * 64 + (threadIdx.x & 63);
-  if (x >= a.d.nx) return;
+  if (x >= a.d.nx || (te & ((threadIdx.x & 32) ? kTileDeadHi : kTileDeadLo))) return;
   const uint32_t y = row % a.d.ny, z = row / a.d.ny;
   const uint32_t i = row * a.d.nx + x;
   if (te & kTilePlain)
@@ -631,10 +631,10 @@ __global__ __launch_bounds__(256) void k_hydro_batch(const HydroArgs* __restrict
     const uint32_t te = __builtin_amdgcn_readfirstlane(ent[2 * (size_t)e]);
     const uint32_t k = __builtin_amdgcn_readfirstlane(ent[2 * (size_t)e + 1]);
     const HydroArgs& a = tab[k];
-    const uint32_t tile = te & ~kTilePlain;
+    const uint32_t tile = te & kTileId;
     const uint32_t row = tile / a.nseg, seg = tile - row * a.nseg;
     const uint32_t x = seg * 64 + (threadIdx.x & 63);
-    if (x < a.d.nx) {
+    if (x < a.d.nx && !(te & ((threadIdx.x & 32) ? kTileDeadHi : kTileDeadLo))) {
       const uint32_t y = row % a.d.ny, z = row / a.d.ny;
       const uint32_t i = row * a.d.nx + x;
       if (te & kTilePlain)
@@ -884,21 +884,23 @@ __global__ void k_build_mask(Dom d, const uint8_t* solid, const float* wall_u, u
   mask[i] = m;
 }
 
-// one flag per 64-cell x-segment: 0 nothing for the main kernel, 1 some
-// cell for it, 2 all 64 cells plain fluid (mask word 0)
+// flags per 64-cell x-segment: bit 0 some cell for the main kernel, bit 1
+// all 64 cells plain fluid (mask word 0), bits 2/3 lanes 0-31 / 32-63 all
+// solid (or past the row end)
 __global__ void k_tile_flags(Dom d, const uint32_t* mask, uint8_t* flags) {
   const uint32_t nseg = (d.nx + 63) / 64;
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nseg * d.ny * d.nz) return;
   const uint32_t row = t / nseg, seg = t - row * nseg;
   const uint32_t x0 = seg * 64, x1 = min(x0 + 64, d.nx);
-  bool any = false, plain = x1 - x0 == 64;
+  bool any = false, plain = x1 - x0 == 64, dead[2] = {true, true};
   for (uint32_t x = x0; x < x1; ++x) {
     const uint32_t m = mask[row * d.nx + x];
     if (!(m & (kMaskSolid | kMaskForced))) any = true;
     if (m != 0u) plain = false;
+    if (!(m & kMaskSolid)) dead[(x - x0) >> 5] = false;
   }
-  flags[t] = any ? (plain ? 2 : 1) : 0;
+  flags[t] = (uint8_t)((any ? 1 : 0) | (plain ? 2 : 0) | (dead[0] ? 4 : 0) | (dead[1] ? 8 : 0));
 }
 
 __global__ void k_mark_forced(const uint32_t* flist, uint32_t n, uint32_t* mask) {

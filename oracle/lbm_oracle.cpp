@@ -11,6 +11,8 @@
 //   lattice invariants, omegas, equilibrium fixed point, CM rest fixed point,
 //   uniform-u uniformity, closed-box mass conservation, TGV decay, geometry
 //   counts/volumes, QOI closed forms, IBM single-marker exactness.
+#include <xmmintrin.h>
+
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -1214,6 +1216,22 @@ void orc_cm_transform(int Q, double* raw_from_pop, double* pop_from_raw, uint8_t
     }
     for (int a = 0; a < 3; ++a) basis[3 * m + a] = t.basis[m][a];
   }
+}
+
+// Flush subnormal results and operands to zero on every OpenMP worker and
+// the calling thread (x86 MXCSR FTZ|DAZ).  Long f64 runs otherwise crawl
+// once decaying deviations and tracer tails underflow below 2.2e-308, where
+// x86 arithmetic takes a microcode path; values that small cannot change a
+// quantity of interest.  Off unless a caller asks (fixture generation).
+void orc_flush_subnormals(int on) {
+  const unsigned bits = 0x8040u;  // FTZ (bit 15) | DAZ (bit 6)
+#pragma omp parallel
+  {
+    const unsigned c = _mm_getcsr();
+    _mm_setcsr(on ? (c | bits) : (c & ~bits));
+  }
+  const unsigned c = _mm_getcsr();
+  _mm_setcsr(on ? (c | bits) : (c & ~bits));
 }
 
 void* orc_new(int prec, int Q, const int dims[3], const int periodic[3], int collision, double nu, double magic,

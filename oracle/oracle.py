@@ -44,6 +44,7 @@ def lib():
         L.orc_new.restype = vp
         L.orc_new.argtypes = [C.c_int, C.c_int, i, i, C.c_int, C.c_double, C.c_double, C.c_double, d, u8, d]
         L.orc_free.argtypes = [vp]
+        L.orc_flush_subnormals.argtypes = [C.c_int]
         L.orc_step.argtypes = [vp, C.c_int]
         L.orc_collide_only.argtypes = [vp]
         L.orc_update_moments.argtypes = [vp]
@@ -89,6 +90,11 @@ def _i(a):
 
 def _u8(a):
     return a.ctypes.data_as(C.POINTER(C.c_uint8))
+
+
+def flush_subnormals(on=True):
+    """x86 FTZ|DAZ on the oracle's threads (long f64 runs; see the C++)."""
+    lib().orc_flush_subnormals(int(bool(on)))
 
 
 def lattice(Q):

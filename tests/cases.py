@@ -100,7 +100,7 @@ def tank_case(n, *, nu=0.01, tip=0.05, collision="cm", Q=27, sigma=None):
             "c": c, "blob": blob, "collision": collision, "Q": Q}
 
 
-def run_tank_oracle(tc, steps, prec=4, every=1, scalar_D=None, d3q7=True, cv_every=None):
+def run_tank_oracle(tc, steps, prec=4, every=1, scalar_D=None, d3q7=True, cv_every=None, progress=0):
     s = O.OracleSolver(tc["Q"], (tc["n"],) * 3, prec=prec, collision=COLL[tc["collision"]], nu=tc["nu"],
                        solid=tc["solid"])
     k = None
@@ -110,6 +110,8 @@ def run_tank_oracle(tc, steps, prec=4, every=1, scalar_D=None, d3q7=True, cv_eve
     torque, cv = [], []
     fl = (tc["solid"] == 0).astype(np.uint8)
     for step in range(1, steps + 1):
+        if progress and step % progress == 0:
+            print(f"[oracle] step {step}/{steps}", flush=True)
         s.clear_force_field()
         d = s.apply_ibm(tc["pos"], tc["w"], tc["c"], (0.0, 0.0, tc["omega"]))
         s.step()
