@@ -159,7 +159,7 @@ def main():
     ap.add_argument("--collision", default="cm", choices=["cm", "trt", "bgk"])
     ap.add_argument("--lattice", type=int, default=27, choices=[19, 27])
     ap.add_argument("--cpu-size", type=int, default=128)
-    ap.add_argument("--cpu-steps", type=int, default=10)
+    ap.add_argument("--cpu-steps", type=int, default=100)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-sweep", action="store_true", help="skip the 16-bit storage and Np-accuracy sweep")
     args = ap.parse_args()

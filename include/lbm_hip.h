@@ -254,6 +254,27 @@ int lbm_batch_run_span(lbm_batch* b, int steps, int use_graph, double* torque);
  * keeps the fastest. */
 int lbm_batch_autotune(lbm_batch* b, int trial_steps, uint32_t* best_grid, double* best_ms);
 
+/* ---- adjoint of the time-averaged shaft torque --------------------------
+ * No reference counterpart (the reference differentiates only by finite
+ * differences, crates/lbm-core/src/uq.rs:47-70).  Runs `steps` runner steps
+ * from the current state (as lbm_run_ibm_span) and sets *J to the mean shaft
+ * torque (-torque_z) of the last `window` steps; then, if dJ_domega or
+ * dJ_dnu is non-null, the discrete adjoint of those steps (D3Q19 TRT/BGK
+ * collide-stream with half-way bounce-back, stale-moment Guo forcing and the
+ * direct-forcing sweeps), recomputing the forward trajectory from
+ * `snapshots` binomial checkpoints, gives dJ/dOmega_z of the rotating body
+ * and dJ/dnu per cell (nu = (1/omega_plus - 1/2)/3; omega_minus fixed, as
+ * the omega field does).  Adjoint arithmetic is f64.  The fields end in the
+ * state after the `steps` forward steps.  *forward_steps (nullable) counts
+ * every forward step taken, the first pass included. */
+int lbm_adjoint_torque(lbm_fields* fl, int steps, int window, int snapshots, double* J, double* dJ_domega,
+                       double* dJ_dnu, int64_t* forward_steps);
+/* The checkpoint schedule used above, for inspection: `count` (action,
+ * argument) pairs; action 0 advance arg steps, 1 save the state to slot arg,
+ * 2 restore slot arg, 3 adjoint of step arg.  actions may be null to get the
+ * count only. */
+int lbm_revolve_schedule(int steps, int snapshots, int* actions, int capacity, int* count);
+
 /* ---- passive scalar ADE (solver.rs:2113-2205, scalar.rs:6-15) ------------ */
 int lbm_scalar_enable(lbm_fields* fl, double diffusivity, int lattice, int* index);
 /* set_scalar_concentration, solver.rs:2910-2965 (h = h_eq(C, u)). */

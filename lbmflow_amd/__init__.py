@@ -15,8 +15,9 @@ from .solver import (  # noqa: F401
     omegas,
     power_qois,
     radial_impeller,
+    revolve_schedule,
 )
 from ._lib import LbmError, lib  # noqa: F401
 
 __all__ = ["Solver", "Batch", "Context", "DirectForcingConfig", "omegas", "radial_impeller", "power_qois",
-           "mixing_time_from_cv", "MAGIC_STD", "LbmError", "lib", "default_context"]
+           "mixing_time_from_cv", "MAGIC_STD", "LbmError", "lib", "default_context", "revolve_schedule"]

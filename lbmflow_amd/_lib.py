@@ -98,6 +98,8 @@ SIGNATURES = {
     "lbm_batch_set_grid": (_I, [_P, C.c_uint32]),
     "lbm_batch_run_span": (_I, [_P, _I, _I, _pD]),
     "lbm_batch_autotune": (_I, [_P, _I, _pU32, _pD]),
+    "lbm_adjoint_torque": (_I, [_P, _I, _I, _I, _pD, _pD, _pD, C.POINTER(C.c_int64)]),
+    "lbm_revolve_schedule": (_I, [_I, _I, _pI, _I, _pI]),
     "lbm_scalar_enable": (_I, [_P, _D, _I, _pI]),
     "lbm_scalar_set": (_I, [_P, _I, _pF]),
     "lbm_scalar_gather": (_I, [_P, _I, _pF]),

@@ -1481,3 +1481,217 @@ int lbm_batch_autotune(lbm_batch* b, int trial_steps, uint32_t* best_grid, doubl
 }
 
 }  // extern "C"
+
+// ---- adjoint of the time-averaged shaft torque --------------------------------
+// No reference counterpart (the reference has only finite differences,
+// crates/lbm-core/src/uq.rs:47-70).  Forward: `steps` runner steps from the
+// current state; J = mean shaft torque (-torque_z of the IBM diagnostics,
+// runner.rs:1396-1425) over the last `window` steps.  Reverse: discrete
+// adjoint of every step (collide-stream with bounce-back, the stale-moment
+// force, the direct-forcing sweeps) with a binomial checkpoint schedule.
+
+namespace {
+
+enum { ACT_ADVANCE = 0, ACT_SAVE = 1, ACT_RESTORE = 2, ACT_ADJOINT = 3 };
+
+// C(s + r, s): the most steps s free snapshots can reverse with at most r
+// forward recomputations of any step (Griewank's binomial bound)
+double binom_reach(int s, int r) {
+  double b = 1.0;
+  for (int k = 1; k <= s; ++k) b = b * (double)(r + k) / (double)k;
+  return b;
+}
+
+// Reverse steps [lo, hi); the state at lo is in `slot`, `free_slots` more
+// slots (slot + 1, slot + 2, ...) may be used.  The first checkpoint goes
+// m = n - C(s - 1 + r, s - 1) steps ahead (r the smallest repetition count
+// with C(s + r, s) >= n), so every step is recomputed at most r times.
+void revolve(int lo, int hi, int slot, int free_slots, std::vector<int>& out) {
+  while (hi > lo) {
+    const int n = hi - lo;
+    if (n == 1) {
+      out.insert(out.end(), {ACT_RESTORE, slot, ACT_ADJOINT, lo});
+      return;
+    }
+    if (free_slots == 0) {
+      for (int k = hi - 1; k >= lo; --k) {
+        out.insert(out.end(), {ACT_RESTORE, slot});
+        if (k > lo) out.insert(out.end(), {ACT_ADVANCE, k - lo});
+        out.insert(out.end(), {ACT_ADJOINT, k});
+      }
+      return;
+    }
+    int r = 1;
+    while (binom_reach(free_slots, r) < (double)n) ++r;
+    const double right = binom_reach(free_slots - 1, r);
+    const int m = right >= (double)(n - 1) ? 1 : n - (int)right;
+    out.insert(out.end(), {ACT_RESTORE, slot, ACT_ADVANCE, m, ACT_SAVE, slot + 1});
+    revolve(lo + m, hi, slot + 1, free_slots - 1, out);
+    hi = lo + m;
+  }
+}
+
+struct Snapshot {
+  void* f = nullptr;
+  float* F_mom = nullptr;
+  int parity = 0;
+  int64_t time = 0;
+};
+
+int snap_save(lbm_fields* fl, Snapshot& sn) {
+  hipStream_t s = fl->ctx->stream;
+  if (!sn.f) HIPCHK(fl->ctx, hipMalloc(&sn.f, fl->f_bytes));
+  if (!sn.F_mom) HIPCHK(fl->ctx, hipMalloc((void**)&sn.F_mom, N3(fl) * 4));
+  HIPCHK(fl->ctx, hipMemcpyAsync(sn.f, fl->f_alloc, fl->f_bytes, hipMemcpyDeviceToDevice, s));
+  HIPCHK(fl->ctx, hipMemcpyAsync(sn.F_mom, fl->F_mom, N3(fl) * 4, hipMemcpyDeviceToDevice, s));
+  sn.parity = fl->parity;
+  sn.time = fl->time;
+  return LBM_OK;
+}
+
+int snap_restore(lbm_fields* fl, const Snapshot& sn) {
+  hipStream_t s = fl->ctx->stream;
+  HIPCHK(fl->ctx, hipMemcpyAsync(fl->f_alloc, sn.f, fl->f_bytes, hipMemcpyDeviceToDevice, s));
+  HIPCHK(fl->ctx, hipMemcpyAsync(fl->F_mom, sn.F_mom, N3(fl) * 4, hipMemcpyDeviceToDevice, s));
+  fl->parity = sn.parity;
+  fl->time = sn.time;
+  return LBM_OK;
+}
+
+void snap_free(Snapshot& sn) {
+  if (sn.f) (void)hipFree(sn.f);
+  if (sn.F_mom) (void)hipFree(sn.F_mom);
+  sn = Snapshot{};
+}
+
+}  // namespace
+
+extern "C" {
+
+int lbm_revolve_schedule(int steps, int snapshots, int* actions, int capacity, int* count) {
+  if (steps < 1 || snapshots < 0 || !count) return LBM_ERR_INVALID;
+  std::vector<int> out;
+  revolve(0, steps, 0, snapshots, out);
+  *count = (int)(out.size() / 2);
+  if (actions) {
+    if (capacity < *count) return LBM_ERR_INVALID;
+    std::copy(out.begin(), out.end(), actions);
+  }
+  return LBM_OK;
+}
+
+int lbm_adjoint_torque(lbm_fields* fl, int steps, int window, int snapshots, double* J, double* dJ_domega,
+                       double* dJ_dnu, int64_t* forward_steps) {
+  if (!fl || steps < 1 || window < 1 || window > steps || snapshots < 0 || !J) return LBM_ERR_INVALID;
+  if (!fl->ibm) return fail(fl->ctx, LBM_ERR_STATE, "the torque adjoint needs a rotating body");
+  if (fl->Q != 19 || fl->collision == LBM_COLLISION_CM)
+    return fail(fl->ctx, LBM_ERR_UNSUPPORTED, "the adjoint is implemented for D3Q19 TRT/BGK");
+  if (fl->wall_u) return fail(fl->ctx, LBM_ERR_UNSUPPORTED, "the adjoint does not cover moving walls");
+  if (fl->hp.force_on) return fail(fl->ctx, LBM_ERR_UNSUPPORTED, "the adjoint does not cover a body force");
+  int rc = check_steppable(fl);
+  if (rc) return rc;
+  hipStream_t s = fl->ctx->stream;
+  const size_t N = fl->d.N, Q = (size_t)fl->Q;
+  if (fl->field_present && fl->field_uniform) {
+    rc = lbm_clear_force_field(fl);
+    if (rc) return rc;
+  }
+  {
+    const IbmDomain D = ibm_domain_of(view_of(fl));
+    rc = ibm_set_domains(view_of(fl), fl->ibm, &D);
+    if (rc) return rc;
+  }
+  std::vector<Snapshot> snap((size_t)snapshots + 1);
+  Snapshot fin;
+  double *lamA = nullptr, *lamB = nullptr, *obar = nullptr, *Fbar = nullptr, *G0 = nullptr, *G1 = nullptr,
+         *wbar = nullptr;
+  auto cleanup = [&]() {
+    for (Snapshot& sn : snap) snap_free(sn);
+    snap_free(fin);
+    for (double* p : {lamA, lamB, obar, Fbar, G0, G1, wbar})
+      if (p) (void)hipFree(p);
+  };
+#define ADJ_TRY(expr)   \
+  do {                  \
+    int _r = (expr);    \
+    if (_r) {           \
+      cleanup();        \
+      return _r;        \
+    }                   \
+  } while (0)
+  // forward pass: J
+  ADJ_TRY(snap_save(fl, snap[0]));
+  std::vector<double> tq(3 * (size_t)steps);
+  ADJ_TRY(lbm_run_ibm_span(fl, steps, 0, tq.data(), nullptr));
+  double acc = 0.0;
+  for (int k = steps - window; k < steps; ++k) acc += -tq[3 * (size_t)k + 2];
+  *J = acc / window;
+  ADJ_TRY(snap_save(fl, fin));
+  if (!dJ_domega && !dJ_dnu) {
+    cleanup();
+    return LBM_OK;
+  }
+  // reverse pass
+  auto dalloc = [&](double** p, size_t n) -> int {
+    if (hipMalloc((void**)p, n * sizeof(double)) != hipSuccess)
+      return fail(fl->ctx, LBM_ERR_NOMEM, "adjoint buffer of %zu doubles", n);
+    HIPCHK(fl->ctx, launch_fill_f64(*p, n, 0.0, s));
+    return LBM_OK;
+  };
+  ADJ_TRY(dalloc(&lamA, Q * N));
+  ADJ_TRY(dalloc(&lamB, Q * N));
+  ADJ_TRY(dalloc(&obar, 1));
+  ADJ_TRY(dalloc(&Fbar, 3 * N));
+  ADJ_TRY(dalloc(&G0, 3 * N));
+  ADJ_TRY(dalloc(&G1, 3 * N));
+  ADJ_TRY(dalloc(&wbar, N));
+  std::vector<int> plan;
+  revolve(0, steps, 0, snapshots, plan);
+  int64_t advanced = steps;
+  const double tq_bar = 1.0 / window;
+  for (size_t a = 0; a < plan.size(); a += 2) {
+    const int op = plan[a], arg = plan[a + 1];
+    if (op == ACT_RESTORE) {
+      ADJ_TRY(snap_restore(fl, snap[arg]));
+    } else if (op == ACT_SAVE) {
+      ADJ_TRY(snap_save(fl, snap[arg]));
+    } else if (op == ACT_ADVANCE) {
+      for (int k = 0; k < arg; ++k) ADJ_TRY(ibm_step(fl));
+      advanced += arg;
+    } else {  // adjoint of step `arg`; the solver holds the state before it
+      if (!fl->flist_host.empty())
+        HIPCHK(fl->ctx, launch_copy_force_list(fl->flist, (uint32_t)fl->flist_host.size(), fl->F_cur, nullptr, s));
+      fl->field_present = false;
+      ADJ_TRY(ibm_enqueue_recorded(view_of(fl), fl->ibm));
+      fl->field_present = true;
+      HIPCHK(fl->ctx, launch_adj_collide_stream(fl->storage, fl->parity, fl->d, fl->f, fl->mask, fl->omega,
+                                                fl->F_cur, fl->F_mom, fl->hp, lamA, lamB, wbar, Fbar, G1, s));
+      ADJ_TRY(ibm_adjoint(view_of(fl), fl->ibm, Fbar, G0, arg >= steps - window ? tq_bar : 0.0, lamB, obar));
+      std::swap(lamA, lamB);
+      std::swap(G0, G1);
+    }
+  }
+  double ob = 0.0;
+  std::vector<double> wb(dJ_dnu ? N : 0);
+  HIPCHK(fl->ctx, hipMemcpyAsync(&ob, obar, sizeof ob, hipMemcpyDeviceToHost, s));
+  if (dJ_dnu) HIPCHK(fl->ctx, hipMemcpyAsync(wb.data(), wbar, N * sizeof(double), hipMemcpyDeviceToHost, s));
+  std::vector<float> om(fl->omega && dJ_dnu ? N : 0);
+  if (!om.empty()) HIPCHK(fl->ctx, hipMemcpyAsync(om.data(), fl->omega, N * 4, hipMemcpyDeviceToHost, s));
+  HIPCHK(fl->ctx, hipStreamSynchronize(s));
+  if (dJ_domega) *dJ_domega = ob;
+  if (dJ_dnu) {
+    // omega_plus = 1 / (3 nu + 1/2): d omega_plus / d nu = -3 omega_plus^2
+    for (size_t i = 0; i < N; ++i) {
+      const double op = om.empty() ? (double)fl->hp.omega_p : (double)om[i];
+      dJ_dnu[i] = wb[i] * (-3.0 * op * op);
+    }
+  }
+  if (forward_steps) *forward_steps = advanced;
+  ADJ_TRY(snap_restore(fl, fin));
+  HIPCHK(fl->ctx, hipStreamSynchronize(s));
+  cleanup();
+#undef ADJ_TRY
+  return LBM_OK;
+}
+
+}  // extern "C"

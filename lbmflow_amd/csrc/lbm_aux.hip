@@ -105,6 +105,16 @@ struct IbmState {
   double* mslip = nullptr;   // M
   uint8_t* mact = nullptr;   // M
   IbmDevDiag* diag = nullptr;  // K
+  // adjoint (single body): per-sweep impulses and du recorded by the
+  // forward update, and the reverse-sweep work arrays
+  double* imp_hist = nullptr;  // max_iterations x M x 3
+  double* du_hist = nullptr;   // max_iterations x U x 3
+  double* a_T = nullptr;       // M x 3  adjoint of the target velocity
+  double* a_um = nullptr;      // M x 3  adjoint of the interpolated velocity
+  double* a_mob = nullptr;     // M      adjoint of the mobility
+  double* a_du = nullptr;      // U x 3
+  double* a_unow = nullptr;    // U x 3
+  double* a_rho = nullptr;     // U
 };
 
 // u_now = (m + F/2) / rho at the stencil cells, rho the cached density
@@ -356,7 +366,8 @@ void ibm_destroy(IbmState* s) {
   void* ptrs[] = {s->moff,    s->cen,    s->scale,   s->dom,    s->pos,    s->weight, s->target, s->mk_off,
                   s->mk_inst, s->pt_u,   s->pt_m,    s->pt_w,   s->uc_cell, s->uc_inst, s->uc_off, s->uc_ent,
                   s->ksum,    s->unow,   s->rho,     s->du,     s->spread, s->imp,    s->mrep,   s->mtq,
-                  s->mslip,   s->mact,   s->diag};
+                  s->mslip,   s->mact,   s->diag,    s->imp_hist, s->du_hist, s->a_T, s->a_um, s->a_mob,
+                  s->a_du,    s->a_unow, s->a_rho};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete s;
@@ -589,7 +600,20 @@ static void diag_to_abi(const IbmDevDiag& h, lbm_ibm_diag* out) {
   out->momentum_error_rel = h.mom_err;
 }
 
+__global__ void k_copy_d(double* dst, const double* src, size_t n) {
+  const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) dst[k] = src[k];
+}
+
+static int ibm_enqueue_impl(const FieldsView& v, IbmState* s, void* ring, uint32_t cap, uint32_t* pos,
+                            bool record);
+
 int ibm_enqueue(const FieldsView& v, IbmState* s, void* ring, uint32_t cap, uint32_t* pos) {
+  return ibm_enqueue_impl(v, s, ring, cap, pos, false);
+}
+
+static int ibm_enqueue_impl(const FieldsView& v, IbmState* s, void* ring, uint32_t cap, uint32_t* pos,
+                            bool record) {
   if (!s->dom_valid) {
     if (v.ctx_err) *v.ctx_err = "IBM domain table not set";
     return LBM_ERR_STATE;
@@ -616,8 +640,14 @@ int ibm_enqueue(const FieldsView& v, IbmState* s, void* ring, uint32_t cap, uint
   AUXCHK(v, hipGetLastError());
   const uint32_t gm = nblk((size_t)s->M), gu = nblk(s->U);
   for (int it = 0; it < s->cfg.max_iterations; ++it) {
+    if (record && s->U)
+      hipLaunchKernelGGL(k_copy_d, dim3(nblk(3 * (size_t)s->U)), dim3(256), 0, st,
+                         s->du_hist + 3 * (size_t)s->U * it, s->du, 3 * (size_t)s->U);
     hipLaunchKernelGGL(k_ibm_impulse, dim3(gm), dim3(256), 0, st, s->M, s->mk_off, s->mk_inst, s->pt_u, s->pt_w,
                        s->target, s->unow, s->du, s->rho, s->ksum, s->cfg.relaxation, s->imp, s->diag);
+    if (record)
+      hipLaunchKernelGGL(k_copy_d, dim3(nblk(3 * (size_t)s->M)), dim3(256), 0, st,
+                         s->imp_hist + 3 * (size_t)s->M * it, s->imp, 3 * (size_t)s->M);
     if (s->U)
       hipLaunchKernelGGL(k_ibm_spread, dim3(gu), dim3(256), 0, st, s->U, s->uc_off, s->uc_inst, s->uc_ent, s->pt_m,
                          s->pt_w, s->imp, s->rho, s->spread, s->du, s->diag);
@@ -636,6 +666,199 @@ int ibm_enqueue(const FieldsView& v, IbmState* s, void* ring, uint32_t cap, uint
     hipLaunchKernelGGL(k_ibm_record, dim3(1), dim3(64), 0, st, s->diag, s->K, (IbmDevDiag*)ring, cap, pos);
     AUXCHK(v, hipGetLastError());
   }
+  return LBM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint of one direct-forcing update (single body).  Forward, per sweep s
+// (solver.rs:3107-3208): um = sum_p w (unow + du^s); mob = sum_p w ksum/rho;
+// imp^s = relax (T - um)/mob; spread += imp^s w; du^{s+1} = du^s + imp^s w/rho;
+// shaft torque += sum_m W_m (r x imp^s)_z, W_m = sum_p w.  Output F = spread.
+// Reverse, for the recorded sweeps s = S-1..0, with Fbar = adjoint of F and
+// tq_bar = adjoint of the shaft torque; then through unow = m/rho and
+// rho = 1 + sum f, m = sum c f into the population adjoint.
+// ---------------------------------------------------------------------------
+__global__ void k_ibm_adj_zero(uint32_t U, int64_t M, double* a_du, double* a_unow, double* a_rho, double* a_T,
+                               double* a_mob) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < (int64_t)U) {
+    for (int a = 0; a < 3; ++a) a_du[3 * k + a] = a_unow[3 * k + a] = 0.0;
+    a_rho[k] = 0.0;
+  }
+  if (k < M) {
+    for (int a = 0; a < 3; ++a) a_T[3 * k + a] = 0.0;
+    a_mob[k] = 0.0;
+  }
+}
+
+// per marker, sweep s: impulse adjoint -> target, interpolated velocity and
+// mobility adjoints
+__global__ void k_ibm_adj_marker(int s, int64_t M, const uint32_t* mk_off, const uint32_t* pt_u, const double* pt_w,
+                                 const uint32_t* uc_cell, const double* Fbar, const double* Gbar, const double* rho,
+                                 const double* ksum, const double* a_du, const double* imp_hist, const double* pos,
+                                 const double* cen, double relax, double tq_bar, const IbmDevDiag* diag, double* a_T,
+                                 double* a_um, double* a_mob) {
+  const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= M || s >= diag->iterations) return;
+  double ib[3] = {0.0, 0.0, 0.0}, W = 0.0, mob = 0.0;
+  for (uint32_t p = mk_off[m]; p < mk_off[m + 1]; ++p) {
+    const uint32_t u = pt_u[p];
+    const double w = pt_w[p];
+    const size_t i = uc_cell[u];
+    const double ir = 1.0 / fmax(rho[u], 1.0e-30);
+    for (int a = 0; a < 3; ++a) {
+      double fb = Fbar[3 * i + a];
+      if (Gbar) fb += Gbar[3 * i + a];
+      ib[a] += w * (fb + a_du[3 * (size_t)u + a] * ir);
+    }
+    W += w;
+    mob += w * ksum[u] / fmax(rho[u], 1.0e-30);
+  }
+  const double rx = pos[3 * m] - cen[0], ry = pos[3 * m + 1] - cen[1];
+  ib[0] += tq_bar * W * -ry;
+  ib[1] += tq_bar * W * rx;
+  const double* imp = imp_hist + 3 * (size_t)M * s + 3 * m;
+  if (mob == 0.0) {
+    for (int a = 0; a < 3; ++a) a_um[3 * m + a] = 0.0;
+    return;
+  }
+  double dot = 0.0;
+  for (int a = 0; a < 3; ++a) {
+    a_T[3 * m + a] += relax / mob * ib[a];
+    a_um[3 * m + a] = -relax / mob * ib[a];
+    dot += imp[a] * ib[a];
+  }
+  a_mob[m] += -dot / mob;
+}
+
+// per stencil cell, sweep s: rho adjoint of the du update, then the
+// interpolated-velocity adjoint gathered in marker order
+__global__ void k_ibm_adj_cell(int s, uint32_t U, int64_t M, const uint32_t* uc_off, const uint32_t* uc_ent,
+                               const uint32_t* pt_m, const double* pt_w, const double* rho, const double* imp_hist,
+                               const double* a_um, const IbmDevDiag* diag, double* a_du, double* a_unow,
+                               double* a_rho) {
+  const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= U || s >= diag->iterations) return;
+  const double r = fmax(rho[u], 1.0e-30);
+  double sw[3] = {0.0, 0.0, 0.0}, g[3] = {0.0, 0.0, 0.0};
+  for (uint32_t e = uc_off[u]; e < uc_off[u + 1]; ++e) {
+    const uint32_t p = uc_ent[e];
+    const uint32_t m = pt_m[p];
+    const double w = pt_w[p];
+    for (int a = 0; a < 3; ++a) {
+      sw[a] += imp_hist[3 * (size_t)M * s + 3 * (size_t)m + a] * w;
+      g[a] += w * a_um[3 * (size_t)m + a];
+    }
+  }
+  double rb = 0.0;
+  for (int a = 0; a < 3; ++a) rb += a_du[3 * (size_t)u + a] * sw[a];
+  a_rho[u] += -rb / (r * r);
+  for (int a = 0; a < 3; ++a) {
+    a_unow[3 * (size_t)u + a] += g[a];
+    a_du[3 * (size_t)u + a] += g[a];
+  }
+}
+
+// per stencil cell, after the sweeps: mobility adjoint into rho, then
+// unow = m/rho and the moments into the population adjoint (q-major f64)
+template <int Q>
+__global__ void k_ibm_adj_final(uint32_t U, uint32_t N, const uint32_t* uc_cell, const uint32_t* uc_off,
+                                const uint32_t* uc_ent, const uint32_t* pt_m, const double* pt_w, const double* rho,
+                                const double* unow, const double* ksum, const double* a_mob, const double* a_unow,
+                                const double* a_rho, double* lam) {
+  const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= U) return;
+  const double r = fmax(rho[u], 1.0e-30);
+  double rb = a_rho[u];
+  for (uint32_t e = uc_off[u]; e < uc_off[u + 1]; ++e) {
+    const uint32_t p = uc_ent[e];
+    rb += -pt_w[p] * ksum[u] * a_mob[pt_m[p]] / (r * r);
+  }
+  double mb[3];
+  double ud = 0.0;
+  for (int a = 0; a < 3; ++a) {
+    ud += unow[3 * (size_t)u + a] * a_unow[3 * (size_t)u + a];
+    mb[a] = a_unow[3 * (size_t)u + a] / r;
+  }
+  rb += -ud / r;
+  const size_t i = uc_cell[u];
+  sfor<0, Q>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    double v = rb;
+    if constexpr (lat_c(q, 0) != 0) v += (double)lat_c(q, 0) * mb[0];
+    if constexpr (lat_c(q, 1) != 0) v += (double)lat_c(q, 1) * mb[1];
+    if constexpr (lat_c(q, 2) != 0) v += (double)lat_c(q, 2) * mb[2];
+    lam[(size_t)q * N + i] += v;
+  });
+}
+
+// one block: rotation-rate adjoint, T = Omega x r with Omega = (0, 0, W):
+// dT/dW = (-r_y, r_x, 0); fixed-tree sum over markers, added to *out
+__global__ __launch_bounds__(256) void k_ibm_adj_omega(int64_t M, const double* pos, const double* cen,
+                                                       const double* a_T, double* out) {
+  __shared__ double sh[256];
+  double acc = 0.0;
+  for (int64_t m = threadIdx.x; m < M; m += 256)
+    acc += -(pos[3 * m + 1] - cen[1]) * a_T[3 * m] + (pos[3 * m] - cen[0]) * a_T[3 * m + 1];
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out += sh[0];
+}
+
+int ibm_adjoint_prepare(const FieldsView& v, IbmState* s) {
+  if (s->K != 1) {
+    if (v.ctx_err) *v.ctx_err = "the IBM adjoint takes a single body";
+    return LBM_ERR_UNSUPPORTED;
+  }
+  if (s->imp_hist) return LBM_OK;
+  const size_t it = (size_t)s->cfg.max_iterations;
+  AUXCHK(v, scratch(&s->imp_hist, it * 3 * (size_t)s->M));
+  AUXCHK(v, scratch(&s->du_hist, it * 3 * (size_t)s->U));
+  AUXCHK(v, scratch(&s->a_T, 3 * (size_t)s->M));
+  AUXCHK(v, scratch(&s->a_um, 3 * (size_t)s->M));
+  AUXCHK(v, scratch(&s->a_mob, s->M));
+  AUXCHK(v, scratch(&s->a_du, 3 * (size_t)s->U));
+  AUXCHK(v, scratch(&s->a_unow, 3 * (size_t)s->U));
+  AUXCHK(v, scratch(&s->a_rho, s->U));
+  return LBM_OK;
+}
+
+int ibm_enqueue_recorded(const FieldsView& v, IbmState* s) {
+  int rc = ibm_adjoint_prepare(v, s);
+  if (rc) return rc;
+  return ibm_enqueue_impl(v, s, nullptr, 0, nullptr, true);
+}
+
+int ibm_adjoint(const FieldsView& v, IbmState* s, const double* Fbar, const double* Gbar, double tq_bar, double* lam,
+                double* omega_bar) {
+  int rc = ibm_adjoint_prepare(v, s);
+  if (rc) return rc;
+  hipStream_t st = v.stream;
+  const uint32_t gm = nblk((size_t)s->M), gu = nblk(s->U), gz = nblk(std::max<size_t>(s->U, (size_t)s->M));
+  hipLaunchKernelGGL(k_ibm_adj_zero, dim3(gz), dim3(256), 0, st, s->U, s->M, s->a_du, s->a_unow, s->a_rho, s->a_T,
+                     s->a_mob);
+  for (int it = s->cfg.max_iterations - 1; it >= 0; --it) {
+    hipLaunchKernelGGL(k_ibm_adj_marker, dim3(gm), dim3(256), 0, st, it, s->M, s->mk_off, s->pt_u, s->pt_w,
+                       s->uc_cell, Fbar, Gbar, s->rho, s->ksum, s->a_du, s->imp_hist, s->pos, s->cen,
+                       s->cfg.relaxation, tq_bar, s->diag, s->a_T, s->a_um, s->a_mob);
+    if (s->U)
+      hipLaunchKernelGGL(k_ibm_adj_cell, dim3(gu), dim3(256), 0, st, it, s->U, s->M, s->uc_off, s->uc_ent, s->pt_m,
+                         s->pt_w, s->rho, s->imp_hist, s->a_um, s->diag, s->a_du, s->a_unow, s->a_rho);
+  }
+  if (s->U) {
+    if (v.Q == 19)
+      hipLaunchKernelGGL((k_ibm_adj_final<19>), dim3(gu), dim3(256), 0, st, s->U, v.d.N, s->uc_cell, s->uc_off,
+                         s->uc_ent, s->pt_m, s->pt_w, s->rho, s->unow, s->ksum, s->a_mob, s->a_unow, s->a_rho, lam);
+    else
+      hipLaunchKernelGGL((k_ibm_adj_final<27>), dim3(gu), dim3(256), 0, st, s->U, v.d.N, s->uc_cell, s->uc_off,
+                         s->uc_ent, s->pt_m, s->pt_w, s->rho, s->unow, s->ksum, s->a_mob, s->a_unow, s->a_rho, lam);
+  }
+  hipLaunchKernelGGL(k_ibm_adj_omega, dim3(1), dim3(256), 0, st, s->M, s->pos, s->cen, s->a_T, omega_bar);
+  AUXCHK(v, hipGetLastError());
   return LBM_OK;
 }
 

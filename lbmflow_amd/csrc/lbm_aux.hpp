@@ -61,6 +61,15 @@ int ibm_apply(const FieldsView& v, IbmState* s, lbm_ibm_diag* diag);
 // *pos advances.  Launches kernels only, so a stream capture can record it.
 int ibm_enqueue(const FieldsView& v, IbmState* s, void* ring, uint32_t cap, uint32_t* pos);
 size_t ibm_ring_entry_bytes();
+// Adjoint support (single body).  ibm_enqueue_recorded is ibm_enqueue that
+// also keeps every sweep's impulses and du for ibm_adjoint, which runs the
+// sweeps backwards: Fbar (+ Gbar, nullable) is the adjoint of the force
+// field (dense N x 3 f64), tq_bar the adjoint of the update's shaft torque
+// (-torque_z); it adds the population adjoint at the stencil cells to lam
+// (Q x N f64, q-major) and the rotation-rate adjoint to *omega_bar (device).
+int ibm_enqueue_recorded(const FieldsView& v, IbmState* s);
+int ibm_adjoint(const FieldsView& v, IbmState* s, const double* Fbar, const double* Gbar, double tq_bar, double* lam,
+                double* omega_bar);
 // n ring records (entry-major, instance-minor) to ABI diagnostics
 int ibm_ring_read(const FieldsView& v, const void* ring, uint32_t n, lbm_ibm_diag* out);
 

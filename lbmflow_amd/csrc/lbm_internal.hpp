@@ -76,6 +76,12 @@ hipError_t launch_hydro_batch(int Q, int storage, int collision, int odd, const 
 hipError_t launch_hydro_list_batch(int Q, int storage, int collision, int odd, const HydroArgs* tab,
                                    const uint32_t* cells, uint32_t n, hipStream_t s);
 hipError_t launch_clear_batch(const HydroArgs* tab, const uint32_t* cells, uint32_t n, hipStream_t s);
+// Adjoint of one D3Q19 TRT/BGK collide-stream step (lbm_adjoint.hip).
+hipError_t launch_adj_collide_stream(int storage, int odd, const Dom& d, const void* f, const uint32_t* mask,
+                                     const float* omega, const float* F_cur, const float* F_mom,
+                                     const HydroParams& p, const double* lam_in, double* lam_out, double* omega_bar,
+                                     double* Fbar, double* Gout, hipStream_t s);
+hipError_t launch_fill_f64(double* p, size_t n, double v, hipStream_t s);
 hipError_t launch_build_mask(int Q, const Dom& d, const uint8_t* solid, const float* wall_u, uint32_t* mask,
                              unsigned int* err_count, hipStream_t s);
 hipError_t launch_mark_forced(const uint32_t* flist, uint32_t n, uint32_t* mask, hipStream_t s);

@@ -321,6 +321,23 @@ class Solver:
         self._chk(self.lib.lbm_run_ibm_span(self.h, steps, int(bool(graph)), _dp(tq), C.byref(d)))
         return tq, (_diag_dict(d) if steps > 0 else None)
 
+    def adjoint_torque(self, steps: int, window: int, snapshots: int = 16, gradients: bool = True):
+        """Mean shaft torque J over the last ``window`` of ``steps`` runner
+        steps from the current state and, with ``gradients``, its discrete
+        adjoint: dJ/dOmega_z of the rotating body and dJ/dnu per cell
+        (lbm_adjoint_torque).  The solver ends after the ``steps`` steps."""
+        J, dw = C.c_double(), C.c_double()
+        fw = C.c_int64()
+        dnu = np.zeros(self.n) if gradients else None
+        self._chk(self.lib.lbm_adjoint_torque(self.h, int(steps), int(window), int(snapshots), C.byref(J),
+                                              C.byref(dw) if gradients else None,
+                                              _dp(dnu) if gradients else None, C.byref(fw)))
+        out = {"J": J.value, "forward_steps": fw.value}
+        if gradients:
+            out["dJ_domega"] = dw.value
+            out["dJ_dnu"] = dnu
+        return out
+
     # ---- scalar distributions ----------------------------------------------
     def enable_scalar_distribution(self, diffusivity: float, lattice: str = "hydro") -> int:
         idx = C.c_int()
@@ -440,3 +457,19 @@ class Batch:
 
     def __del__(self):
         self.close()
+
+
+def revolve_schedule(steps: int, snapshots: int):
+    """The binomial checkpoint schedule of the torque adjoint as a list of
+    (action, argument): 'advance' n, 'save' slot, 'restore' slot, 'adjoint' step."""
+    lib = _L.lib()
+    n = C.c_int()
+    rc = lib.lbm_revolve_schedule(int(steps), int(snapshots), None, 0, C.byref(n))
+    if rc != 0:
+        raise _L.LbmError(rc, "invalid schedule request")
+    buf = np.zeros(2 * n.value, np.int32)
+    rc = lib.lbm_revolve_schedule(int(steps), int(snapshots), _ip(buf), n.value, C.byref(n))
+    if rc != 0:
+        raise _L.LbmError(rc, "schedule")
+    names = ["advance", "save", "restore", "adjoint"]
+    return [(names[buf[2 * k]], int(buf[2 * k + 1])) for k in range(n.value)]
