@@ -149,11 +149,136 @@ def np_error_vs_f64(storages, collision, Q):
     return out
 
 
+def splitmix64(seed):
+    """Deterministic uniform draws in [0, 1) (SplitMix64)."""
+    state = seed & 0xFFFFFFFFFFFFFFFF
+    while True:
+        state = (state + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+        z = state
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
+        z ^= z >> 31
+        yield (z >> 11) / float(1 << 53)
+
+
+def small_tank(n, tip, nu, Q=19, collision="trt"):
+    """A tank at the benchmark's ratios on an n^3 grid, tip speed `tip`."""
+    import lbmflow_amd as L
+
+    T = 15.0 * n / 16.0
+    D = T / 3.0
+    omega = tip / (0.5 * D)
+    s = L.Solver(Q, (n, n, n), nu=nu, collision=collision)
+    s.build_stirred_tank(1.0, T, T, 4, T / 10.0, T / 48.0)
+    pos, w, vel, c = L.radial_impeller((n, n, n), 1.0, T, T, D, T / 3.0, 6, 0.0, omega)
+    s.set_rotating_body(pos, w, c, (0.0, 0.0, omega))
+    return s, {"n": n, "T": T, "D": D, "omega": omega, "tip": tip, "nu": nu, "markers": len(w)}
+
+
+def workload_batch(args):
+    """64 tanks in one persistent launch (edges U[48,160], tip U[0.02,0.06],
+    nu U[0.002,0.02], D3Q19 TRT fp32), graph-captured step, autotuned grid."""
+    import lbmflow_amd as L
+
+    rng = splitmix64(args.seed)
+    specs = [(48 + int(next(rng) * 113), 0.02 + 0.04 * next(rng), 0.002 + 0.018 * next(rng)) for _ in range(64)]
+    t0 = time.perf_counter()
+    solvers, meta = [], []
+    for n, tip, nu in specs:
+        sv, m = small_tank(n, tip, nu)
+        solvers.append(sv)
+        meta.append(m)
+    batch = L.Batch(solvers)
+    grid, trial_ms = batch.autotune(10)
+    setup_s = time.perf_counter() - t0
+    fluid = sum(sv.fluid_cells() for sv in solvers)
+    total = sum(m["n"] ** 3 for m in meta)
+    print(f"[bench] batch setup {setup_s:.1f}s  tanks=64  cells={total}  fluid={int(fluid)}  grid={grid}",
+          file=sys.stderr, flush=True)
+    batch.run(args.warmup, graph=True)
+    solvers[0].synchronize()
+    t0 = time.perf_counter()
+    tq = batch.run(args.steps, graph=True)
+    solvers[0].synchronize()
+    dt = time.perf_counter() - t0
+    nps = []
+    for k, m in enumerate(meta):
+        tz = -tq[k, :, 2]
+        tmean = float(np.mean(tz[len(tz) // 2:]))
+        nps.append(L.power_qois(tmean, m["omega"], 1.0, m["D"], 1.0)["np"])
+    batch.close()
+    return {
+        "metric": "aggregate MLUPS, 64-tank batched sweep (D3Q19 TRT fp32, one persistent launch)",
+        "value": round(fluid * args.steps / dt / 1e6, 2), "unit": "MLUPS", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(1e3 * dt / args.steps, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": f"synthetic (SplitMix64 seed {args.seed}: edges U[48,160], tip U[0.02,0.06], nu U[0.002,0.02])",
+        "config": {"workload": "64 stirred tanks, D3Q19 TRT, rotating IBM, hipGraph step", "tanks": 64,
+                   "cells": int(total), "fluid_cells": int(fluid), "persistent_grid": grid,
+                   "autotune_trial_ms": round(trial_ms, 4), "parallelism": "single GPU"},
+        "mlups_total": round(total * args.steps / dt / 1e6, 2),
+        "per_instance": [{"n": m["n"], "tip": round(m["tip"], 5), "nu": round(m["nu"], 6), "np": round(v, 5)}
+                         for m, v in zip(meta, nps)],
+    }
+
+
+def workload_adjoint(args):
+    """Adjoint of the time-averaged torque: 128^3 D3Q19 TRT tank, nu = 0.01
+    +-10 % smooth seeded field, 4000 steps, window 1000, binomial
+    checkpointing with 64 snapshots; checked by central differences."""
+    n, steps, window, snaps = 128, args.steps, args.steps // 4, 64
+    rng = splitmix64(args.seed)
+    ph = [2 * np.pi * next(rng) for _ in range(3)]
+    z, y, x = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    x, y, z = x.reshape(-1), y.reshape(-1), z.reshape(-1)
+    field = np.sin(2 * np.pi * x / n + ph[0]) * np.cos(2 * np.pi * y / n + ph[1]) * np.sin(2 * np.pi * z / n + ph[2])
+    nu = 0.01 * (1.0 + 0.1 * field)
+
+    def make(omega_scale=1.0, nuf=nu):
+        s, m = small_tank(n, 0.05 * omega_scale, 0.01)
+        s.set_omega_field((1.0 / (3.0 * nuf + 0.5)).astype(np.float32))
+        return s, m
+
+    s, m = make()
+    t0 = time.perf_counter()
+    res = s.adjoint_torque(steps, window, snaps)
+    s.synchronize()
+    dt = time.perf_counter() - t0
+    solid = s.solid()
+    s.close()
+
+    def J(scale=1.0, nuf=nu):
+        sv, _ = make(scale, nuf)
+        v = sv.adjoint_torque(steps, window, 0, gradients=False)["J"]
+        sv.close()
+        return v
+
+    eps = 1e-2
+    fd_w = (J(1 + eps) - J(1 - eps)) / (2 * eps * m["omega"])
+    d = 0.01 * np.cos(2 * np.pi * (x + 2 * y) / n) * np.sin(2 * np.pi * z / n) * (solid == 0)
+    fd_nu = (J(1.0, nu + eps * d) - J(1.0, nu - eps * d)) / (2 * eps)
+    ad_nu = float(np.dot(res["dJ_dnu"], d))
+    return {
+        "metric": "torque adjoint: time for forward + reverse, 128^3 D3Q19 TRT tank with rotating IBM",
+        "value": round(dt, 3), "unit": "s", "n_gpus": 1, "steps": steps, "warmup": 0,
+        "ms_per_step": round(1e3 * dt / steps, 4), "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 forward, f64 adjoint",
+        "data": f"synthetic (tank ratios of the benchmark, nu field seed {args.seed})",
+        "config": {"workload": "adjoint of mean shaft torque, window %d, %d binomial snapshots" % (window, snaps),
+                   "grid": [n] * 3, "forward_steps_total": res["forward_steps"], "parallelism": "single GPU"},
+        "J": res["J"],
+        "dJ_domega": {"adjoint": res["dJ_domega"], "central_fd_fp32": fd_w,
+                      "rel_diff": abs(res["dJ_domega"] - fd_w) / abs(fd_w)},
+        "dJ_dnu_directional": {"adjoint": ad_nu, "central_fd_fp32": fd_nu,
+                               "rel_diff": abs(ad_nu - fd_nu) / abs(fd_nu)},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=500)
+    ap.add_argument("--steps", type=int, default=None, help="default 200 (tank), 5000 (batch), 4000 (adjoint)")
+    ap.add_argument("--warmup", type=int, default=None, help="default 500 (tank), 100 (batch), 0 (adjoint)")
     ap.add_argument("--size", type=int, default=512)
     ap.add_argument("--storage", default="f32", choices=["f32", "f16", "bf16"])
     ap.add_argument("--collision", default="cm", choices=["cm", "trt", "bgk"])
@@ -162,11 +287,22 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=100)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-sweep", action="store_true", help="skip the 16-bit storage and Np-accuracy sweep")
+    ap.add_argument("--workload", default="tank", choices=["tank", "batch", "adjoint"],
+                    help="tank: the 512^3 benchmark (default); batch: 64-tank sweep; adjoint: torque gradient")
+    ap.add_argument("--seed", type=int, default=20260512)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU path is not built",
               file=sys.stderr)
         sys.exit(2)
+    if args.steps is None:
+        args.steps = {"tank": 200, "batch": 5000, "adjoint": 4000}[args.workload]
+    if args.warmup is None:
+        args.warmup = {"tank": 500, "batch": 100, "adjoint": 0}[args.workload]
+    if args.workload != "tank":
+        fn = workload_batch if args.workload == "batch" else workload_adjoint
+        print(json.dumps(fn(args)), flush=True)
+        return
 
     cfg = tank_config(args.size)
     t_setup = time.perf_counter()
