@@ -392,14 +392,23 @@ __device__ __forceinline__ float collide_cm27_free_direct(float* fv, float os_ba
     X[ab * 3 + 1] = g_0;
     X[ab * 3 + 2] = g_p;
   }
-  // f' = feq + B(d)
+  // f' = feq + B(d).  Opposite directions share feq but for the sign of the
+  // linear term: feq_{q, opp q} = s -+ t, s = w (b + 4.5 r cu^2), t = 3 w r cu.
   const float usq = ux2 + uy2 + uz2;
   const float b = dr - r * 1.5f * usq;
-  sfor<0, 27>([&](auto qc) {
-    constexpr int q = decltype(qc)::value;
-    constexpr int g = gidx(lat_c(q, 0), lat_c(q, 1), lat_c(q, 2));
-    const float cu = cdot<q>(ux, uy, uz);
-    fv[q] = lat_wf(27, q) * (b + r * cu * (3.0f + 4.5f * cu)) + X[g];
+  const float r45 = 4.5f * r, r3 = 3.0f * r;
+  fv[0] = lat_wf(27, 0) * b + X[gidx(0, 0, 0)];
+  sfor<0, 13>([&](auto kc) {
+    constexpr int qa = 2 * decltype(kc)::value + 1;
+    constexpr int qb = qa + 1;
+    constexpr int ga = gidx(lat_c(qa, 0), lat_c(qa, 1), lat_c(qa, 2));
+    constexpr int gb = gidx(lat_c(qb, 0), lat_c(qb, 1), lat_c(qb, 2));
+    constexpr float w = lat_wf(27, qa);
+    const float cu = cdot<qa>(ux, uy, uz);
+    const float sym = w * (b + r45 * (cu * cu));
+    const float lin = (w * r3) * cu;
+    fv[qa] = (sym + lin) + X[ga];
+    fv[qb] = (sym - lin) + X[gb];
   });
   return r;
 }
