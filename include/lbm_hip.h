@@ -275,6 +275,27 @@ int lbm_adjoint_torque(lbm_fields* fl, int steps, int window, int snapshots, dou
  * count only. */
 int lbm_revolve_schedule(int steps, int snapshots, int* actions, int capacity, int* count);
 
+/* ---- checkpoint rank file (format v3) -------------------------------------
+ * The binary part of Solver::save / Solver::restore for one monolithic part
+ * (crates/lbm-core/src/solver.rs:4536-4630 writer, 1113-1215 reader; layout
+ * constants 29-45): header, section table, F_PRIMARY / STALE_STASH /
+ * MOMENTS / SOLID / FORCE_FIELD sections in the reference's padded host
+ * layout, FNV-1a 64 payload hash.  The manifest and the scenario /
+ * decomposition hashes are the caller's (lbmflow_amd/checkpoint.py).
+ * Errors carry the reference's CKPT_* codes in lbm_last_error. */
+typedef struct lbm_ckpt_info {
+  uint64_t bytes;         /* rank file size */
+  uint64_t payload_hash;  /* FNV-1a 64 of the payload */
+  uint64_t mask_hash;     /* part_mask_hash of the current masks (solver.rs:967-979) */
+  int64_t step;
+} lbm_ckpt_info;
+int lbm_checkpoint_write_part(lbm_fields* fl, const char* path, lbm_ckpt_info* info);
+/* Installs the state of `path` (masks, parameters, IBM body must already be
+ * those of the saved run) and sets the step counter to `step`. */
+int lbm_checkpoint_read_part(lbm_fields* fl, const char* path, int64_t step, lbm_ckpt_info* info);
+/* FNV-1a 64 (offset basis 0xcbf29ce484222325, prime 0x100000001b3). */
+uint64_t lbm_fnv1a64(const void* data, size_t n);
+
 /* ---- passive scalar ADE (solver.rs:2113-2205, scalar.rs:6-15) ------------ */
 int lbm_scalar_enable(lbm_fields* fl, double diffusivity, int lattice, int* index);
 /* set_scalar_concentration, solver.rs:2910-2965 (h = h_eq(C, u)). */

@@ -42,6 +42,11 @@ class IbmDiag(C.Structure):
                 ("momentum_error_rel", C.c_double)]
 
 
+class CkptInfo(C.Structure):
+    _fields_ = [("bytes", C.c_uint64), ("payload_hash", C.c_uint64), ("mask_hash", C.c_uint64),
+                ("step", C.c_int64)]
+
+
 _lib = None
 
 # name -> (restype, argtypes)
@@ -100,6 +105,9 @@ SIGNATURES = {
     "lbm_batch_autotune": (_I, [_P, _I, _pU32, _pD]),
     "lbm_adjoint_torque": (_I, [_P, _I, _I, _I, _pD, _pD, _pD, C.POINTER(C.c_int64)]),
     "lbm_revolve_schedule": (_I, [_I, _I, _pI, _I, _pI]),
+    "lbm_checkpoint_write_part": (_I, [_P, C.c_char_p, C.POINTER(CkptInfo)]),
+    "lbm_checkpoint_read_part": (_I, [_P, C.c_char_p, C.c_int64, C.POINTER(CkptInfo)]),
+    "lbm_fnv1a64": (C.c_uint64, [C.c_void_p, C.c_size_t]),
     "lbm_scalar_enable": (_I, [_P, _D, _I, _pI]),
     "lbm_scalar_set": (_I, [_P, _I, _pF]),
     "lbm_scalar_gather": (_I, [_P, _I, _pF]),

@@ -1695,3 +1695,344 @@ int lbm_adjoint_torque(lbm_fields* fl, int steps, int window, int snapshots, dou
 }
 
 }  // extern "C"
+
+// ---- checkpoint rank file, format v3 ------------------------------------------
+// The binary part of Solver::save / Solver::restore for one monolithic part
+// (solver.rs:4536-4630 writer, 1113-1215 reader, format constants 29-45):
+// "LBMKPT\0\0", u32 version 3, u8 endianness 0, u8 dtype (0 = f32), u16
+// lattice id (319 / 327), u16 Q, u16 D, u64 padded cells, u64 core cells,
+// u32 section count, (u32 id, u64 offset, u64 length) per section, the
+// payload, and the FNV-1a 64 hash of the payload.  Arrays use the
+// reference's host layout: q-major planes over the box padded by one halo
+// cell per side (fields.rs:1-16, subdomain.rs:41); moments and the force
+// field are compact.  Halo cells of periodic axes carry the copies the mask
+// exchange would make (halo.rs:156-196, 456-492), other halo cells zero.
+// The stream stash section holds the populations again: on the closed-wall
+// path no slot it keeps is ever read.  The manifest (JSON) and the scenario
+// and decomposition hashes belong to the caller.
+
+namespace {
+
+constexpr uint32_t kCkptVersion = 3;
+constexpr uint32_t kSecF = 1, kSecStash = 2, kSecMoments = 3, kSecSolid = 4, kSecForce = 5;
+
+uint64_t fnv1a(const void* data, size_t n, uint64_t h = 0xcbf29ce484222325ull) {
+  const uint8_t* p = (const uint8_t*)data;
+  for (size_t k = 0; k < n; ++k) {
+    h ^= p[k];
+    h *= 0x100000001b3ull;
+  }
+  return h;
+}
+
+struct PadGeom {
+  int64_t n[3], p[3];
+  size_t np, nc;
+  explicit PadGeom(const Dom& d) {
+    n[0] = d.nx;
+    n[1] = d.ny;
+    n[2] = d.nz;
+    for (int a = 0; a < 3; ++a) p[a] = n[a] + 2;
+    np = (size_t)(p[0] * p[1] * p[2]);
+    nc = (size_t)(n[0] * n[1] * n[2]);
+  }
+  size_t pidx(int64_t x, int64_t y, int64_t z) const { return (size_t)(((z + 1) * p[1] + (y + 1)) * p[0] + (x + 1)); }
+  size_t cidx(int64_t x, int64_t y, int64_t z) const { return (size_t)((z * n[1] + y) * n[0] + x); }
+};
+
+// one padded plane from a compact plane, halo filled as a single periodic
+// part's mask exchange does: axes in order x, y, z, low face then high face;
+// tangents of earlier axes span the halo (forwarding corners)
+template <typename T>
+void pad_plane(const PadGeom& g, const int per[3], const T* core, T* out) {
+  std::fill(out, out + g.np, T{});
+  for (int64_t z = 0; z < g.n[2]; ++z)
+    for (int64_t y = 0; y < g.n[1]; ++y)
+      for (int64_t x = 0; x < g.n[0]; ++x) out[g.pidx(x, y, z)] = core[g.cidx(x, y, z)];
+  for (int a = 0; a < 3; ++a) {
+    if (!per[a]) continue;
+    const int t1 = a == 0 ? 1 : 0, t2 = a == 2 ? 1 : 2;
+    auto range = [&](int t, int64_t& lo, int64_t& hi) {
+      lo = t < a ? -1 : 0;
+      hi = t < a ? g.n[t] + 1 : g.n[t];
+    };
+    int64_t l1, h1, l2, h2;
+    range(t1, l1, h1);
+    range(t2, l2, h2);
+    for (int side = 0; side < 2; ++side) {
+      const int64_t dst = side == 0 ? -1 : g.n[a];
+      const int64_t src = side == 0 ? g.n[a] - 1 : 0;
+      for (int64_t c2 = l2; c2 < h2; ++c2)
+        for (int64_t c1 = l1; c1 < h1; ++c1) {
+          int64_t ps[3], pd[3];
+          ps[a] = src;
+          pd[a] = dst;
+          ps[t1] = pd[t1] = c1;
+          ps[t2] = pd[t2] = c2;
+          out[g.pidx(pd[0], pd[1], pd[2])] = out[g.pidx(ps[0], ps[1], ps[2])];
+        }
+    }
+  }
+}
+
+// part_mask_hash (solver.rs:967-979): padded solid flags, then padded wall
+// velocities as f32 bit patterns
+uint64_t mask_hash(const PadGeom& g, const int per[3], const std::vector<uint8_t>& solid,
+                   const std::vector<float>& wall) {
+  std::vector<uint8_t> ps(g.np);
+  pad_plane(g, per, solid.data(), ps.data());
+  uint64_t h = 0xcbf29ce484222325ull;
+  const uint64_t np = g.np;
+  h = fnv1a(&np, 8, h);
+  for (size_t k = 0; k < g.np; ++k) {
+    const uint64_t v = ps[k] ? 1 : 0;
+    h = fnv1a(&v, 8, h);
+  }
+  std::vector<float> comp(g.nc), pw(g.np);
+  std::vector<float> padded(3 * g.np, 0.0f);
+  if (!wall.empty())
+    for (int c = 0; c < 3; ++c) {
+      for (size_t k = 0; k < g.nc; ++k) comp[k] = wall[3 * k + c];
+      pad_plane(g, per, comp.data(), pw.data());
+      for (size_t k = 0; k < g.np; ++k) padded[3 * k + c] = pw[k];
+    }
+  return fnv1a(padded.data(), padded.size() * 4, h);
+}
+
+struct CkptWriter {
+  FILE* fp = nullptr;
+  uint64_t h = 0xcbf29ce484222325ull;
+  bool ok = true;
+  void raw(const void* p, size_t n) {
+    if (ok && fwrite(p, 1, n, fp) != n) ok = false;
+  }
+  void payload(const void* p, size_t n) {
+    h = fnv1a(p, n, h);
+    raw(p, n);
+  }
+};
+
+int ckpt_host_state(lbm_fields* fl, std::vector<float>& f, std::vector<uint8_t>& solid, std::vector<float>& wall,
+                    std::vector<float>& mom, std::vector<float>& force, int* present) {
+  const size_t N = fl->d.N;
+  f.resize((size_t)fl->Q * N);
+  solid.resize(N);
+  mom.resize(4 * N);
+  force.resize(3 * N);
+  int rc = lbm_stage_out(fl, f.data());
+  if (!rc) rc = lbm_read_solid(fl, solid.data());
+  if (!rc) rc = lbm_read_moments(fl, mom.data(), mom.data() + N, mom.data() + 2 * N, mom.data() + 3 * N);
+  if (!rc) rc = lbm_read_force_field(fl, force.data(), present);
+  if (rc) return rc;
+  wall.clear();
+  if (fl->wall_u) {
+    wall.resize(3 * N);
+    HIPCHK(fl->ctx, hipMemcpyAsync(wall.data(), fl->wall_u, 3 * N * 4, hipMemcpyDeviceToHost, fl->ctx->stream));
+    HIPCHK(fl->ctx, hipStreamSynchronize(fl->ctx->stream));
+  }
+  return LBM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+uint64_t lbm_fnv1a64(const void* data, size_t n) { return fnv1a(data, n); }
+
+int lbm_checkpoint_write_part(lbm_fields* fl, const char* path, lbm_ckpt_info* info) {
+  if (!fl || !path || !info) return LBM_ERR_INVALID;
+  if (!fl->scalars.empty())
+    return fail(fl->ctx, LBM_ERR_UNSUPPORTED, "CKPT_UNSUPPORTED: scalar distributions are not checkpointed");
+  const PadGeom g(fl->d);
+  const int per[3] = {(int)fl->d.per[0], (int)fl->d.per[1], (int)fl->d.per[2]};
+  std::vector<float> f, wall, mom, force;
+  std::vector<uint8_t> solid;
+  int present = 0;
+  int rc = ckpt_host_state(fl, f, solid, wall, mom, force, &present);
+  if (rc) return rc;
+  const size_t Q = (size_t)fl->Q;
+  std::vector<std::pair<uint32_t, uint64_t>> sec = {{kSecF, Q * g.np * 4}, {kSecStash, Q * g.np * 4},
+                                                    {kSecMoments, 4 * g.nc * 4}, {kSecSolid, g.np}};
+  if (present) sec.push_back({kSecForce, 3 * g.nc * 4});
+  CkptWriter w;
+  w.fp = fopen(path, "wb");
+  if (!w.fp) return fail(fl->ctx, LBM_ERR_INVALID, "CKPT_IO: cannot create %s", path);
+  const uint32_t ver = kCkptVersion, nsec = (uint32_t)sec.size();
+  const uint8_t endian = 0, dtype = 0;
+  const uint16_t lat = fl->Q == 19 ? 319 : 327, q16 = (uint16_t)fl->Q, d16 = 3;
+  const uint64_t np = g.np, nc = g.nc;
+  w.raw("LBMKPT\0\0", 8);
+  w.raw(&ver, 4);
+  w.raw(&endian, 1);
+  w.raw(&dtype, 1);
+  w.raw(&lat, 2);
+  w.raw(&q16, 2);
+  w.raw(&d16, 2);
+  w.raw(&np, 8);
+  w.raw(&nc, 8);
+  w.raw(&nsec, 4);
+  uint64_t off = 0;
+  for (auto& e : sec) {
+    w.raw(&e.first, 4);
+    w.raw(&off, 8);
+    w.raw(&e.second, 8);
+    off += e.second;
+  }
+  std::vector<float> plane(g.np);
+  for (int pass = 0; pass < 2; ++pass)  // populations, then the stash copy
+    for (size_t q = 0; q < Q; ++q) {
+      pad_plane(g, per, f.data() + q * g.nc, plane.data());
+      w.payload(plane.data(), g.np * 4);
+    }
+  w.payload(mom.data(), 4 * g.nc * 4);
+  std::vector<uint8_t> ps(g.np);
+  pad_plane(g, per, solid.data(), ps.data());
+  w.payload(ps.data(), g.np);
+  if (present) w.payload(force.data(), 3 * g.nc * 4);
+  const uint64_t ph = w.h;
+  w.raw(&ph, 8);
+  const long bytes = ftell(w.fp);
+  if (fclose(w.fp) != 0) w.ok = false;
+  if (!w.ok) return fail(fl->ctx, LBM_ERR_INVALID, "CKPT_IO: short write to %s", path);
+  info->bytes = (uint64_t)bytes;
+  info->payload_hash = ph;
+  info->mask_hash = mask_hash(g, per, solid, wall);
+  info->step = fl->time;
+  return LBM_OK;
+}
+
+int lbm_checkpoint_read_part(lbm_fields* fl, const char* path, int64_t step, lbm_ckpt_info* info) {
+  if (!fl || !path || !info || step < 0) return LBM_ERR_INVALID;
+  if (!fl->scalars.empty())
+    return fail(fl->ctx, LBM_ERR_UNSUPPORTED, "CKPT_UNSUPPORTED: scalar distributions are not checkpointed");
+  FILE* fp = fopen(path, "rb");
+  if (!fp) return fail(fl->ctx, LBM_ERR_INVALID, "CKPT_IO: cannot open %s", path);
+  std::vector<uint8_t> b;
+  {
+    uint8_t buf[1 << 16];
+    size_t k;
+    while ((k = fread(buf, 1, sizeof buf, fp)) > 0) b.insert(b.end(), buf, buf + k);
+    fclose(fp);
+  }
+  size_t pos = 0;
+  auto take = [&](void* dst, size_t n) -> bool {
+    if (b.size() - pos < n) return false;
+    memcpy(dst, b.data() + pos, n);
+    pos += n;
+    return true;
+  };
+  char magic[8];
+  uint32_t ver = 0, nsec = 0;
+  uint8_t endian = 0, dtype = 0;
+  uint16_t lat = 0, q16 = 0, d16 = 0;
+  uint64_t np = 0, nc = 0;
+  if (!take(magic, 8) || memcmp(magic, "LBMKPT\0\0", 8) != 0)
+    return fail(fl->ctx, LBM_ERR_INVALID, "CKPT_BAD_MAGIC: rank file magic is not LBMKPT");
+  if (!take(&ver, 4) || ver != kCkptVersion)
+    return fail(fl->ctx, LBM_ERR_INVALID, "CKPT_VERSION_MISMATCH: format_version %u, supported %u", ver,
+                kCkptVersion);
+  if (!take(&endian, 1) || endian != 0)
+    return fail(fl->ctx, LBM_ERR_INVALID, "CKPT_BAD_MAGIC: cross-endian checkpoints are not supported");
+  if (!take(&dtype, 1) || !take(&lat, 2) || !take(&q16, 2) || !take(&d16, 2) || !take(&np, 8) || !take(&nc, 8) ||
+      !take(&nsec, 4))
+    return fail(fl->ctx, LBM_ERR_INVALID, "CKPT_TRUNCATED: rank file ended inside the header");
+  if (dtype != 0) return fail(fl->ctx, LBM_ERR_INVALID, "CKPT_DTYPE_MISMATCH: checkpoint dtype is not f32");
+  const PadGeom g(fl->d);
+  if (q16 != fl->Q || d16 != 3 || lat != (fl->Q == 19 ? 319 : 327))
+    return fail(fl->ctx, LBM_ERR_INVALID, "CKPT_LATTICE_MISMATCH: checkpoint lattice %u (Q %u)", lat, q16);
+  if (np != g.np || nc != g.nc)
+    return fail(fl->ctx, LBM_ERR_INVALID, "CKPT_GEOM_MISMATCH: %llu/%llu cells, run has %zu/%zu",
+                (unsigned long long)np, (unsigned long long)nc, g.np, g.nc);
+  struct Ent {
+    uint32_t id;
+    uint64_t off, len;
+  };
+  std::vector<Ent> tab(nsec);
+  for (auto& e : tab)
+    if (!take(&e.id, 4) || !take(&e.off, 8) || !take(&e.len, 8))
+      return fail(fl->ctx, LBM_ERR_INVALID, "CKPT_TRUNCATED: rank file ended inside the section table");
+  if (b.size() - pos < 8) return fail(fl->ctx, LBM_ERR_INVALID, "CKPT_TRUNCATED: missing payload hash trailer");
+  const size_t pend = b.size() - 8;
+  uint64_t ph;
+  memcpy(&ph, b.data() + pend, 8);
+  const uint8_t* payload = b.data() + pos;
+  const size_t plen = pend - pos;
+  if (fnv1a(payload, plen) != ph)
+    return fail(fl->ctx, LBM_ERR_INVALID, "CKPT_PAYLOAD_CORRUPT: payload hash does not match its contents");
+  auto section = [&](uint32_t id, uint64_t want, const uint8_t** out) -> int {
+    for (auto& e : tab)
+      if (e.id == id) {
+        if (e.off + e.len > plen)
+          return fail(fl->ctx, LBM_ERR_INVALID, "CKPT_TRUNCATED: section %u exceeds the payload", id);
+        if (e.len != want)
+          return fail(fl->ctx, LBM_ERR_INVALID, "CKPT_GEOM_MISMATCH: section %u has %llu bytes, expected %llu",
+                      id, (unsigned long long)e.len, (unsigned long long)want);
+        *out = payload + e.off;
+        return LBM_OK;
+      }
+    *out = nullptr;
+    return LBM_OK;
+  };
+  const size_t Q = (size_t)fl->Q;
+  const uint8_t *sf = nullptr, *sm = nullptr, *ss = nullptr, *sff = nullptr;
+  int rc = section(kSecF, Q * g.np * 4, &sf);
+  if (!rc) rc = section(kSecMoments, 4 * g.nc * 4, &sm);
+  if (!rc) rc = section(kSecSolid, g.np, &ss);
+  if (!rc) rc = section(kSecForce, 3 * g.nc * 4, &sff);
+  if (rc) return rc;
+  if (!sf) return fail(fl->ctx, LBM_ERR_INVALID, "CKPT_TRUNCATED: checkpoint is missing required section F_PRIMARY");
+  if (!sm) return fail(fl->ctx, LBM_ERR_INVALID, "CKPT_TRUNCATED: checkpoint is missing required section MOMENTS");
+  const int per[3] = {(int)fl->d.per[0], (int)fl->d.per[1], (int)fl->d.per[2]};
+  std::vector<uint8_t> solid(g.nc);
+  rc = lbm_read_solid(fl, solid.data());
+  if (rc) return rc;
+  if (ss) {
+    std::vector<uint8_t> mine(g.np);
+    pad_plane(g, per, solid.data(), mine.data());
+    for (size_t k = 0; k < g.np; ++k)
+      if ((ss[k] != 0) != (mine[k] != 0))
+        return fail(fl->ctx, LBM_ERR_INVALID, "CKPT_GEOM_MISMATCH: serialized solid mask differs from rebuilt mask");
+  }
+  // install: populations (core cells), force field, cached-moment force
+  std::vector<float> f(Q * g.nc);
+  for (size_t q = 0; q < Q; ++q)
+    for (int64_t z = 0; z < g.n[2]; ++z)
+      for (int64_t y = 0; y < g.n[1]; ++y)
+        for (int64_t x = 0; x < g.n[0]; ++x)
+          memcpy(&f[q * g.nc + g.cidx(x, y, z)], sf + 4 * (q * g.np + g.pidx(x, y, z)), 4);
+  rc = lbm_stage_in(fl, f.data());
+  if (rc) return rc;
+  if (sff) {
+    std::vector<float> ff(3 * g.nc);
+    memcpy(ff.data(), sff, ff.size() * 4);
+    rc = lbm_set_force_field(fl, ff.data());
+  } else {
+    rc = lbm_clear_force_field(fl);
+  }
+  if (!rc) rc = refresh_moments(fl);
+  if (rc) return rc;
+  fl->time = step;
+  // the cached moments recomputed from the installed state must be the saved ones
+  std::vector<float> mom(4 * g.nc);
+  rc = lbm_read_moments(fl, mom.data(), mom.data() + g.nc, mom.data() + 2 * g.nc, mom.data() + 3 * g.nc);
+  if (rc) return rc;
+  for (size_t k = 0; k < g.nc; ++k) {
+    if (solid[k]) continue;
+    for (int c = 0; c < 4; ++c)
+      if (memcmp(&mom[c * g.nc + k], sm + 4 * (c * g.nc + k), 4) != 0)
+        return fail(fl->ctx, LBM_ERR_STATE, "CKPT_MOMENTS_MISMATCH: cell %zu moment %d differs after restore", k, c);
+  }
+  std::vector<float> wall;
+  if (fl->wall_u) {
+    wall.resize(3 * g.nc);
+    HIPCHK(fl->ctx, hipMemcpyAsync(wall.data(), fl->wall_u, 3 * g.nc * 4, hipMemcpyDeviceToHost, fl->ctx->stream));
+    HIPCHK(fl->ctx, hipStreamSynchronize(fl->ctx->stream));
+  }
+  info->bytes = b.size();
+  info->payload_hash = ph;
+  info->mask_hash = mask_hash(g, per, solid, wall);
+  info->step = step;
+  return LBM_OK;
+}
+
+}  // extern "C"

@@ -129,6 +129,9 @@ class Solver:
         self.storage = storage
         self.collision = collision
         self.nu = float(nu)
+        self.magic = float(magic)
+        self.periodic = tuple(bool(p) for p in periodic)
+        self.force = tuple(float(v) for v in force)
         d = np.array(self.dims, np.int32)
         per = np.array([int(bool(p)) for p in periodic], np.int32)
         h = C.c_void_p()
@@ -337,6 +340,17 @@ class Solver:
             out["dJ_domega"] = dw.value
             out["dJ_dnu"] = dnu
         return out
+
+    # ---- checkpoints (format v3, solver.rs:4536-4830) ------------------------
+    def save_checkpoint(self, directory: str) -> dict:
+        from . import checkpoint
+
+        return checkpoint.save(self, directory)
+
+    def restore_checkpoint(self, directory: str) -> int:
+        from . import checkpoint
+
+        return checkpoint.restore(self, directory)
 
     # ---- scalar distributions ----------------------------------------------
     def enable_scalar_distribution(self, diffusivity: float, lattice: str = "hydro") -> int:

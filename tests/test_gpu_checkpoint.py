@@ -1,0 +1,159 @@
+"""Checkpoint v3 save / restore (solver.rs:4536-4830): bit-exact resume,
+the rank file read back by an independent parser, and the reference's
+error codes for corrupt or mismatched checkpoints."""
+import os
+import struct
+
+import numpy as np
+import pytest
+
+import cases
+from lbmflow_amd import checkpoint as K
+
+pytestmark = pytest.mark.gpu
+
+
+def _parse_rank(path):
+    b = open(path, "rb").read()
+    assert b[:8] == b"LBMKPT\0\0"
+    ver, endian, dtype, lat, q, d, np_, nc, nsec = struct.unpack_from("<IBBHHHQQI", b, 8)
+    pos = 8 + struct.calcsize("<IBBHHHQQI")
+    tab = []
+    for _ in range(nsec):
+        tab.append(struct.unpack_from("<IQQ", b, pos))
+        pos += struct.calcsize("<IQQ")
+    payload, trailer = b[pos:-8], struct.unpack("<Q", b[-8:])[0]
+    sec = {i: payload[o:o + n] for i, o, n in tab}
+    return dict(ver=ver, endian=endian, dtype=dtype, lat=lat, q=q, d=d, np=np_, nc=nc, sec=sec, payload=payload,
+                hash=trailer)
+
+
+def _pad(a, dims, periodic):
+    """Halo of one cell per side: wrap on periodic axes, zeros elsewhere,
+    axis by axis (x, then y, then z) as the mask exchange fills it."""
+    nx, ny, nz = dims
+    a = a.reshape(nz, ny, nx)
+    for axis, arr_axis in ((0, 2), (1, 1), (2, 0)):
+        widths = [(0, 0)] * 3
+        widths[arr_axis] = (1, 1)
+        a = np.pad(a, widths, mode="wrap" if periodic[axis] else "constant")
+    return a.reshape(-1)
+
+
+def _cavity():
+    import lbmflow_amd as L
+
+    n = 16
+    solid, wall = cases.cavity_masks(n)
+    return L.Solver(19, (n, n, n), nu=0.03, collision="trt", solid=solid, wall_u=wall.astype(np.float32))
+
+
+def test_cavity_resume_bit_exact_and_file_layout(tmp_path):
+    a = _cavity()
+    a.run(7)
+    man = a.save_checkpoint(str(tmp_path))
+    f7 = a.populations()
+    rho, ux, uy, uz = a.moments()
+    a.run(5)
+    b = _cavity()
+    assert b.restore_checkpoint(str(tmp_path)) == 7
+    assert np.array_equal(b.populations(), f7)
+    b.run(5)
+    assert np.array_equal(a.populations(), b.populations())
+    # independent read of the rank file
+    r = _parse_rank(os.path.join(str(tmp_path), man["ranks"][0]["file"]))
+    assert (r["ver"], r["endian"], r["dtype"], r["lat"], r["q"], r["d"]) == (3, 0, 0, 319, 19, 3)
+    n = 16
+    assert r["np"] == 18 ** 3 and r["nc"] == n ** 3
+    assert K.fnv1a64(r["payload"]) == r["hash"]
+    assert man["ranks"][0]["payload_hash"] == f"fnv1a64:{r['hash']:016x}"
+    fp = np.frombuffer(r["sec"][1], np.float32).reshape(19, -1)
+    for q in range(19):
+        assert np.array_equal(fp[q], _pad(f7[q], (n, n, n), (0, 0, 0)))
+    assert np.array_equal(np.frombuffer(r["sec"][4], np.uint8), _pad(a.solid(), (n, n, n), (0, 0, 0)))
+    mom = np.frombuffer(r["sec"][3], np.float32).reshape(4, -1)
+    fl = a.solid() == 0
+    for k, v in enumerate((rho, ux, uy, uz)):
+        assert np.array_equal(mom[k][fl], v.astype(np.float32)[fl])
+    assert 5 not in r["sec"]  # no force field in this run
+
+
+def test_periodic_channel_halo_copies(tmp_path):
+    import lbmflow_amd as L
+
+    dims = (12, 10, 8)
+    x, y, z = cases.coords(dims)
+    solid = (((x - 5) ** 2 + (y - 4) ** 2) < 6).astype(np.uint8)
+    solid[(y == 0) | (y == dims[1] - 1)] = 1
+    per = (1, 0, 1)
+    mk = lambda: L.Solver(19, dims, nu=0.05, collision="trt", periodic=per, force=(1e-5, 0, 0), solid=solid)
+    a = mk()
+    a.run(9)
+    man = a.save_checkpoint(str(tmp_path))
+    r = _parse_rank(os.path.join(str(tmp_path), man["ranks"][0]["file"]))
+    assert np.array_equal(np.frombuffer(r["sec"][4], np.uint8), _pad(solid, dims, per))
+    fp = np.frombuffer(r["sec"][1], np.float32).reshape(19, -1)
+    f9 = a.populations()
+    assert np.array_equal(fp[7], _pad(f9[7], dims, per))
+    a.run(4)
+    b = mk()
+    b.restore_checkpoint(str(tmp_path))
+    b.run(4)
+    assert np.array_equal(a.populations(), b.populations())
+
+
+def test_tank_with_ibm_resume_bit_exact(tmp_path):
+    import lbmflow_amd as L
+
+    tc = cases.tank_case(24, nu=0.01, collision="cm", Q=27)
+
+    def mk():
+        s = L.Solver(27, (24,) * 3, nu=tc["nu"], collision="cm", solid=tc["solid"])
+        s.set_rotating_body(tc["pos"], tc["w"], tc["c"], (0.0, 0.0, tc["omega"]))
+        return s
+
+    a = mk()
+    a.run_with_ibm(5, graph=False)
+    a.save_checkpoint(str(tmp_path))
+    ta, _ = a.run_with_ibm(6, graph=False)
+    b = mk()
+    assert b.restore_checkpoint(str(tmp_path)) == 5
+    tb, _ = b.run_with_ibm(6, graph=False)
+    assert np.array_equal(ta, tb)
+    assert np.array_equal(a.populations(), b.populations())
+
+
+def test_corrupt_and_mismatched_checkpoints_are_refused(tmp_path):
+    import lbmflow_amd as L
+
+    a = _cavity()
+    a.run(3)
+    man = a.save_checkpoint(str(tmp_path))
+    path = os.path.join(str(tmp_path), man["ranks"][0]["file"])
+    good = open(path, "rb").read()
+    # other viscosity: scenario hash
+    n = 16
+    solid, wall = cases.cavity_masks(n)
+    other = L.Solver(19, (n, n, n), nu=0.031, collision="trt", solid=solid, wall_u=wall.astype(np.float32))
+    with pytest.raises(K.CheckpointError) as e:
+        other.restore_checkpoint(str(tmp_path))
+    assert e.value.code == "CKPT_SCENARIO_MISMATCH"
+    # other solid mask: serialized mask check
+    s2 = solid.copy()
+    s2[(n // 2) * n * n + (n // 2) * n + n // 2] = 1
+    other = L.Solver(19, (n, n, n), nu=0.03, collision="trt", solid=s2, wall_u=wall.astype(np.float32))
+    with pytest.raises(K.CheckpointError) as e:
+        other.restore_checkpoint(str(tmp_path))
+    assert e.value.code == "CKPT_GEOM_MISMATCH"
+    # one flipped payload byte
+    bad = bytearray(good)
+    bad[len(bad) // 2] ^= 0x40
+    open(path, "wb").write(bytes(bad))
+    with pytest.raises(K.CheckpointError) as e:
+        _cavity().restore_checkpoint(str(tmp_path))
+    assert e.value.code == "CKPT_PAYLOAD_CORRUPT"
+    # truncated file
+    open(path, "wb").write(good[:100])
+    with pytest.raises(K.CheckpointError) as e:
+        _cavity().restore_checkpoint(str(tmp_path))
+    assert e.value.code == "CKPT_TRUNCATED"
