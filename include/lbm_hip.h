@@ -74,6 +74,7 @@ extern "C" {
 #define LBM_CAP_TWO_PASS 0x400u      /* not set: one fused pass per step */
 #define LBM_CAP_GRAVITY 0x800u       /* not set */
 #define LBM_CAP_PROBE 0x1000u        /* not set: momentum-exchange probe */
+#define LBM_CAP_WALE 0x2000u         /* WALE omega_plus producer (les.rs) */
 
 typedef struct lbm_ctx lbm_ctx;
 typedef struct lbm_fields lbm_fields;
@@ -148,6 +149,37 @@ int lbm_boundary_links(lbm_fields* fl, int64_t* count, int64_t* cells, uint8_t* 
 int lbm_set_params(lbm_fields* fl, const lbm_step_params* p);
 /* Per-cell omega_plus (docs/KERNEL_EXTENSION_POINTS.md:22-73): NULL clears. */
 int lbm_set_omega_field(lbm_fields* fl, const float* omega);
+
+/* WALE large-eddy closure as an omega_plus producer.
+ * WaleLesDiagnostics, les.rs:18-31 (tau_eff_max / nu_t_max are 0 when the
+ * update does not clip). */
+typedef struct lbm_wale_diag {
+  int64_t clipped_cells;
+  double clipped_fraction;
+  double max_nu_t_before_clipping;
+  double tau_eff_max;
+  double nu_t_max;
+} lbm_wale_diag;
+/* WaleLes::update, les.rs:120-211 (replaces WaleLes::new().update(&mut
+ * solver), or with_tau_eff_max(t).update): computes the velocity gradient of
+ * the current state (Solver::gather_velocity_gradient, solver.rs:5490-5581),
+ * nu_t with Delta = 1 and installs omega = 1/(3 (nu0 + nu_t) + 1/2) as the
+ * per-cell omega_plus field of the next steps.  cw: model coefficient (the
+ * reference fixes 0.325); tau_eff_max <= 0: no clip, otherwise it must exceed
+ * 0.5 and the laminar tau (LBM_ERR_INVALID).  nu_t (host, N floats, after
+ * the clip) and diag may be NULL. */
+int lbm_wale_update(lbm_fields* fl, double cw, double tau_eff_max, float* nu_t, lbm_wale_diag* diag);
+/* On-device WALE every step (GpuSolver::set_wale, gpu/solver.rs:60-73): with
+ * enabled != 0 each lbm_run_span / lbm_run_ibm_span step first recomputes the
+ * omega_plus field from the pre-collision state (after the IBM spread), as
+ * lbm_wale_update does, then collides with it.  Disabling leaves the last
+ * field installed; lbm_set_omega_field(fl, NULL) removes it
+ * (WaleLes::clear, les.rs:213-220). */
+int lbm_set_wale(lbm_fields* fl, int enabled, double cw, double tau_eff_max);
+/* Installed omega_plus field to the host (GpuSolver::gather_wale_omega,
+ * gpu/solver.rs:228-235): N floats; *present = 0 (and omega untouched)
+ * when no field is installed. */
+int lbm_gather_omega_field(lbm_fields* fl, float* omega, int* present);
 
 /* Backend::stage_in / stage_out, backend.rs:141-155 (compact q-major).
  * stage_in makes the cached moments those of the staged populations with

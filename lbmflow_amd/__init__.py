@@ -6,6 +6,7 @@ reference's Solver/Backend interface over that ABI.
 """
 from .solver import (  # noqa: F401
     MAGIC_STD,
+    WALE_CW,
     Batch,
     Context,
     DirectForcingConfig,
@@ -16,8 +17,10 @@ from .solver import (  # noqa: F401
     power_qois,
     radial_impeller,
     revolve_schedule,
+    WaleLes,
 )
 from ._lib import LbmError, lib  # noqa: F401
 
 __all__ = ["Solver", "Batch", "Context", "DirectForcingConfig", "omegas", "radial_impeller", "power_qois",
-           "mixing_time_from_cv", "MAGIC_STD", "LbmError", "lib", "default_context", "revolve_schedule"]
+           "mixing_time_from_cv", "MAGIC_STD", "LbmError", "lib", "default_context", "revolve_schedule",
+           "WaleLes", "WALE_CW"]

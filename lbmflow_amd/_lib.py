@@ -47,6 +47,11 @@ class CkptInfo(C.Structure):
                 ("step", C.c_int64)]
 
 
+class WaleDiag(C.Structure):
+    _fields_ = [("clipped_cells", C.c_int64), ("clipped_fraction", C.c_double),
+                ("max_nu_t_before_clipping", C.c_double), ("tau_eff_max", C.c_double), ("nu_t_max", C.c_double)]
+
+
 _lib = None
 
 # name -> (restype, argtypes)
@@ -75,6 +80,9 @@ SIGNATURES = {
     "lbm_boundary_links": (_I, [_P, _pI64, _pI64, _pU8, C.c_int64]),
     "lbm_set_params": (_I, [_P, C.POINTER(StepParams)]),
     "lbm_set_omega_field": (_I, [_P, _pF]),
+    "lbm_wale_update": (_I, [_P, _D, _D, _pF, C.POINTER(WaleDiag)]),
+    "lbm_set_wale": (_I, [_P, _I, _D, _D]),
+    "lbm_gather_omega_field": (_I, [_P, _pF, _pI]),
     "lbm_stage_in": (_I, [_P, _pF]),
     "lbm_stage_out": (_I, [_P, _pF]),
     "lbm_init_with": (_I, [_P, _pF, _pF]),

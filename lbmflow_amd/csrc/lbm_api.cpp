@@ -86,6 +86,12 @@ struct lbm_fields {
   HydroArgs g_args{};
   int g_parity = -1;
   uint64_t g_gen = 0;
+  // WALE LES: cached-moment scratch (4 x N), diagnostics words, and the
+  // per-step mode's coefficients
+  float* wale_mom = nullptr;
+  uint32_t* wale_stats = nullptr;
+  bool wale_on = false;
+  float wale_cw2 = 0.0f, wale_nut_max = -1.0f;
   size_t bytes = 0;
 };
 
@@ -397,7 +403,7 @@ int lbm_stream_copy(lbm_ctx* ctx, size_t bytes, int iterations, double* gbps) {
 int lbm_capabilities(const lbm_ctx* ctx, uint32_t* flags) {
   if (!ctx || !flags) return LBM_ERR_INVALID;
   *flags = LBM_CAP_TRT | LBM_CAP_CENTRAL_MOMENT | LBM_CAP_F16_STORAGE | LBM_CAP_BF16_STORAGE | LBM_CAP_FORCE_FIELD |
-           LBM_CAP_OMEGA_FIELD | LBM_CAP_ROTATING_IBM | LBM_CAP_SCALAR | LBM_CAP_PERIODIC;
+           LBM_CAP_OMEGA_FIELD | LBM_CAP_ROTATING_IBM | LBM_CAP_SCALAR | LBM_CAP_PERIODIC | LBM_CAP_WALE;
   return LBM_OK;
 }
 
@@ -487,7 +493,7 @@ void lbm_fields_free(lbm_fields* fl) {
   if (fl->graph) (void)hipGraphDestroy(fl->graph);
   for (void* p : {(void*)fl->f_alloc, (void*)fl->mask, (void*)fl->solid, (void*)fl->wall_u, (void*)fl->omega,
                   (void*)fl->F_cur, (void*)fl->F_mom, (void*)fl->flist, (void*)fl->partials, (void*)fl->tiles,
-                  (void*)fl->ring, (void*)fl->ring_pos})
+                  (void*)fl->ring, (void*)fl->ring_pos, (void*)fl->wale_mom, (void*)fl->wale_stats})
     if (p) (void)hipFree(p);
   delete fl;
 }
@@ -574,11 +580,14 @@ int lbm_set_omega_field(lbm_fields* fl, const float* omega) {
     HIPCHK(fl->ctx, hipStreamSynchronize(fl->ctx->stream));
     dev_free(fl, fl->omega, n * 4);
     fl->omega = nullptr;
+    fl->wale_on = false;  // the per-step WALE field goes with it
+    fl->gen++;
     return LBM_OK;
   }
   if (!fl->omega) {
     int rc = dev_alloc(fl, (void**)&fl->omega, n * 4);
     if (rc) return rc;
+    fl->gen++;
   }
   HIPCHK(fl->ctx, hipMemcpyAsync(fl->omega, omega, n * 4, hipMemcpyHostToDevice, fl->ctx->stream));
   HIPCHK(fl->ctx, hipStreamSynchronize(fl->ctx->stream));
@@ -689,6 +698,125 @@ int lbm_read_force_field(lbm_fields* fl, float* ff, int* present) {
   return LBM_OK;
 }
 
+// ---- WALE LES ------------------------------------------------------------
+// Laminar viscosity and the f32 quantities WaleLes::update derives from it
+// (les.rs:136-147): nu0, tau0 = 1/2 + 3 nu0, nu_t_max = (tau_eff_max -
+// tau0)/3.
+static double wale_nu(const lbm_fields* fl) { return (1.0 / fl->params.omega_p - 0.5) / 3.0; }
+
+static int wale_coeffs(lbm_fields* fl, double cw, double tau_eff_max, float* cw2, float* nut_max) {
+  if (!fl->params_set) return fail(fl->ctx, LBM_ERR_STATE, "set the step parameters before WALE");
+  if (!(cw > 0.0) || !std::isfinite(cw)) return fail(fl->ctx, LBM_ERR_INVALID, "WALE coefficient %g", cw);
+  const float c = (float)cw;
+  *cw2 = c * c;
+  *nut_max = -1.0f;
+  if (tau_eff_max > 0.0) {
+    // set_tau_eff_max, les.rs:82-95, and the laminar bound of update, les.rs:141-147
+    if (!std::isfinite(tau_eff_max) || !(tau_eff_max > 0.5))
+      return fail(fl->ctx, LBM_ERR_INVALID, "tau_eff_max must be finite and greater than 0.5");
+    const float nu0 = (float)wale_nu(fl);
+    const float tau0 = 0.5f + 3.0f * nu0;
+    const float t = (float)tau_eff_max;
+    if (!(t >= tau0))
+      return fail(fl->ctx, LBM_ERR_INVALID, "tau_eff_max %g is below the laminar tau %g", tau_eff_max, (double)tau0);
+    *nut_max = (t - tau0) / 3.0f;
+  }
+  return LBM_OK;
+}
+
+static int wale_buffers(lbm_fields* fl) {
+  const size_t n = fl->d.N;
+  if (!fl->omega) {
+    int rc = dev_alloc(fl, (void**)&fl->omega, n * 4);
+    if (rc) return rc;
+    fl->gen++;
+  }
+  if (!fl->wale_mom) {
+    int rc = dev_alloc(fl, (void**)&fl->wale_mom, 4 * n * 4);
+    if (rc) return rc;
+  }
+  if (!fl->wale_stats) {
+    int rc = dev_alloc(fl, (void**)&fl->wale_stats, 2 * sizeof(uint32_t));
+    if (rc) return rc;
+  }
+  return LBM_OK;
+}
+
+// Cached moments of the current state, then the omega_plus field; launches
+// only (graph-capturable: the statistics words are cleared by a kernel).
+static int wale_enqueue(lbm_fields* fl, float cw2, float nut_max, float* nut, uint32_t* stats) {
+  hipStream_t s = fl->ctx->stream;
+  float* m = fl->wale_mom;
+  const size_t n = fl->d.N;
+  HIPCHK(fl->ctx, launch_moments(fl->Q, fl->storage, fl->parity, fl->d, fl->f, fl->mask, fl->F_mom, fl->hp, m, m + n,
+                                 m + 2 * n, m + 3 * n, s));
+  HIPCHK(fl->ctx, launch_wale(fl->Q, fl->storage, fl->parity, fl->d, fl->f, fl->mask, fl->wall_u,
+                              fl->field_present ? fl->F_cur : nullptr, m, fl->hp, (float)(1.0 / fl->params.omega_p),
+                              cw2, (float)wale_nu(fl), nut_max, fl->omega, nut, stats, s));
+  return LBM_OK;
+}
+
+int lbm_wale_update(lbm_fields* fl, double cw, double tau_eff_max, float* nu_t, lbm_wale_diag* diag) {
+  if (!fl) return LBM_ERR_INVALID;
+  float cw2, nut_max;
+  int rc = wale_coeffs(fl, cw, tau_eff_max, &cw2, &nut_max);
+  if (!rc) rc = wale_buffers(fl);
+  if (rc) return rc;
+  const size_t n = fl->d.N;
+  float* dnut = nullptr;
+  if (nu_t) {
+    rc = dev_alloc(fl, (void**)&dnut, n * 4);
+    if (rc) return rc;
+  }
+  hipStream_t s = fl->ctx->stream;
+  rc = wale_enqueue(fl, cw2, nut_max, dnut, fl->wale_stats);
+  uint32_t st[2] = {0, 0};
+  if (!rc) {
+    HIPCHK(fl->ctx, hipMemcpyAsync(st, fl->wale_stats, sizeof st, hipMemcpyDeviceToHost, s));
+    if (nu_t) HIPCHK(fl->ctx, hipMemcpyAsync(nu_t, dnut, n * 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(fl->ctx, hipStreamSynchronize(s));
+  }
+  dev_free(fl, dnut, n * 4);
+  if (rc) return rc;
+  if (diag) {
+    diag->clipped_cells = st[0];
+    diag->clipped_fraction = n ? (double)((float)st[0] / (float)n) : 0.0;
+    float mx;
+    memcpy(&mx, &st[1], sizeof mx);
+    diag->max_nu_t_before_clipping = mx;
+    diag->tau_eff_max = nut_max >= 0.0f ? (double)(float)tau_eff_max : 0.0;
+    diag->nu_t_max = nut_max >= 0.0f ? (double)nut_max : 0.0;
+  }
+  return LBM_OK;
+}
+
+int lbm_gather_omega_field(lbm_fields* fl, float* omega, int* present) {
+  if (!fl || !omega || !present) return LBM_ERR_INVALID;
+  *present = fl->omega ? 1 : 0;
+  if (!fl->omega) return LBM_OK;
+  HIPCHK(fl->ctx, hipMemcpyAsync(omega, fl->omega, (size_t)fl->d.N * 4, hipMemcpyDeviceToHost, fl->ctx->stream));
+  HIPCHK(fl->ctx, hipStreamSynchronize(fl->ctx->stream));
+  return LBM_OK;
+}
+
+int lbm_set_wale(lbm_fields* fl, int enabled, double cw, double tau_eff_max) {
+  if (!fl) return LBM_ERR_INVALID;
+  if (!enabled) {
+    if (fl->wale_on) fl->gen++;
+    fl->wale_on = false;
+    return LBM_OK;
+  }
+  float cw2, nut_max;
+  int rc = wale_coeffs(fl, cw, tau_eff_max, &cw2, &nut_max);
+  if (!rc) rc = wale_buffers(fl);
+  if (rc) return rc;
+  fl->wale_on = true;
+  fl->wale_cw2 = cw2;
+  fl->wale_nut_max = nut_max;
+  fl->gen++;
+  return LBM_OK;
+}
+
 static int check_steppable(lbm_fields* fl) {
   if (fl->open_face_cells)
     return fail(fl->ctx, LBM_ERR_UNSUPPORTED,
@@ -702,6 +830,10 @@ static int check_steppable(lbm_fields* fl) {
 // scalar distributions; launches only.
 static int hydro_step(lbm_fields* fl) {
   hipStream_t s = fl->ctx->stream;
+  if (fl->wale_on) {
+    int rc = wale_enqueue(fl, fl->wale_cw2, fl->wale_nut_max, nullptr, nullptr);
+    if (rc) return rc;
+  }
   HydroArgs a = hydro_args(fl);
   if (fl->timing) {
     if (fl->ev_used + 2 > fl->ev_pool.size()) {
@@ -1587,6 +1719,8 @@ int lbm_adjoint_torque(lbm_fields* fl, int steps, int window, int snapshots, dou
   if (fl->Q != 19 || fl->collision == LBM_COLLISION_CM)
     return fail(fl->ctx, LBM_ERR_UNSUPPORTED, "the adjoint is implemented for D3Q19 TRT/BGK");
   if (fl->wall_u) return fail(fl->ctx, LBM_ERR_UNSUPPORTED, "the adjoint does not cover moving walls");
+  if (fl->wale_on)
+    return fail(fl->ctx, LBM_ERR_UNSUPPORTED, "the adjoint does not differentiate the per-step WALE closure");
   if (fl->hp.force_on) return fail(fl->ctx, LBM_ERR_UNSUPPORTED, "the adjoint does not cover a body force");
   int rc = check_steppable(fl);
   if (rc) return rc;

@@ -82,6 +82,13 @@ hipError_t launch_adj_collide_stream(int storage, int odd, const Dom& d, const v
                                      const HydroParams& p, const double* lam_in, double* lam_out, double* omega_bar,
                                      double* Fbar, double* Gout, hipStream_t s);
 hipError_t launch_fill_f64(double* p, size_t n, double v, hipStream_t s);
+// WALE omega_plus field from the state and its cached moments `mom` (4 x N:
+// rho, ux, uy, uz) (lbm_les.hip).  `stats` (2 words, cleared here by a
+// kernel): clipped cell count, bits of the largest raw nu_t.
+hipError_t launch_wale(int Q, int storage, int odd, const Dom& d, const void* f, const uint32_t* mask,
+                       const float* wall_u, const float* F_cur, const float* mom, const HydroParams& p,
+                       float tau_eff, float cw_delta_sq, float nu0, float nut_max, float* omega, float* nut,
+                       uint32_t* stats, hipStream_t s);
 hipError_t launch_build_mask(int Q, const Dom& d, const uint8_t* solid, const float* wall_u, uint32_t* mask,
                              unsigned int* err_count, hipStream_t s);
 hipError_t launch_mark_forced(const uint32_t* flist, uint32_t n, uint32_t* mask, hipStream_t s);

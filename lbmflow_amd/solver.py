@@ -20,6 +20,7 @@ import numpy as np
 from . import _lib as _L
 
 MAGIC_STD = 3.0 / 16.0  # CollisionKind::MAGIC_STD, params.rs:202
+WALE_CW = 0.325  # WALE_CW, les.rs:16
 
 
 def omegas(collision: str, nu: float, magic: float = MAGIC_STD):
@@ -226,6 +227,38 @@ class Solver:
         else:
             om = np.ascontiguousarray(omega, np.float32).reshape(-1)
             self._chk(self.lib.lbm_set_omega_field(self.h, _fp(om)))
+
+    def wale_update(self, cw: float = WALE_CW, tau_eff_max: float | None = None, want_nu_t: bool = False):
+        """One ``WaleLes::update`` (les.rs:120-211): install the WALE omega
+        field of the current state.  Returns the diagnostics dict
+        (``WaleLesDiagnostics``, les.rs:18-31) and, with ``want_nu_t``, the
+        eddy viscosity per cell (after the clip)."""
+        d = _L.WaleDiag()
+        nut = np.zeros(self.n, np.float32) if want_nu_t else None
+        self._chk(self.lib.lbm_wale_update(self.h, cw, 0.0 if tau_eff_max is None else tau_eff_max,
+                                           _fp(nut) if nut is not None else None, C.byref(d)))
+        diag = {
+            "clipped_cells": int(d.clipped_cells),
+            "clipped_fraction": d.clipped_fraction,
+            "max_nu_t_before_clipping": d.max_nu_t_before_clipping,
+            "tau_eff_max": None if tau_eff_max is None else d.tau_eff_max,
+            "nu_t_max": None if tau_eff_max is None else d.nu_t_max,
+        }
+        return (diag, nut) if want_nu_t else diag
+
+    def set_wale(self, enabled: bool, cw: float = WALE_CW, tau_eff_max: float | None = None):
+        """Recompute the WALE omega field on the device before every step
+        (``GpuSolver::set_wale``, gpu/solver.rs:60-73)."""
+        self._chk(self.lib.lbm_set_wale(self.h, int(bool(enabled)), cw,
+                                        0.0 if tau_eff_max is None else tau_eff_max))
+
+    def omega_field(self):
+        """The installed per-cell omega_plus field, or None
+        (``GpuSolver::gather_wale_omega``, gpu/solver.rs:228-235)."""
+        out = np.zeros(self.n, np.float32)
+        present = C.c_int()
+        self._chk(self.lib.lbm_gather_omega_field(self.h, _fp(out), C.byref(present)))
+        return out if present.value else None
 
     def set_body_force_field(self, ff):
         """``Solver::set_body_force_field`` (solver.rs:2776-2801)."""
@@ -471,6 +504,49 @@ class Batch:
 
     def __del__(self):
         self.close()
+
+
+class WaleLes:
+    """``WaleLes`` (les.rs:46-221): WALE SGS-viscosity driver with
+    ``Cw = 0.325`` and ``Delta = 1``.  ``update(solver)`` computes ``nu_t``
+    from the solver's current velocity gradient and installs
+    ``omega_plus = 1/(3 (nu0 + nu_t) + 1/2)`` for the next collisions;
+    ``clear(solver)`` removes the field."""
+
+    def __init__(self, tau_eff_max: float | None = None):
+        self.cw = WALE_CW
+        self.delta = 1.0
+        self._tau_eff_max = None
+        self.set_tau_eff_max(tau_eff_max)
+        self._nu_t = np.zeros(0, np.float32)
+        self._diag = {"clipped_cells": 0, "clipped_fraction": 0.0, "max_nu_t_before_clipping": 0.0,
+                      "tau_eff_max": None, "nu_t_max": None}
+
+    def with_tau_eff_max(self, tau_eff_max: float) -> "WaleLes":
+        self.set_tau_eff_max(tau_eff_max)
+        return self
+
+    def set_tau_eff_max(self, tau_eff_max: float | None):
+        """les.rs:82-95: the bound must be finite and above 0.5."""
+        if tau_eff_max is not None and not (np.isfinite(tau_eff_max) and tau_eff_max > 0.5):
+            raise ValueError("tau_eff_max must be finite and greater than 0.5")
+        self._tau_eff_max = tau_eff_max
+
+    def tau_eff_max(self):
+        return self._tau_eff_max
+
+    def nu_t(self):
+        """Last computed eddy viscosity, compact order."""
+        return self._nu_t
+
+    def diagnostics(self) -> dict:
+        return dict(self._diag)
+
+    def update(self, solver: "Solver"):
+        self._diag, self._nu_t = solver.wale_update(self.cw * self.delta, self._tau_eff_max, want_nu_t=True)
+
+    def clear(self, solver: "Solver"):
+        solver.set_omega_field(None)
 
 
 def revolve_schedule(steps: int, snapshots: int):

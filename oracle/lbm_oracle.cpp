@@ -673,6 +673,157 @@ struct Solver {
     }
   }
 
+  // strain_rate_at, solver.rs:5393-5447: S = -(Pi_neq + Pi_force) /
+  // (2 cs^2 rho tau) from the stored populations and cached moments, with
+  // Pi_neq = sum_q c c (f_q - feq_q(rho, u)), Pi_force = u F (diagonal) and
+  // (u_a F_b + u_b F_a)/2 (off-diagonal), F = force_at (the current field)
+  // and tau = 1/omega_p (the global rate, also under an omega field).
+  // Order [Sxx, Syy, Szz, Sxy, Sxz, Syz]; solid cells give zeros.
+  void strain_at(long x, long y, long z, T s[6]) const {
+    for (int k = 0; k < 6; ++k) s[k] = T(0);
+    const size_t pi = pidx(x, y, z);
+    if (solid[pi]) return;
+    const size_t c = cidx(x, y, z);
+    const T r = rho[c];
+    const T u[3] = {ux[c], uy[c], uz[c]};
+    T feq[27];
+    equilibrium(r, u, feq);
+    for (int q = 0; q < L.Q; ++q) {
+      const T fn = f[q * np + pi] - feq[q];
+      const T cx = cr[q][0], cy = cr[q][1], cz = cr[q][2];
+      s[0] = s[0] + cx * cx * fn;
+      s[1] = s[1] + cy * cy * fn;
+      s[2] = s[2] + cz * cz * fn;
+      s[3] = s[3] + cx * cy * fn;
+      s[4] = s[4] + cx * cz * fn;
+      s[5] = s[5] + cy * cz * fn;
+    }
+    T fv[3];
+    force_at(c, fv);
+    const T half = T(0.5);
+    s[0] = s[0] + u[0] * fv[0];
+    s[1] = s[1] + u[1] * fv[1];
+    s[2] = s[2] + u[2] * fv[2];
+    s[3] = s[3] + half * (u[0] * fv[1] + u[1] * fv[0]);
+    s[4] = s[4] + half * (u[0] * fv[2] + u[2] * fv[0]);
+    s[5] = s[5] + half * (u[1] * fv[2] + u[2] * fv[1]);
+    const T tau_eff = T(1.0 / omega_p);
+    const T scale = -(T(1) / (T(2.0 / 3.0) * r * tau_eff));
+    for (int k = 0; k < 6; ++k) s[k] = s[k] * scale;
+  }
+
+  // gather_velocity_gradient, solver.rs:5490-5581: g[i][j] = du_i/dx_j.
+  // Diagonal: finite differences of the cached velocity; off-diagonal: the
+  // stress-based strain plus the antisymmetric part of the finite
+  // differences.  Along each axis: central difference between two fluid
+  // neighbours; next to a solid neighbour the one-sided second-order form
+  // through the wall velocity stored at that solid (a half-way wall); a
+  // first-order difference at a non-periodic domain face; zero otherwise.
+  void vgrad_at(long x, long y, long z, T g[3][3]) const {
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) g[a][b] = T(0);
+    if (solid[pidx(x, y, z)]) return;
+    T sv[6];
+    strain_at(x, y, z, sv);
+    const T sm[3][3] = {{sv[0], sv[3], sv[4]}, {sv[3], sv[1], sv[5]}, {sv[4], sv[5], sv[2]}};
+    const size_t c = cidx(x, y, z);
+    const T own[3] = {ux[c], uy[c], uz[c]};
+    const long n[3] = {nx, ny, nz};
+    // 0: no neighbour, 1: fluid, 2: solid (value = its wall velocity)
+    auto nbr = [&](int a, int da, T v[3]) -> int {
+      long p[3] = {x, y, z};
+      p[a] += da;
+      if (p[a] < 0 || p[a] >= n[a]) {
+        if (!periodic[a]) return 0;
+        p[a] = (p[a] + n[a]) % n[a];
+      }
+      const size_t pi = pidx(p[0], p[1], p[2]);
+      if (solid[pi]) {
+        for (int k = 0; k < 3; ++k) v[k] = wall_u[3 * pi + k];
+        return 2;
+      }
+      const size_t cc = cidx(p[0], p[1], p[2]);
+      v[0] = ux[cc];
+      v[1] = uy[cc];
+      v[2] = uz[cc];
+      return 1;
+    };
+    T fd[3][3];
+    const T c43 = T(4.0 / 3.0), c13 = T(1.0 / 3.0), half = T(0.5);
+    for (int a = 0; a < 3; ++a) {
+      T pv[3], mv[3];
+      const int kp = nbr(a, 1, pv), km = nbr(a, -1, mv);
+      for (int k = 0; k < 3; ++k) {
+        T v = T(0);
+        if (kp == 1 && km == 1)
+          v = (pv[k] - mv[k]) * half;
+        else if (kp == 1 && km == 2)
+          v = -c43 * mv[k] + own[k] + c13 * pv[k];
+        else if (kp == 2 && km == 1)
+          v = c43 * pv[k] - own[k] - c13 * mv[k];
+        else if (kp == 1 && km == 0)
+          v = pv[k] - own[k];
+        else if (kp == 0 && km == 1)
+          v = own[k] - mv[k];
+        fd[k][a] = v;
+      }
+    }
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) g[a][b] = a == b ? fd[a][b] : sm[a][b] + half * (fd[a][b] - fd[b][a]);
+  }
+
+  // WaleLes::update, les.rs:120-211: nu_t = (Cw Delta)^2 (Sd:Sd)^(3/2) /
+  // ((S:S)^(5/2) + (Sd:Sd)^(5/4)) with Delta = 1 and 0/0 -> 0, optionally
+  // clipped to (tau_eff_max - tau0)/3, then omega = 1/(3 (nu0 + nu_t) + 1/2)
+  // for every cell (solid cells get the laminar rate).  Returns the clipped
+  // cell count; *max_nut = the largest raw nu_t.
+  long wale(T cw, double tau_eff_max, T* omega_out, T* nut_out, T* max_nut) const {
+    const T cw_delta_sq = (cw * T(1)) * (cw * T(1));
+    const T nu0 = T(nu);
+    const T tau0 = T(0.5) + T(3.0) * nu0;
+    const bool clip = tau_eff_max > 0.0;
+    const T nut_max = clip ? (T(tau_eff_max) - tau0) / T(3.0) : T(0);
+    long clipped = 0;
+    T mx = T(0);
+    for (long z = 0; z < nz; ++z)
+      for (long y = 0; y < ny; ++y)
+        for (long x = 0; x < nx; ++x) {
+          T g[3][3];
+          vgrad_at(x, y, z, g);
+          T s[3][3], g2[3][3], sd[3][3];
+          for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) {
+              s[a][b] = T(0.5) * (g[a][b] + g[b][a]);
+              g2[a][b] = T(0);
+              for (int k = 0; k < 3; ++k) g2[a][b] = g2[a][b] + g[a][k] * g[k][b];
+            }
+          const T tr = g2[0][0] + g2[1][1] + g2[2][2];
+          T ss = T(0), sdsd = T(0);
+          for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) {
+              sd[a][b] = T(0.5) * (g2[a][b] + g2[b][a]);
+              if (a == b) sd[a][b] = sd[a][b] - tr / T(3.0);
+            }
+          for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) {
+              ss = ss + s[a][b] * s[a][b];
+              sdsd = sdsd + sd[a][b] * sd[a][b];
+            }
+          const T denom = std::pow(ss, T(2.5)) + std::pow(sdsd, T(1.25));
+          T nut = denom > T(0) ? cw_delta_sq * std::pow(sdsd, T(1.5)) / denom : T(0);
+          mx = std::max(mx, nut);
+          if (clip && nut > nut_max) {
+            ++clipped;
+            nut = nut_max;
+          }
+          const size_t c = cidx(x, y, z);
+          if (nut_out) nut_out[c] = nut;
+          omega_out[c] = T(1) / (T(3.0) * (nu0 + nut) + T(0.5));
+        }
+    *max_nut = mx;
+    return clipped;
+  }
+
   // Solver::init_with, solver.rs:2474-2623: f = feq + f_neq(grad u), then
   // update_moments.  rho_in/u_in are compact global arrays.
   void init_with(const double* rho_in, const double* u_in) {
@@ -1377,6 +1528,56 @@ void orc_set_omega_field(void* p, const double* om) {
       for (size_t c = 0; c < S->nc; ++c) S->omega_field[c] = (TT)om[c];
     }
   });
+}
+
+// Strain rate (N x 6) and velocity gradient (N x 9, row-major du_i/dx_j) of
+// the current state, compact order.
+void orc_strain(void* p, double* out) {
+  Handle* h = (Handle*)p;
+  DISPATCH(h, {
+    using TT = std::remove_reference_t<decltype(S->f[0])>;
+    for (long z = 0; z < S->nz; ++z)
+      for (long y = 0; y < S->ny; ++y)
+        for (long x = 0; x < S->nx; ++x) {
+          TT s[6];
+          S->strain_at(x, y, z, s);
+          for (int k = 0; k < 6; ++k) out[6 * S->cidx(x, y, z) + k] = (double)s[k];
+        }
+  });
+}
+
+void orc_vgrad(void* p, double* out) {
+  Handle* h = (Handle*)p;
+  DISPATCH(h, {
+    using TT = std::remove_reference_t<decltype(S->f[0])>;
+    for (long z = 0; z < S->nz; ++z)
+      for (long y = 0; y < S->ny; ++y)
+        for (long x = 0; x < S->nx; ++x) {
+          TT g[3][3];
+          S->vgrad_at(x, y, z, g);
+          for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) out[9 * S->cidx(x, y, z) + 3 * a + b] = (double)g[a][b];
+        }
+  });
+}
+
+// WALE omega field of the current state (tau_eff_max <= 0: no clip);
+// nut may be null.  Returns the clipped cell count.
+long orc_wale(void* p, double cw, double tau_eff_max, double* omega, double* nut, double* max_nut) {
+  Handle* h = (Handle*)p;
+  long clipped = 0;
+  DISPATCH(h, {
+    using TT = std::remove_reference_t<decltype(S->f[0])>;
+    std::vector<TT> om(S->nc), nt(S->nc);
+    TT mx;
+    clipped = S->wale((TT)cw, tau_eff_max, om.data(), nt.data(), &mx);
+    for (size_t c = 0; c < S->nc; ++c) {
+      omega[c] = (double)om[c];
+      if (nut) nut[c] = (double)nt[c];
+    }
+    *max_nut = (double)mx;
+  });
+  return clipped;
 }
 
 int orc_apply_ibm(void* p, int nm, const double* pos, const double* weight, const double center[3],

@@ -22,6 +22,7 @@ _lib = None
 
 BGK, TRT, CM = 0, 1, 2
 MAGIC_STD = 3.0 / 16.0
+WALE_CW = 0.325  # les.rs:16
 
 
 def build():
@@ -60,6 +61,10 @@ def lib():
         L.orc_clear_force_field.argtypes = [vp]
         L.orc_get_force_field.argtypes = [vp, d]
         L.orc_set_omega_field.argtypes = [vp, d]
+        L.orc_strain.argtypes = [vp, d]
+        L.orc_vgrad.argtypes = [vp, d]
+        L.orc_wale.restype = C.c_long
+        L.orc_wale.argtypes = [vp, C.c_double, C.c_double, d, d, d]
         L.orc_apply_ibm.argtypes = [vp, C.c_int, d, d, d, d, C.c_int, C.c_double, C.c_int, C.c_double, d]
         L.orc_scalar_enable.argtypes = [vp, C.c_double, C.c_int]
         L.orc_scalar_set.argtypes = [vp, C.c_int, d]
@@ -210,6 +215,25 @@ class OracleSolver:
         else:
             om = np.ascontiguousarray(om, np.float64)
             lib().orc_set_omega_field(self.h, _d(om))
+
+    def strain(self):
+        """Strain rate per cell, [Sxx, Syy, Szz, Sxy, Sxz, Syz] (solver.rs:5393-5447)."""
+        out = np.zeros((self.n, 6))
+        lib().orc_strain(self.h, _d(out))
+        return out
+
+    def velocity_gradient(self):
+        """du_i/dx_j per cell as (N, 3, 3) (solver.rs:5490-5581)."""
+        out = np.zeros((self.n, 3, 3))
+        lib().orc_vgrad(self.h, _d(out))
+        return out
+
+    def wale(self, cw=WALE_CW, tau_eff_max=None):
+        """WaleLes::update (les.rs:120-211) without installing the field:
+        returns (omega, nu_t, clipped cells, max raw nu_t)."""
+        om, nt, mx = np.zeros(self.n), np.zeros(self.n), np.zeros(1)
+        k = lib().orc_wale(self.h, cw, 0.0 if tau_eff_max is None else tau_eff_max, _d(om), _d(nt), _d(mx))
+        return om, nt, int(k), float(mx[0])
 
     def apply_ibm(self, pos, weight, center, omega, max_iter=3, tol=1e-2, radius=1, relax=1.0):
         pos = np.ascontiguousarray(pos, np.float64).reshape(-1)
