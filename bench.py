@@ -274,6 +274,58 @@ def workload_adjoint(args):
     }
 
 
+def workload_les(args):
+    """The 512^3 tank step with the WALE closure recomputed on the device
+    before every collision (moments, WALE omega field, collide-stream with
+    that field); reports the step rate and the closure's share of it."""
+    cfg = tank_config(args.size)
+    Q = args.lattice
+    s, n_markers = build_gpu(cfg, args.storage, args.collision, Q)
+    fluid = s.fluid_cells()
+    s.run_with_ibm(args.warmup, graph=False)
+    dt0, k0, _, _ = timed_span(s, args.steps)
+    s.set_wale(True)
+    s.run_with_ibm(10, graph=False)
+    dt, k_ms, tq, diag = timed_span(s, args.steps)
+    d, nut = s.wale_update(want_nu_t=True)
+    copy_gbps = s.ctx.stream_copy_gbps(4 << 30, 20)
+    s.close()
+    ms, ms0 = 1e3 * dt / args.steps, 1e3 * dt0 / args.steps
+    S = STORAGE_BYTES[args.storage]
+    # the closure's algorithmic bytes: per fluid cell the moment pass reads
+    # Q populations and the mask and writes 4 moments, the WALE pass reads Q
+    # populations, the cell's moments and mask and writes omega (neighbour
+    # velocities come from L2); per solid cell both passes read the mask and
+    # write 4 moments and omega
+    n_total = args.size ** 3
+    b_cell = (Q * S + 4 + 16) + (Q * S + 16 + 4 + 4)
+    b_solid = (4 + 16) + (4 + 4)
+    closure_bytes = fluid * b_cell + (n_total - fluid) * b_solid
+    wale_ms = ms - ms0
+    print(f"[bench] les: {ms:.3f} ms/step (plain {ms0:.3f}), closure {wale_ms:.3f} ms, "
+          f"max nu_t {d['max_nu_t_before_clipping']:.3e}", file=sys.stderr, flush=True)
+    return {
+        "metric": "MLUPS, D3Q27 tank 512^3 with per-step WALE LES",
+        "value": round(fluid * args.steps / dt / 1e6, 2), "unit": "MLUPS", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32" if args.storage == "f32" else f"f32 math, {args.storage} storage",
+        "data": "synthetic (deterministic tank geometry and Rushton IBM markers, flow from rest)",
+        "config": {"workload": f"stirred tank D3Q{Q} {args.collision} {args.size}^3 + rotating IBM + WALE per step",
+                   "grid": [args.size] * 3, "fluid_cells": int(fluid), "markers": n_markers,
+                   "parallelism": "single GPU"},
+        "plain_ms_per_step": round(ms0, 4),
+        "closure_ms_per_step": round(wale_ms, 4),
+        "closure_algorithmic_gbs": round(closure_bytes / (wale_ms * 1e-3) / 1e9, 1) if wale_ms > 0 else None,
+        "closure_bytes_per_fluid_cell": b_cell,
+        "closure_bytes_per_solid_cell": b_solid,
+        "stream_copy_gbs": round(copy_gbps, 1),
+        "k_hydro_avg_ms": round(k_ms, 4),
+        "max_nu_t": d["max_nu_t_before_clipping"],
+        "mean_nu_t_over_nu": float(np.mean(nut[nut > 0]) / cfg["nu"]) if np.any(nut > 0) else 0.0,
+        "checks": {"torque_z": float(diag["torque"][2])},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -287,8 +339,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=100)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-sweep", action="store_true", help="skip the 16-bit storage and Np-accuracy sweep")
-    ap.add_argument("--workload", default="tank", choices=["tank", "batch", "adjoint"],
-                    help="tank: the 512^3 benchmark (default); batch: 64-tank sweep; adjoint: torque gradient")
+    ap.add_argument("--workload", default="tank", choices=["tank", "batch", "adjoint", "les"],
+                    help="tank: the 512^3 benchmark (default); batch: 64-tank sweep; adjoint: torque gradient; "
+                         "les: the tank with per-step WALE")
     ap.add_argument("--seed", type=int, default=20260512)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -296,11 +349,11 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     if args.steps is None:
-        args.steps = {"tank": 200, "batch": 5000, "adjoint": 4000}[args.workload]
+        args.steps = {"tank": 200, "batch": 5000, "adjoint": 4000, "les": 100}[args.workload]
     if args.warmup is None:
-        args.warmup = {"tank": 500, "batch": 100, "adjoint": 0}[args.workload]
+        args.warmup = {"tank": 500, "batch": 100, "adjoint": 0, "les": 200}[args.workload]
     if args.workload != "tank":
-        fn = workload_batch if args.workload == "batch" else workload_adjoint
+        fn = {"batch": workload_batch, "adjoint": workload_adjoint, "les": workload_les}[args.workload]
         print(json.dumps(fn(args)), flush=True)
         return
 

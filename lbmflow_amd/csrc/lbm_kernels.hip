@@ -695,9 +695,10 @@ __global__ void k_moments(Dom d, const S* f, const uint32_t* mask, const float* 
     constexpr int q = decltype(qc)::value;
     fv[q] = read_pop<Q, S, q>(f, d.ps, nb, i, m, odd != 0);
   });
+  // the field is zero off the force list, whose fluid cells carry kMaskForced
   float Fm[3] = {p.F[0], p.F[1], p.F[2]};
   bool fon = p.force_on != 0;
-  if (F_mom) {
+  if (F_mom && (m & kMaskForced)) {
     for (int c = 0; c < 3; ++c) Fm[c] = p.F[c] + F_mom[3 * (size_t)i + c];
     fon = true;
   }

@@ -26,7 +26,8 @@ struct WaleArgs {
   const void* f;
   const uint32_t* mask;
   const float* wall_u;  // N x 3 or null (velocity of each solid cell's wall)
-  const float* F_cur;   // N x 3 or null (current force field)
+  const float* F_cur;   // N x 3 or null (current force field, nonzero only on
+                        // the force list, whose fluid cells carry kMaskForced)
   const float* mom;     // 4 x N: rho, ux, uy, uz (cached moments)
   HydroParams p;        // the uniform force
   int odd;
@@ -38,6 +39,41 @@ struct WaleArgs {
   float* nut;         // N or null: nu_t out (after the clip)
   uint32_t* stats;    // null or [clipped cells, bits of the largest raw nu_t]
 };
+
+// Population loads marked non-temporal: each population is read once, so
+// the L2 is left to the velocity planes the neighbour differences reuse.
+template <typename S>
+__device__ __forceinline__ float ld_nt(const S* p, uint32_t i);
+template <>
+__device__ __forceinline__ float ld_nt<float>(const float* p, uint32_t i) {
+  return __builtin_nontemporal_load(p + i);
+}
+template <>
+__device__ __forceinline__ float ld_nt<_Float16>(const _Float16* p, uint32_t i) {
+  return (float)__builtin_nontemporal_load(p + i);
+}
+template <>
+__device__ __forceinline__ float ld_nt<bf16s>(const bf16s* p, uint32_t i) {
+  const uint16_t v = __builtin_nontemporal_load((const uint16_t*)p + i);
+  return __uint_as_float((uint32_t)v << 16);
+}
+template <int Q, typename S, int q>
+__device__ __forceinline__ float read_pop_nt(const S* base, size_t ps, const Nbr& nb, uint32_t i, uint32_t m,
+                                             bool odd) {
+  constexpr int o = lat_opp(q);
+  const S* plane;
+  uint32_t j;
+  if (!odd || ((m >> q) & 1u)) {
+    plane = base + (size_t)q * ps;
+    j = i;
+  } else {
+    plane = base + (size_t)o * ps;
+    j = i + (uint32_t)nb.template off<-lat_c(q, 0), -lat_c(q, 1), -lat_c(q, 2)>();
+  }
+  float v = ld_nt<S>(plane, j);
+  if constexpr (Store<S>::kScaled) v = v * lat_wf(Q, q);
+  return v;
+}
 
 template <int Q, typename S>
 __global__ __launch_bounds__(256) void k_wale(WaleArgs a) {
@@ -61,7 +97,7 @@ __global__ __launch_bounds__(256) void k_wale(WaleArgs a) {
     sfor<0, Q>([&](auto qc) {
       constexpr int q = decltype(qc)::value;
       constexpr int cx = lat_c(q, 0), cy = lat_c(q, 1), cz = lat_c(q, 2);
-      const float fq = read_pop<Q, S, q>((const S*)a.f, a.d.ps, nb, i, m, a.odd != 0);
+      const float fq = read_pop_nt<Q, S, q>((const S*)a.f, a.d.ps, nb, i, m, a.odd != 0);
       const float cu = cdot<q>(u0, u1, u2);
       const float feq = lat_wf(Q, q) * (drho + r * (3.0f * cu + 4.5f * cu * cu - 1.5f * usq));
       const float fn = fq - feq;
@@ -78,7 +114,7 @@ __global__ __launch_bounds__(256) void k_wale(WaleArgs a) {
     // Guo-force correction of the stress: + u F (diagonal), + (u_a F_b +
     // u_b F_a)/2 (off-diagonal), F = uniform force + current field
     float F0 = a.p.F[0], F1 = a.p.F[1], F2 = a.p.F[2];
-    if (a.F_cur) {
+    if (a.F_cur && (m & kMaskForced)) {  // the field is zero off the force list
       F0 = F0 + a.F_cur[3 * (size_t)i + 0];
       F1 = F1 + a.F_cur[3 * (size_t)i + 1];
       F2 = F2 + a.F_cur[3 * (size_t)i + 2];

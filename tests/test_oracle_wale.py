@@ -12,9 +12,11 @@ mathematics fixes:
   * WALE closed forms: nu_t = 0 in pure shear, Cw^2 (2/3)^(1/4) |w| in
     solid-body rotation, Cw^2 |a| 6^1.5 / (6^2.5 + 6^1.25) in axisymmetric
     strain;
-  * the Guo-force term of the stress: in body-force Poiseuille flow (exact
-    for TRT with magic 3/16) the corrected S_xx vanishes and S_xy is the
-    analytic du/dy / 2.
+  * body-force Poiseuille flow (exact for TRT with magic 3/16): S_xy is
+    the analytic du/dy / 2;
+  * the Guo-force term of the stress: a uniformly accelerated flow has zero
+    strain, which the corrected stress reproduces to round-off while the raw
+    stress does not.
 """
 import numpy as np
 import pytest
