@@ -70,7 +70,7 @@ extern "C" {
 #define LBM_CAP_ROTATING_IBM 0x40u
 #define LBM_CAP_SCALAR 0x80u
 #define LBM_CAP_PERIODIC 0x100u
-#define LBM_CAP_OPEN_FACES 0x200u    /* not set: open-face BCs are not on this path */
+#define LBM_CAP_OPEN_FACES 0x200u    /* Zou-He velocity/pressure, outflow, convective faces */
 #define LBM_CAP_TWO_PASS 0x400u      /* not set: one fused pass per step */
 #define LBM_CAP_GRAVITY 0x800u       /* not set */
 #define LBM_CAP_PROBE 0x1000u        /* not set: momentum-exchange probe */
@@ -136,8 +136,8 @@ size_t lbm_fields_device_bytes(const lbm_fields* fl);
  * cell, nonzero = solid) and the per-cell wall velocity (meaningful on
  * solids).  Either pointer may be NULL (all fluid / resting walls).
  * A domain with a fluid cell on a non-periodic face cannot be stepped
- * (open faces are not on this path): lbm_run_span then fails with
- * LBM_ERR_UNSUPPORTED until the masks close it. */
+ * unless an open-face rule covers that face (lbm_set_face_bc): lbm_run_span
+ * fails with LBM_ERR_UNSUPPORTED until the masks or a face rule close it. */
 int lbm_set_masks(lbm_fields* fl, const uint8_t* solid, const float* wall_u);
 int lbm_read_solid(lbm_fields* fl, uint8_t* solid);
 /* Half-way bounce-back link list (implicit in stream_row, kernels.rs:572-603):
@@ -149,6 +149,29 @@ int lbm_boundary_links(lbm_fields* fl, int64_t* count, int64_t* cells, uint8_t* 
 int lbm_set_params(lbm_fields* fl, const lbm_step_params* p);
 /* Per-cell omega_plus (docs/KERNEL_EXTENSION_POINTS.md:22-73): NULL clears. */
 int lbm_set_omega_field(lbm_fields* fl, const float* omega);
+
+/* Open faces: FaceBC, params.rs:232-252, on the faces in Face::ALL order
+ * (lattice.rs:38-45).  Applied after every collide-stream step in face
+ * order, after the populations skipped by the stream have been restored to
+ * their previous post-collide values (apply_open_faces_impl,
+ * backend.rs:602-705; kernels.rs:849-1299).  No inlet profiles or face
+ * patches on this path. */
+#define LBM_FACE_XNEG 0
+#define LBM_FACE_XPOS 1
+#define LBM_FACE_YNEG 2
+#define LBM_FACE_YPOS 3
+#define LBM_FACE_ZNEG 4
+#define LBM_FACE_ZPOS 5
+#define LBM_FACE_CLOSED 0     /* no open-face pass (wall rim or periodic) */
+#define LBM_FACE_VELOCITY 1   /* Zou-He, value = u[3] */
+#define LBM_FACE_PRESSURE 2   /* Zou-He, value[0] = rho */
+#define LBM_FACE_OUTFLOW 3    /* zero-gradient copy from one cell inward */
+#define LBM_FACE_CONVECTIVE 4 /* value[0] = u_conv, mass pinned to the inner neighbour */
+/* GlobalSpec faces[face] = kind (params.rs:302-320): an open face on a
+ * periodic axis is LBM_ERR_INVALID.  Stepping refuses fluid cells on a
+ * non-periodic face that no open face covers, and scalar distributions
+ * together with open faces (LBM_ERR_UNSUPPORTED). */
+int lbm_set_face_bc(lbm_fields* fl, int face, int kind, const double value[3]);
 
 /* WALE large-eddy closure as an omega_plus producer.
  * WaleLesDiagnostics, les.rs:18-31 (tau_eff_max / nu_t_max are 0 when the

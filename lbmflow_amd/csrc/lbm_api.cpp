@@ -86,6 +86,18 @@ struct lbm_fields {
   HydroArgs g_args{};
   int g_parity = -1;
   uint64_t g_gen = 0;
+  // open faces (Face::ALL order): rule and values, the per-face stash
+  // (two generations of unknowns x face cells), which generation is current,
+  // and the fluid face cells no open face covers
+  int face_kind[6] = {0, 0, 0, 0, 0, 0};
+  float face_u[6][3] = {};
+  float face_rho[6] = {};
+  float face_uconv[6] = {};
+  float* stash[6][2] = {};
+  size_t stash_bytes[6] = {};
+  int stash_sel = 0;
+  unsigned int uncovered = 0;
+  int g_stash = -1;
   // WALE LES: cached-moment scratch (4 x N), diagnostics words, and the
   // per-step mode's coefficients
   float* wale_mom = nullptr;
@@ -266,6 +278,36 @@ static int normalize_parity(lbm_fields* fl) {
   return LBM_OK;
 }
 
+uint32_t open_bits(const lbm_fields* fl) {
+  uint32_t b = 0;
+  for (int k = 0; k < 6; ++k)
+    if (fl->face_kind[k]) b |= 1u << k;
+  return b;
+}
+
+// Fluid cells on a non-periodic face that no open face rule covers (their
+// skipped populations would stay stale forever): with no open face these are
+// all the face fluid cells.
+int count_uncovered(lbm_fields* fl) {
+  const uint32_t bits = open_bits(fl);
+  if (!bits || !fl->open_face_cells) {
+    fl->uncovered = fl->open_face_cells;
+    return LBM_OK;
+  }
+  unsigned int* cnt = nullptr;
+  int rc = dev_alloc(fl, (void**)&cnt, sizeof(unsigned int));
+  if (rc) return rc;
+  hipStream_t s = fl->ctx->stream;
+  HIPCHK(fl->ctx, hipMemsetAsync(cnt, 0, sizeof(unsigned int), s));
+  HIPCHK(fl->ctx, launch_uncovered(fl->d, fl->solid, bits, cnt, s));
+  unsigned int n = 0;
+  HIPCHK(fl->ctx, hipMemcpyAsync(&n, cnt, sizeof n, hipMemcpyDeviceToHost, s));
+  HIPCHK(fl->ctx, hipStreamSynchronize(s));
+  dev_free(fl, cnt, sizeof(unsigned int));
+  fl->uncovered = n;
+  return LBM_OK;
+}
+
 int rebuild_mask(lbm_fields* fl) {
   int rc = normalize_parity(fl);
   if (rc) return rc;
@@ -284,6 +326,8 @@ int rebuild_mask(lbm_fields* fl) {
   // Fluid on a non-periodic face needs an open-face BC, which is not on this
   // path; stepping such a domain is refused (lbm_run_span).
   fl->open_face_cells = nerr;
+  int rc2 = count_uncovered(fl);
+  if (rc2) return rc2;
   return rebuild_tiles(fl);
 }
 
@@ -303,6 +347,7 @@ HydroArgs hydro_args(lbm_fields* fl) {
   a.nlist = (uint32_t)fl->flist_host.size();
   a.F_mom = fl->F_mom;
   a.F_cur = fl->F_cur;
+  a.open = fl->open_face_cells ? 1 : 0;
   return a;
 }
 
@@ -403,7 +448,7 @@ int lbm_stream_copy(lbm_ctx* ctx, size_t bytes, int iterations, double* gbps) {
 int lbm_capabilities(const lbm_ctx* ctx, uint32_t* flags) {
   if (!ctx || !flags) return LBM_ERR_INVALID;
   *flags = LBM_CAP_TRT | LBM_CAP_CENTRAL_MOMENT | LBM_CAP_F16_STORAGE | LBM_CAP_BF16_STORAGE | LBM_CAP_FORCE_FIELD |
-           LBM_CAP_OMEGA_FIELD | LBM_CAP_ROTATING_IBM | LBM_CAP_SCALAR | LBM_CAP_PERIODIC | LBM_CAP_WALE;
+           LBM_CAP_OMEGA_FIELD | LBM_CAP_ROTATING_IBM | LBM_CAP_SCALAR | LBM_CAP_PERIODIC | LBM_CAP_WALE | LBM_CAP_OPEN_FACES;
   return LBM_OK;
 }
 
@@ -495,6 +540,9 @@ void lbm_fields_free(lbm_fields* fl) {
                   (void*)fl->F_cur, (void*)fl->F_mom, (void*)fl->flist, (void*)fl->partials, (void*)fl->tiles,
                   (void*)fl->ring, (void*)fl->ring_pos, (void*)fl->wale_mom, (void*)fl->wale_stats})
     if (p) (void)hipFree(p);
+  for (int k = 0; k < 6; ++k)
+    for (int g = 0; g < 2; ++g)
+      if (fl->stash[k][g]) (void)hipFree(fl->stash[k][g]);
   delete fl;
 }
 
@@ -818,11 +866,78 @@ int lbm_set_wale(lbm_fields* fl, int enabled, double cw, double tau_eff_max) {
 }
 
 static int check_steppable(lbm_fields* fl) {
-  if (fl->open_face_cells)
+  if (fl->uncovered)
     return fail(fl->ctx, LBM_ERR_UNSUPPORTED,
-                "%u fluid cells lie on a non-periodic domain face; open faces are not supported "
-                "(close the domain with solid rims or make the axis periodic)",
-                fl->open_face_cells);
+                "%u fluid cells lie on a non-periodic domain face with no open-face rule "
+                "(set one with lbm_set_face_bc, close the face with solid rims or make the axis periodic)",
+                fl->uncovered);
+  if (fl->open_face_cells && !fl->scalars.empty())
+    return fail(fl->ctx, LBM_ERR_UNSUPPORTED, "scalar distributions do not take open faces on this path");
+  return LBM_OK;
+}
+
+// ---- open faces ------------------------------------------------------------
+int lbm_set_face_bc(lbm_fields* fl, int face, int kind, const double value[3]) {
+  if (!fl || face < 0 || face > 5 || kind < LBM_FACE_CLOSED || kind > LBM_FACE_CONVECTIVE)
+    return LBM_ERR_INVALID;
+  const int ax = face / 2;
+  if (kind != LBM_FACE_CLOSED && fl->d.per[ax])
+    return fail(fl->ctx, LBM_ERR_INVALID, "face %d is on a periodic axis and cannot be open", face);
+  if (kind != LBM_FACE_CLOSED && kind != LBM_FACE_OUTFLOW && !value) return LBM_ERR_INVALID;
+  if (kind == LBM_FACE_VELOCITY)
+    for (int a = 0; a < 3; ++a)
+      if (!std::isfinite(value[a])) return fail(fl->ctx, LBM_ERR_INVALID, "face velocity must be finite");
+  if (kind == LBM_FACE_PRESSURE && !(value[0] > 0.0 && std::isfinite(value[0])))
+    return fail(fl->ctx, LBM_ERR_INVALID, "face density must be finite and positive");
+  if (kind == LBM_FACE_CONVECTIVE && !(value[0] >= 0.0 && std::isfinite(value[0])))
+    return fail(fl->ctx, LBM_ERR_INVALID, "convective speed must be finite and non-negative");
+  if (kind != LBM_FACE_CLOSED && fl->Q != 19 && fl->Q != 27) return LBM_ERR_UNSUPPORTED;
+  const uint32_t ext[3] = {fl->d.nx, fl->d.ny, fl->d.nz};
+  if (kind != LBM_FACE_CLOSED && ext[ax] < 3)
+    return fail(fl->ctx, LBM_ERR_INVALID, "an open face needs at least 3 cells along its axis");
+  const uint32_t t1 = ax == 0 ? 1 : 0, t2 = ax == 2 ? 1 : 2;
+  const size_t nf = (size_t)ext[t1] * ext[t2];
+  const size_t bytes = nf * (fl->Q == 27 ? 9 : 5) * sizeof(float);
+  hipStream_t s = fl->ctx->stream;
+  if (kind != LBM_FACE_CLOSED && !fl->stash[face][0]) {
+    for (int g = 0; g < 2; ++g) {
+      int rc = dev_alloc(fl, (void**)&fl->stash[face][g], bytes);
+      if (rc) return rc;
+      // the stale slots start as the reference's zeroed second buffer
+      HIPCHK(fl->ctx, hipMemsetAsync(fl->stash[face][g], 0, bytes, s));
+    }
+    fl->stash_bytes[face] = bytes;
+    HIPCHK(fl->ctx, hipStreamSynchronize(s));
+  }
+  fl->face_kind[face] = kind;
+  for (int a = 0; a < 3; ++a) fl->face_u[face][a] = kind == LBM_FACE_VELOCITY ? (float)value[a] : 0.0f;
+  fl->face_rho[face] = kind == LBM_FACE_PRESSURE ? (float)value[0] : 0.0f;
+  fl->face_uconv[face] = kind == LBM_FACE_CONVECTIVE ? (float)value[0] : 0.0f;
+  fl->gen++;
+  return count_uncovered(fl);
+}
+
+static int open_faces_step(lbm_fields* fl) {
+  FaceArgs a{};
+  a.d = fl->d;
+  a.f = fl->f;
+  a.mask = fl->mask;
+  a.odd = fl->parity ^ 1;  // the state the step just produced
+  a.p = fl->hp;
+  a.F_cur = fl->field_present ? fl->F_cur : nullptr;
+  for (int k = 0; k < 6; ++k) {
+    if (!fl->face_kind[k]) continue;
+    a.faces[a.nfaces++] = k;
+    a.kind[k] = fl->face_kind[k];
+    for (int c = 0; c < 3; ++c) a.u[k][c] = fl->face_u[k][c];
+    a.rho[k] = fl->face_rho[k];
+    a.uconv[k] = fl->face_uconv[k];
+    a.stash_cur[k] = fl->stash[k][fl->stash_sel];
+    a.stash_next[k] = fl->stash[k][fl->stash_sel ^ 1];
+  }
+  if (!a.nfaces) return LBM_OK;
+  HIPCHK(fl->ctx, launch_open_faces(fl->Q, fl->storage, a, fl->ctx->stream));
+  fl->stash_sel ^= 1;
   return LBM_OK;
 }
 
@@ -851,6 +966,10 @@ static int hydro_step(lbm_fields* fl) {
     fl->ev_used += 2;
   }
   if (a.nlist) HIPCHK(fl->ctx, launch_hydro_list(fl->Q, fl->storage, fl->collision, fl->parity, a, s));
+  if (fl->open_face_cells) {
+    int rc = open_faces_step(fl);
+    if (rc) return rc;
+  }
   fl->parity ^= 1;
   fl->time += 1;
   for (ScalarState* sc : fl->scalars) {
@@ -919,7 +1038,7 @@ int lbm_run_ibm_span(lbm_fields* fl, int steps, int use_graph, double* torque, l
   if (use_graph && !fl->timing && steps >= 2) {
     const HydroArgs a = hydro_args(fl);
     const bool stale = !fl->gexec || fl->g_parity != fl->parity || fl->g_gen != fl->gen ||
-                       memcmp(&a, &fl->g_args, sizeof a) != 0;
+                       fl->g_stash != fl->stash_sel || memcmp(&a, &fl->g_args, sizeof a) != 0;
     if (stale) {
       if (fl->gexec) (void)hipGraphExecDestroy(fl->gexec);
       if (fl->graph) (void)hipGraphDestroy(fl->graph);
@@ -946,6 +1065,7 @@ int lbm_run_ibm_span(lbm_fields* fl, int steps, int use_graph, double* torque, l
       fl->g_args = a;
       fl->g_parity = fl->parity;
       fl->g_gen = fl->gen;
+      fl->g_stash = fl->stash_sel;
     }
     for (; k + 2 <= steps; k += 2) {
       HIPCHK(fl->ctx, hipGraphLaunch(fl->gexec, s));
@@ -1721,6 +1841,8 @@ int lbm_adjoint_torque(lbm_fields* fl, int steps, int window, int snapshots, dou
   if (fl->wall_u) return fail(fl->ctx, LBM_ERR_UNSUPPORTED, "the adjoint does not cover moving walls");
   if (fl->wale_on)
     return fail(fl->ctx, LBM_ERR_UNSUPPORTED, "the adjoint does not differentiate the per-step WALE closure");
+  if (fl->open_face_cells)
+    return fail(fl->ctx, LBM_ERR_UNSUPPORTED, "the adjoint covers closed domains (no open faces)");
   if (fl->hp.force_on) return fail(fl->ctx, LBM_ERR_UNSUPPORTED, "the adjoint does not cover a body force");
   int rc = check_steppable(fl);
   if (rc) return rc;
@@ -1977,6 +2099,8 @@ int lbm_checkpoint_write_part(lbm_fields* fl, const char* path, lbm_ckpt_info* i
   if (!fl || !path || !info) return LBM_ERR_INVALID;
   if (!fl->scalars.empty())
     return fail(fl->ctx, LBM_ERR_UNSUPPORTED, "CKPT_UNSUPPORTED: scalar distributions are not checkpointed");
+  if (open_bits(fl))
+    return fail(fl->ctx, LBM_ERR_UNSUPPORTED, "CKPT_UNSUPPORTED: open-face stale slots are not checkpointed");
   const PadGeom g(fl->d);
   const int per[3] = {(int)fl->d.per[0], (int)fl->d.per[1], (int)fl->d.per[2]};
   std::vector<float> f, wall, mom, force;
@@ -2039,6 +2163,8 @@ int lbm_checkpoint_read_part(lbm_fields* fl, const char* path, int64_t step, lbm
   if (!fl || !path || !info || step < 0) return LBM_ERR_INVALID;
   if (!fl->scalars.empty())
     return fail(fl->ctx, LBM_ERR_UNSUPPORTED, "CKPT_UNSUPPORTED: scalar distributions are not checkpointed");
+  if (open_bits(fl))
+    return fail(fl->ctx, LBM_ERR_UNSUPPORTED, "CKPT_UNSUPPORTED: open-face stale slots are not checkpointed");
   FILE* fp = fopen(path, "rb");
   if (!fp) return fail(fl->ctx, LBM_ERR_INVALID, "CKPT_IO: cannot open %s", path);
   std::vector<uint8_t> b;

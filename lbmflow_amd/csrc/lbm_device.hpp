@@ -180,6 +180,19 @@ __device__ __forceinline__ float read_pop(const S* base, size_t ps, const Nbr& n
   return pop_ld<S, Q, q>(base + (size_t)o * ps, s);
 }
 
+// write population q of cell i where read_pop finds it (the inverse map)
+template <int Q, typename S, int q>
+__device__ __forceinline__ void write_pop(S* base, size_t ps, const Nbr& nb, uint32_t i, uint32_t m, bool odd,
+                                          float v) {
+  constexpr int o = lat_opp(q);
+  if (!odd || ((m >> q) & 1u)) {
+    pop_st<S, Q, q>(base + (size_t)q * ps, i, v);
+    return;
+  }
+  const uint32_t s = i + (uint32_t)nb.template off<-lat_c(q, 0), -lat_c(q, 1), -lat_c(q, 2)>();
+  pop_st<S, Q, q>(base + (size_t)o * ps, s, v);
+}
+
 __device__ __forceinline__ void cell_xyz(const Dom& d, uint32_t i, uint32_t& x, uint32_t& y, uint32_t& z) {
   x = i % d.nx;
   const uint32_t t = i / d.nx;

@@ -64,6 +64,27 @@ struct HydroArgs {
   uint32_t nlist;
   float* F_mom;             // N x 3: force of the cached moments
   const float* F_cur;       // N x 3: current field
+  int open;                 // fluid on a non-periodic (open) face: wrap every offset
+};
+
+// Open faces (lbm_open.hip): FaceBC kinds of params.rs:232-252, faces in
+// Face::ALL order (XNeg, XPos, YNeg, YPos, ZNeg, ZPos).
+constexpr int kFaceClosed = 0, kFaceVelocity = 1, kFacePressure = 2, kFaceOutflow = 3, kFaceConvective = 4;
+struct FaceArgs {
+  Dom d;
+  void* f;
+  const uint32_t* mask;
+  int odd;              // parity of the state the step just produced
+  HydroParams p;        // uniform force
+  const float* F_cur;   // N x 3 or null (current field, nonzero only on the force list)
+  int nfaces;           // open faces, in Face::ALL order
+  int faces[6];
+  int kind[6];
+  float u[6][3];
+  float rho[6];
+  float uconv[6];
+  float* stash_cur[6];   // per face: unknowns x face cells, previous step's post-collide values
+  float* stash_next[6];  // this step's, captured before the restore
 };
 
 // Host-side launchers (lbm_kernels.hip).  All enqueue on `s`.
@@ -82,6 +103,10 @@ hipError_t launch_adj_collide_stream(int storage, int odd, const Dom& d, const v
                                      const HydroParams& p, const double* lam_in, double* lam_out, double* omega_bar,
                                      double* Fbar, double* Gout, hipStream_t s);
 hipError_t launch_fill_f64(double* p, size_t n, double v, hipStream_t s);
+// capture, restore and the per-face rules after a step (lbm_open.hip)
+hipError_t launch_open_faces(int Q, int storage, const FaceArgs& a, hipStream_t s);
+hipError_t launch_uncovered(const Dom& d, const uint8_t* solid, uint32_t open_bits, unsigned int* count,
+                            hipStream_t s);
 // WALE omega_plus field from the state and its cached moments `mom` (4 x N:
 // rho, ux, uy, uz) (lbm_les.hip).  `stats` (2 words, cleared here by a
 // kernel): clipped cell count, bits of the largest raw nu_t.

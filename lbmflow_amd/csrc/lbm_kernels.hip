@@ -1022,7 +1022,7 @@ static hipError_t hydro_q(int odd, const HydroArgs& a, hipStream_t s, bool list)
   if (a.ntiles == 0) return hipSuccess;
   const dim3 grid((a.ntiles + 3) / 4), block(256);
   const bool fon = a.p.force_on != 0;
-  const bool wrap = a.d.per[0] || a.d.per[1] || a.d.per[2];
+  const bool wrap = a.d.per[0] || a.d.per[1] || a.d.per[2] || a.open;
   const int sel = (odd ? 4 : 0) | (fon ? 2 : 0) | (wrap ? 1 : 0);
   switch (sel) {
     case 0: hipLaunchKernelGGL((k_hydro<Q, S, KIND, false, false, false>), grid, block, 0, s, a); break;

@@ -228,6 +228,20 @@ class Solver:
             om = np.ascontiguousarray(omega, np.float32).reshape(-1)
             self._chk(self.lib.lbm_set_omega_field(self.h, _fp(om)))
 
+    FACES = {"x-": 0, "x+": 1, "y-": 2, "y+": 3, "z-": 4, "z+": 5}
+    FACE_KINDS = {"closed": 0, "velocity": 1, "pressure": 2, "outflow": 3, "convective": 4}
+
+    def set_face_bc(self, face, kind: str, value=None):
+        """``GlobalSpec::faces[face]`` (params.rs:232-252): face 0..5 or
+        "x-".."z+"; kind closed | velocity (value = u) | pressure (value =
+        rho) | outflow | convective (value = u_conv)."""
+        f = self.FACES[face] if isinstance(face, str) else int(face)
+        v = np.zeros(3)
+        if value is not None:
+            vv = np.atleast_1d(np.asarray(value, np.float64))
+            v[:vv.size] = vv
+        self._chk(self.lib.lbm_set_face_bc(self.h, f, self.FACE_KINDS[kind], _dp(v)))
+
     def wale_update(self, cw: float = WALE_CW, tau_eff_max: float | None = None, want_nu_t: bool = False):
         """One ``WaleLes::update`` (les.rs:120-211): install the WALE omega
         field of the current state.  Returns the diagnostics dict

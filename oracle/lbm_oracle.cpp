@@ -219,6 +219,13 @@ struct Solver {
   T cr[27][3], wr[27];
   std::vector<ScalarDist<T>> scalars;
   long long time = 0;
+  // open faces (FaceBC, params.rs:232-252), Face::ALL order XNeg..ZPos:
+  // kind 0 closed, 1 Zou-He velocity, 2 Zou-He pressure, 3 outflow,
+  // 4 convective
+  int face_kind[6] = {0, 0, 0, 0, 0, 0};
+  T face_u[6][3] = {};
+  T face_rho[6] = {};
+  T face_uconv[6] = {};
 
   size_t pidx(long x, long y, long z) const {
     return ((size_t)(z + 1) * py + (size_t)(y + 1)) * px + (size_t)(x + 1);
@@ -598,13 +605,245 @@ struct Solver {
     }
   }
 
+  // ---- open faces ----------------------------------------------------------
+  // Face geometry, lattice.rs:38-90: axis = face/2, inward normal +1 on the
+  // low face, tangents ascending; unknowns = {q : c_q . n_in > 0} ascending
+  // (face_unknowns, lattice.rs:193-214).
+  int dir_of(int cx, int cy, int cz) const {
+    for (int q = 0; q < L.Q; ++q)
+      if (L.c[q][0] == cx && L.c[q][1] == cy && L.c[q][2] == cz) return q;
+    return -1;
+  }
+  struct FaceGeo {
+    int a, t1, t2, nin[3];
+    long fixed;
+    int unk[9];
+    int nunk;
+  };
+  FaceGeo face_geo(int face) const {
+    FaceGeo g{};
+    g.a = face / 2;
+    const bool neg = face % 2 == 0;
+    g.nin[g.a] = neg ? 1 : -1;
+    g.t1 = g.a == 0 ? 1 : 0;
+    g.t2 = g.a == 2 ? 1 : 2;
+    const long n[3] = {nx, ny, nz};
+    g.fixed = neg ? 0 : n[g.a] - 1;
+    g.nunk = 0;
+    for (int q = 0; q < L.Q; ++q) {
+      int dot = L.c[q][0] * g.nin[0] + L.c[q][1] * g.nin[1] + L.c[q][2] * g.nin[2];
+      if (dot > 0) g.unk[g.nunk++] = q;
+    }
+    return g;
+  }
+  // for_face_cells_selected with FaceCellSelection::All (kernels.rs:742-766):
+  // t2 outer, t1 inner
+  template <class F>
+  void for_face_cells(const FaceGeo& g, F&& body) {
+    const long n[3] = {nx, ny, nz};
+    for (long c2 = 0; c2 < n[g.t2]; ++c2)
+      for (long c1 = 0; c1 < n[g.t1]; ++c1) {
+        long pos[3];
+        pos[g.a] = g.fixed;
+        pos[g.t1] = c1;
+        pos[g.t2] = c2;
+        body(pos);
+      }
+  }
+
+  // zou_he_face_3d (D3Q19, kernels.rs:1085-1219) and zou_he_face_d3q27
+  // (kernels.rs:958-1083); kind 1 velocity, 2 pressure.  Gravity is not on
+  // this path (g = 0); F0 = params force + field (force_independent_at).
+  void zou_he_face(int face) {
+    const FaceGeo g = face_geo(face);
+    const int a = g.a, t1 = g.t1, t2 = g.t2;
+    const T nsign = T(g.nin[a]);
+    const T half = T(0.5), two = T(2.0), one = T(1);
+    const bool vel = face_kind[face] == 1;
+    const T* ub = face_u[face];
+    const T rho_bc = face_rho[face];
+    if (L.Q == 27) {
+      double s2a = 0.0, s2b = 0.0;
+      for (int k = 0; k < g.nunk; ++k) {
+        const int q = g.unk[k];
+        s2a += L.w[q] * (double)L.c[q][t1] * (double)L.c[q][t1];
+        s2b += L.w[q] * (double)L.c[q][t2] * (double)L.c[q][t2];
+      }
+      const T s2_t1 = T(s2a), s2_t2 = T(s2b), six = T(6.0);
+      for_face_cells(g, [&](const long* pos) {
+        const size_t i = pidx(pos[0], pos[1], pos[2]);
+        if (solid[i]) return;
+        T f0[3];
+        force_at(cidx(pos[0], pos[1], pos[2]), f0);
+        T s0 = T(0), sneg = T(0), q0a = T(0), q0b = T(0);
+        for (int q = 0; q < L.Q; ++q) {
+          const int dot = L.c[q][a] * g.nin[a];
+          const T fq = f[q * np + i];
+          if (dot == 0) {
+            s0 = s0 + fq;
+            q0a = q0a + T(L.c[q][t1]) * fq;
+            q0b = q0b + T(L.c[q][t2]) * fq;
+          } else if (dot < 0) {
+            sneg = sneg + fq;
+          }
+        }
+        const T closure = s0 + two * sneg + one;
+        T r, vn, vt1, vt2;
+        zh_solve(vel, ub, rho_bc, closure, f0, a, t1, t2, nsign, r, vn, vt1, vt2);
+        const T c_t1 = r * vt1 * (one - six * s2_t1) - q0a;
+        const T c_t2 = r * vt2 * (one - six * s2_t2) - q0b;
+        for (int k = 0; k < g.nunk; ++k) {
+          const int q = g.unk[k];
+          const T cdotu = T(L.c[q][a]) * nsign * vn + T(L.c[q][t1]) * vt1 + T(L.c[q][t2]) * vt2;
+          const T w = T(L.w[q]);
+          const T delta = c_t1 * w * T(L.c[q][t1]) / s2_t1 + c_t2 * w * T(L.c[q][t2]) / s2_t2;
+          f[q * np + i] = f[L.opp[q] * np + i] + six * w * r * cdotu + delta;
+        }
+      });
+      return;
+    }
+    int e1[3] = {0, 0, 0}, e2[3] = {0, 0, 0};
+    e1[t1] = 1;
+    e2[t2] = 1;
+    const int* n = g.nin;
+    const int q_n = dir_of(n[0], n[1], n[2]);
+    const int q_p1 = dir_of(n[0] + e1[0], n[1] + e1[1], n[2] + e1[2]);
+    const int q_m1 = dir_of(n[0] - e1[0], n[1] - e1[1], n[2] - e1[2]);
+    const int q_p2 = dir_of(n[0] + e2[0], n[1] + e2[1], n[2] + e2[2]);
+    const int q_m2 = dir_of(n[0] - e2[0], n[1] - e2[1], n[2] - e2[2]);
+    const int q_t1 = dir_of(e1[0], e1[1], e1[2]), q_mt1 = dir_of(-e1[0], -e1[1], -e1[2]);
+    const int q_t2 = dir_of(e2[0], e2[1], e2[2]), q_mt2 = dir_of(-e2[0], -e2[1], -e2[2]);
+    const int q_pp = dir_of(e1[0] + e2[0], e1[1] + e2[1], e1[2] + e2[2]);
+    const int q_pm = dir_of(e1[0] - e2[0], e1[1] - e2[1], e1[2] - e2[2]);
+    const int q_mp = dir_of(-e1[0] + e2[0], -e1[1] + e2[1], -e1[2] + e2[2]);
+    const int q_mm = dir_of(-e1[0] - e2[0], -e1[1] - e2[1], -e1[2] - e2[2]);
+    const T c13 = T(1.0 / 3.0), c16 = T(1.0 / 6.0);
+    for_face_cells(g, [&](const long* pos) {
+      const size_t i = pidx(pos[0], pos[1], pos[2]);
+      if (solid[i]) return;
+      T f0[3];
+      force_at(cidx(pos[0], pos[1], pos[2]), f0);
+      auto F = [&](int q) { return f[q * np + i]; };
+      const T s0 = F(0) + F(q_t1) + F(q_mt1) + F(q_t2) + F(q_mt2) + F(q_pp) + F(q_pm) + F(q_mp) + F(q_mm);
+      const T sneg = F(L.opp[q_n]) + F(L.opp[q_p1]) + F(L.opp[q_m1]) + F(L.opp[q_p2]) + F(L.opp[q_m2]);
+      const T closure = s0 + two * sneg + one;
+      T r, vn, vt1, vt2;
+      zh_solve(vel, ub, rho_bc, closure, f0, a, t1, t2, nsign, r, vn, vt1, vt2);
+      const T qt1 = F(q_t1) - F(q_mt1) + F(q_pp) + F(q_pm) - F(q_mp) - F(q_mm);
+      const T qt2 = F(q_t2) - F(q_mt2) + F(q_pp) - F(q_pm) + F(q_mp) - F(q_mm);
+      const T n1 = c13 * r * vt1 - half * qt1;
+      const T n2 = c13 * r * vt2 - half * qt2;
+      f[q_n * np + i] = F(L.opp[q_n]) + c13 * r * vn;
+      f[q_p1 * np + i] = F(L.opp[q_p1]) + c16 * r * (vn + vt1) + n1;
+      f[q_m1 * np + i] = F(L.opp[q_m1]) + c16 * r * (vn - vt1) - n1;
+      f[q_p2 * np + i] = F(L.opp[q_p2]) + c16 * r * (vn + vt2) + n2;
+      f[q_m2 * np + i] = F(L.opp[q_m2]) + c16 * r * (vn - vt2) - n2;
+    });
+  }
+  // density and raw (pre-force) velocity of a Zou-He face cell
+  // (kernels.rs:1154-1197 / 1017-1060, g = 0)
+  void zh_solve(bool vel, const T* u, T rho_bc, T closure, const T f0[3], int a, int t1, int t2, T nsign, T& r,
+                T& vn, T& vt1, T& vt2) const {
+    const T one = T(1), half = T(0.5);
+    if (f0[0] == T(0) && f0[1] == T(0) && f0[2] == T(0)) {
+      if (vel) {
+        const T un = u[a] * nsign;
+        r = closure / (one - un);
+        vn = un;
+        vt1 = u[t1];
+        vt2 = u[t2];
+      } else {
+        r = rho_bc;
+        vn = one - closure / rho_bc;
+        vt1 = T(0);
+        vt2 = T(0);
+      }
+      return;
+    }
+    const T f0n = f0[a] * nsign, gn = T(0);
+    if (vel) {
+      const T un_phys = u[a] * nsign;
+      r = (closure - half * f0n) / (one - un_phys + half * gn);
+      const T inv_r = one / r;
+      vn = un_phys - half * (f0n * inv_r + gn);
+      vt1 = u[t1] - half * (f0[t1] * inv_r + T(0));
+      vt2 = u[t2] - half * (f0[t2] * inv_r + T(0));
+    } else {
+      const T inv_r = one / rho_bc;
+      r = rho_bc;
+      vn = one - closure * inv_r;
+      vt1 = -half * (f0[t1] * inv_r + T(0));
+      vt2 = -half * (f0[t2] * inv_r + T(0));
+    }
+  }
+  // outflow_face_selected, kernels.rs:1221-1248
+  void outflow_face(int face) {
+    const FaceGeo g = face_geo(face);
+    for_face_cells(g, [&](const long* pos) {
+      const size_t i = pidx(pos[0], pos[1], pos[2]);
+      const size_t j = pidx(pos[0] + g.nin[0], pos[1] + g.nin[1], pos[2] + g.nin[2]);
+      if (solid[i] || solid[j]) return;
+      for (int k = 0; k < g.nunk; ++k) f[g.unk[k] * np + i] = f[g.unk[k] * np + j];
+    });
+  }
+  // convective_face_selected, kernels.rs:1254-1299
+  void convective_face(int face) {
+    const FaceGeo g = face_geo(face);
+    const T lam = face_uconv[face];
+    const T inv = T(1) / (T(1) + lam);
+    double ws = 0.0;
+    for (int k = 0; k < g.nunk; ++k) ws += L.w[g.unk[k]];
+    const T wsum = T(ws);
+    for_face_cells(g, [&](const long* pos) {
+      const size_t i = pidx(pos[0], pos[1], pos[2]);
+      const size_t j = pidx(pos[0] + g.nin[0], pos[1] + g.nin[1], pos[2] + g.nin[2]);
+      if (solid[i] || solid[j]) return;
+      for (int k = 0; k < g.nunk; ++k) {
+        const int q = g.unk[k];
+        const T prev = f[q * np + i];
+        f[q * np + i] = (prev + lam * f[q * np + j]) * inv;
+      }
+      T di = T(0), dj = T(0);
+      for (int q = 0; q < L.Q; ++q) {
+        di = di + f[q * np + i];
+        dj = dj + f[q * np + j];
+      }
+      const T corr = dj - di;
+      for (int k = 0; k < g.nunk; ++k) {
+        const int q = g.unk[k];
+        f[q * np + i] = f[q * np + i] + corr * T(L.w[q]) / wsum;
+      }
+    });
+  }
+  // apply_open_faces_impl, backend.rs:602-705 (no face patches or inlet
+  // profiles on this path): the faces in Face::ALL order
+  void apply_open_faces() {
+    for (int face = 0; face < 6; ++face) {
+      const int k = face_kind[face];
+      if (k == 1 || k == 2)
+        zou_he_face(face);
+      else if (k == 3)
+        outflow_face(face);
+      else if (k == 4)
+        convective_face(face);
+    }
+  }
+  bool any_open() const {
+    for (int k : face_kind)
+      if (k) return true;
+    return false;
+  }
+
   // Backend::run_span default body, backend.rs:266-328 (single part,
-  // single pass): collide -> exchange_f -> stream -> swap -> moments.
+  // single pass): collide -> exchange_f -> stream -> swap -> open faces ->
+  // moments.  Sources beyond a non-periodic face are skipped by the stream,
+  // so those slots keep the other buffer's previous post-collide values.
   void hydro_step() {
     collide();
     wrap_halo(f, L.Q, 1);
     stream();
     std::swap(f, ftmp);
+    if (any_open()) apply_open_faces();
     update_moments();
   }
 
@@ -1528,6 +1767,26 @@ void orc_set_omega_field(void* p, const double* om) {
       for (size_t c = 0; c < S->nc; ++c) S->omega_field[c] = (TT)om[c];
     }
   });
+}
+
+// Open face `face` (0..5 = XNeg..ZPos): kind 0 closed, 1 velocity (v = u),
+// 2 pressure (v[0] = rho), 3 outflow, 4 convective (v[0] = u_conv).  Returns
+// -1 for an open face on a periodic axis (GlobalSpec::validate).
+int orc_set_face(void* p, int face, int kind, const double* v) {
+  Handle* h = (Handle*)p;
+  int rc = 0;
+  DISPATCH(h, {
+    using TT = std::remove_reference_t<decltype(S->f[0])>;
+    if (face < 0 || face > 5 || kind < 0 || kind > 4 || (kind && S->periodic[face / 2])) {
+      rc = -1;
+    } else {
+      S->face_kind[face] = kind;
+      for (int a = 0; a < 3; ++a) S->face_u[face][a] = kind == 1 ? (TT)v[a] : TT(0);
+      S->face_rho[face] = kind == 2 ? (TT)v[0] : TT(0);
+      S->face_uconv[face] = kind == 4 ? (TT)v[0] : TT(0);
+    }
+  });
+  return rc;
 }
 
 // Strain rate (N x 6) and velocity gradient (N x 9, row-major du_i/dx_j) of

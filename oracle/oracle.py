@@ -23,6 +23,9 @@ _lib = None
 BGK, TRT, CM = 0, 1, 2
 MAGIC_STD = 3.0 / 16.0
 WALE_CW = 0.325  # les.rs:16
+# Face::ALL order (lattice.rs:38-45) and FaceBC kinds (params.rs:232-252)
+XNEG, XPOS, YNEG, YPOS, ZNEG, ZPOS = range(6)
+FACE_CLOSED, FACE_VELOCITY, FACE_PRESSURE, FACE_OUTFLOW, FACE_CONVECTIVE = range(5)
 
 
 def build():
@@ -62,6 +65,7 @@ def lib():
         L.orc_get_force_field.argtypes = [vp, d]
         L.orc_set_omega_field.argtypes = [vp, d]
         L.orc_strain.argtypes = [vp, d]
+        L.orc_set_face.argtypes = [vp, C.c_int, C.c_int, d]
         L.orc_vgrad.argtypes = [vp, d]
         L.orc_wale.restype = C.c_long
         L.orc_wale.argtypes = [vp, C.c_double, C.c_double, d, d, d]
@@ -215,6 +219,15 @@ class OracleSolver:
         else:
             om = np.ascontiguousarray(om, np.float64)
             lib().orc_set_omega_field(self.h, _d(om))
+
+    def set_face(self, face, kind, value=(0.0, 0.0, 0.0)):
+        """Open-face BC of ``face``: value = u (velocity), (rho,) (pressure),
+        (u_conv,) (convective)."""
+        v = np.zeros(3)
+        vv = np.atleast_1d(np.asarray(value, np.float64))
+        v[:vv.size] = vv
+        if lib().orc_set_face(self.h, face, kind, _d(v)) != 0:
+            raise ValueError("open face on a periodic axis or bad face/kind")
 
     def strain(self):
         """Strain rate per cell, [Sxx, Syy, Szz, Sxy, Sxz, Syz] (solver.rs:5393-5447)."""

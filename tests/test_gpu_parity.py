@@ -55,7 +55,7 @@ def links_from_solid(solid, dims, Q, periodic=(0, 0, 0)):
 def test_context_and_capabilities(gpu_ctx):
     caps = gpu_ctx.capabilities()
     assert caps & 0x1 and caps & 0x2 and caps & 0x40 and caps & 0x80
-    assert not caps & 0x200  # open faces are not on this path
+    assert caps & 0x200 and caps & 0x2000  # open faces, WALE
 
 
 @pytest.mark.parametrize("Q", [19, 27])
