@@ -23,6 +23,8 @@
 //            which is the reference's post-swap state of the skipped slots;
 //   BC       (one launch per face, Face::ALL order): the face's rule, on
 //            the logical populations read and written in place.
+// When every open face lies on one axis the faces share no cell and no
+// thread reads what another writes, so one launch does all three per cell.
 // The stash is double-buffered and starts at zero (the reference's ftmp).
 // Face kernels are templated on the face so every lattice direction is a
 // compile-time constant (no indexed register arrays).
@@ -218,10 +220,14 @@ __device__ __forceinline__ void zh_solve(int kind, const float* u, float rho_bc,
   }
 }
 
-template <int Q, typename S, int FACE>
+// FUSED: capture and restore of this cell in the same thread, then the rule
+// (legal when every open face lies on one axis: the faces then share no
+// cell, and no thread reads a population another thread writes)
+template <int Q, typename S, int FACE, bool FUSED>
 __device__ __forceinline__ void bc_cell(const FaceArgs& a, uint32_t coord) {
   using F = FaceC<FACE>;
   constexpr int ax = F::a, t1 = F::t1, t2 = F::t2;
+  if constexpr (FUSED) capture_cell<Q, S, FACE>(a, coord);
   uint32_t x, y, z;
   const uint32_t i = face_cell<FACE>(a.d, coord, x, y, z);
   const uint32_t m = a.mask[i];
@@ -230,9 +236,17 @@ __device__ __forceinline__ void bc_cell(const FaceArgs& a, uint32_t coord) {
   const Nbr nb(a.d, x, y, z);
   S* f = (S*)a.f;
   float fv[Q];
+  const uint32_t nf = face_cells<FACE>(a.d);
   sfor<0, Q>([&](auto qc) {
     constexpr int q = decltype(qc)::value;
-    fv[q] = read_pop<Q, S, q>(f, a.d.ps, nb, i, m, odd);
+    if constexpr (FUSED && F::template dot<q>() > 0) {
+      // restore: the stale slot takes the previous post-collide value
+      const float v = a.stash_cur[FACE][(size_t)unk_index<Q, FACE, q>() * nf + coord];
+      write_pop<Q, S, q>(f, a.d.ps, nb, i, m, odd, v);
+      fv[q] = v;
+    } else {
+      fv[q] = read_pop<Q, S, q>(f, a.d.ps, nb, i, m, odd);
+    }
   });
   const int kind = a.kind[FACE];
   if (kind == kFaceVelocity || kind == kFacePressure) {
@@ -357,13 +371,16 @@ __device__ __forceinline__ void bc_cell(const FaceArgs& a, uint32_t coord) {
   });
 }
 
-template <int Q, typename S>
+// FUSED: every face of a.faces at once (blockIdx.y = slot); otherwise the
+// one face `face`
+template <int Q, typename S, bool FUSED>
 __global__ __launch_bounds__(256) void k_face_bc(FaceArgs a, int face) {
   const uint32_t coord = blockIdx.x * blockDim.x + threadIdx.x;
+  if constexpr (FUSED) face = a.faces[blockIdx.y];
   switch (face) {
-#define LBM_FACE_CASE(F)                                             \
-  case F:                                                            \
-    if (coord < face_cells<F>(a.d)) bc_cell<Q, S, F>(a, coord);      \
+#define LBM_FACE_CASE(F)                                                  \
+  case F:                                                                 \
+    if (coord < face_cells<F>(a.d)) bc_cell<Q, S, F, FUSED>(a, coord);    \
     break;
     LBM_FACE_CASE(0)
     LBM_FACE_CASE(1)
@@ -402,6 +419,15 @@ template <typename S>
 static hipError_t open_s(int Q, const FaceArgs& a, hipStream_t s) {
   if (a.nfaces == 0) return hipSuccess;
   const dim3 block(256), grid((max_face_cells(a.d) + 255) / 256, (unsigned)a.nfaces);
+  bool one_axis = true;
+  for (int k = 1; k < a.nfaces; ++k) one_axis = one_axis && a.faces[k] / 2 == a.faces[0] / 2;
+  if (one_axis) {
+    if (Q == 19)
+      hipLaunchKernelGGL((k_face_bc<19, S, true>), grid, block, 0, s, a, 0);
+    else
+      hipLaunchKernelGGL((k_face_bc<27, S, true>), grid, block, 0, s, a, 0);
+    return hipGetLastError();
+  }
   if (Q == 19) {
     hipLaunchKernelGGL((k_face_stash<19, S, false>), grid, block, 0, s, a);
     hipLaunchKernelGGL((k_face_stash<19, S, true>), grid, block, 0, s, a);
@@ -413,9 +439,9 @@ static hipError_t open_s(int Q, const FaceArgs& a, hipStream_t s) {
     const int face = a.faces[k];
     const dim3 g1((max_face_cells(a.d) + 255) / 256);
     if (Q == 19)
-      hipLaunchKernelGGL((k_face_bc<19, S>), g1, block, 0, s, a, face);
+      hipLaunchKernelGGL((k_face_bc<19, S, false>), g1, block, 0, s, a, face);
     else
-      hipLaunchKernelGGL((k_face_bc<27, S>), g1, block, 0, s, a, face);
+      hipLaunchKernelGGL((k_face_bc<27, S, false>), g1, block, 0, s, a, face);
   }
   return hipGetLastError();
 }
