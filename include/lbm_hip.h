@@ -73,7 +73,7 @@ extern "C" {
 #define LBM_CAP_OPEN_FACES 0x200u    /* Zou-He velocity/pressure, outflow, convective faces */
 #define LBM_CAP_TWO_PASS 0x400u      /* not set: one fused pass per step */
 #define LBM_CAP_GRAVITY 0x800u       /* not set */
-#define LBM_CAP_PROBE 0x1000u        /* not set: momentum-exchange probe */
+#define LBM_CAP_PROBE 0x1000u        /* momentum-exchange force probe */
 #define LBM_CAP_WALE 0x2000u         /* WALE omega_plus producer (les.rs) */
 
 typedef struct lbm_ctx lbm_ctx;
@@ -172,6 +172,17 @@ int lbm_set_omega_field(lbm_fields* fl, const float* omega);
  * non-periodic face that no open face covers, and scalar distributions
  * together with open faces (LBM_ERR_UNSUPPORTED). */
 int lbm_set_face_bc(lbm_fields* fl, int face, int kind, const double value[3]);
+
+/* Solver::set_force_probe (solver.rs:2741-2760): `mask` marks the solid
+ * cells whose momentum-exchange force is accumulated every step (N bytes,
+ * nonzero = probed; NULL removes the probe). */
+int lbm_set_force_probe(lbm_fields* fl, const uint8_t* mask);
+/* Solver::probed_force / read_probed_force (solver.rs:5065-5079): force on
+ * the probed solids during the most recent step, -sum c_q (f_out + f_in +
+ * 2 w_q) over their half-way links with the reference's row-ordered fp32
+ * sum (stream_row kernels.rs:605-615, backend.rs:521-544).  Zero without a
+ * probe.  Bit-exact for probed walls at rest. */
+int lbm_probed_force(lbm_fields* fl, double out[3]);
 
 /* WALE large-eddy closure as an omega_plus producer.
  * WaleLesDiagnostics, les.rs:18-31 (tau_eff_max / nu_t_max are 0 when the

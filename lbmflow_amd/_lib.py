@@ -83,6 +83,8 @@ SIGNATURES = {
     "lbm_wale_update": (_I, [_P, _D, _D, _pF, C.POINTER(WaleDiag)]),
     "lbm_set_wale": (_I, [_P, _I, _D, _D]),
     "lbm_set_face_bc": (_I, [_P, _I, _I, _pD]),
+    "lbm_set_force_probe": (_I, [_P, C.POINTER(C.c_uint8)]),
+    "lbm_probed_force": (_I, [_P, _pD]),
     "lbm_gather_omega_field": (_I, [_P, _pF, _pI]),
     "lbm_stage_in": (_I, [_P, _pF]),
     "lbm_stage_out": (_I, [_P, _pF]),

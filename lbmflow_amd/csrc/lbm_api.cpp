@@ -21,6 +21,7 @@
 #include "../../include/lbm_hip.h"
 #include "lbm_aux.hpp"
 #include "lbm_internal.hpp"
+#include "lbm_lattice.hpp"
 
 using namespace lbm;
 
@@ -98,6 +99,12 @@ struct lbm_fields {
   int stash_sel = 0;
   unsigned int uncovered = 0;
   int g_stash = -1;
+  // force probe: the probed-solid mask (compact, empty = none) and its link
+  // list on the device, rebuilt after a mask change
+  std::vector<uint8_t> probe_host;
+  bool probe_built = false;
+  ProbeArgs pr{};
+  std::vector<std::pair<void*, size_t>> pr_allocs;
   // WALE LES: cached-moment scratch (4 x N), diagnostics words, and the
   // per-step mode's coefficients
   float* wale_mom = nullptr;
@@ -326,6 +333,7 @@ int rebuild_mask(lbm_fields* fl) {
   // Fluid on a non-periodic face needs an open-face BC, which is not on this
   // path; stepping such a domain is refused (lbm_run_span).
   fl->open_face_cells = nerr;
+  fl->probe_built = false;
   int rc2 = count_uncovered(fl);
   if (rc2) return rc2;
   return rebuild_tiles(fl);
@@ -448,7 +456,7 @@ int lbm_stream_copy(lbm_ctx* ctx, size_t bytes, int iterations, double* gbps) {
 int lbm_capabilities(const lbm_ctx* ctx, uint32_t* flags) {
   if (!ctx || !flags) return LBM_ERR_INVALID;
   *flags = LBM_CAP_TRT | LBM_CAP_CENTRAL_MOMENT | LBM_CAP_F16_STORAGE | LBM_CAP_BF16_STORAGE | LBM_CAP_FORCE_FIELD |
-           LBM_CAP_OMEGA_FIELD | LBM_CAP_ROTATING_IBM | LBM_CAP_SCALAR | LBM_CAP_PERIODIC | LBM_CAP_WALE | LBM_CAP_OPEN_FACES;
+           LBM_CAP_OMEGA_FIELD | LBM_CAP_ROTATING_IBM | LBM_CAP_SCALAR | LBM_CAP_PERIODIC | LBM_CAP_WALE | LBM_CAP_OPEN_FACES | LBM_CAP_PROBE;
   return LBM_OK;
 }
 
@@ -543,6 +551,7 @@ void lbm_fields_free(lbm_fields* fl) {
   for (int k = 0; k < 6; ++k)
     for (int g = 0; g < 2; ++g)
       if (fl->stash[k][g]) (void)hipFree(fl->stash[k][g]);
+  for (auto& pa : fl->pr_allocs) (void)hipFree(pa.first);
   delete fl;
 }
 
@@ -865,6 +874,137 @@ int lbm_set_wale(lbm_fields* fl, int enabled, double cw, double tau_eff_max) {
   return LBM_OK;
 }
 
+// ---- force probe -------------------------------------------------------------
+static void probe_release(lbm_fields* fl) {
+  for (auto& pa : fl->pr_allocs) dev_free(fl, pa.first, pa.second);
+  fl->pr_allocs.clear();
+  fl->pr = ProbeArgs{};
+  fl->probe_built = false;
+}
+
+extern "C++" template <typename T>
+static int probe_upload(lbm_fields* fl, T** dst, const std::vector<T>& v) {
+  const size_t bytes = std::max<size_t>(v.size(), 1) * sizeof(T);
+  int rc = dev_alloc(fl, (void**)dst, bytes);
+  if (rc) return rc;
+  fl->pr_allocs.push_back({(void*)*dst, bytes});
+  if (!v.empty())
+    HIPCHK(fl->ctx, hipMemcpyAsync(*dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, fl->ctx->stream));
+  return LBM_OK;
+}
+
+// Links (i, q): fluid i, probed solid i - c_q, sources beyond a non-periodic
+// face skipped as the stream skips them (kernels.rs:572-590); ordered by
+// (row, x, q), the reference's per-row summation order.
+static int probe_build(lbm_fields* fl) {
+  probe_release(fl);
+  const Dom& d = fl->d;
+  const size_t n = d.N;
+  std::vector<uint8_t> solid(n);
+  HIPCHK(fl->ctx, hipMemcpyAsync(solid.data(), fl->solid, n, hipMemcpyDeviceToHost, fl->ctx->stream));
+  HIPCHK(fl->ctx, hipStreamSynchronize(fl->ctx->stream));
+  std::vector<uint64_t> keys;  // (cell << 8) | q, cell order = (row, x)
+  const long ext[3] = {(long)d.nx, (long)d.ny, (long)d.nz};
+  for (size_t sidx = 0; sidx < n; ++sidx) {
+    if (!fl->probe_host[sidx] || !solid[sidx]) continue;
+    const long sx = (long)(sidx % d.nx), t = (long)(sidx / d.nx);
+    const long sy = t % (long)d.ny, sz = t / (long)d.ny;
+    for (int q = 1; q < fl->Q; ++q) {
+      long p[3] = {sx + lat_c(q, 0), sy + lat_c(q, 1), sz + lat_c(q, 2)};
+      bool ok = true;
+      for (int a = 0; a < 3 && ok; ++a)
+        if (p[a] < 0 || p[a] >= ext[a]) {
+          if (d.per[a])
+            p[a] = (p[a] + ext[a]) % ext[a];
+          else
+            ok = false;
+        }
+      if (!ok) continue;
+      const size_t i = ((size_t)p[2] * d.ny + p[1]) * d.nx + p[0];
+      if (solid[i]) continue;
+      keys.push_back(((uint64_t)i << 8) | (uint64_t)q);
+    }
+  }
+  std::sort(keys.begin(), keys.end());
+  std::vector<uint32_t> cells, lcell, row_start;
+  std::vector<uint8_t> lq;
+  uint32_t last_row = UINT32_MAX;
+  for (uint64_t k : keys) {
+    const uint32_t i = (uint32_t)(k >> 8);
+    if (cells.empty() || cells.back() != i) cells.push_back(i);
+    const uint32_t row = i / d.nx;
+    if (row != last_row) {
+      row_start.push_back((uint32_t)lcell.size());
+      last_row = row;
+    }
+    lcell.push_back((uint32_t)cells.size() - 1);
+    lq.push_back((uint8_t)(k & 0xff));
+  }
+  const uint32_t nrows = (uint32_t)row_start.size();
+  row_start.push_back((uint32_t)lcell.size());
+  ProbeArgs a{};
+  uint32_t *dc = nullptr, *dl = nullptr, *dr = nullptr;
+  uint8_t* dq = nullptr;
+  float *drho = nullptr, *dpart = nullptr, *dout = nullptr;
+  int rc = probe_upload(fl, &dc, cells);
+  if (!rc) rc = probe_upload(fl, &dl, lcell);
+  if (!rc) rc = probe_upload(fl, &dq, lq);
+  if (!rc) rc = probe_upload(fl, &dr, row_start);
+  if (!rc) rc = probe_upload(fl, &drho, std::vector<float>(cells.size(), 0.0f));
+  if (!rc) rc = probe_upload(fl, &dpart, std::vector<float>(3 * (size_t)nrows, 0.0f));
+  if (!rc) rc = probe_upload(fl, &dout, std::vector<float>(3, 0.0f));
+  if (rc) return rc;
+  HIPCHK(fl->ctx, hipStreamSynchronize(fl->ctx->stream));
+  a.cells = dc;
+  a.ncells = (uint32_t)cells.size();
+  a.rho = drho;
+  a.lcell = dl;
+  a.lq = dq;
+  a.row_start = dr;
+  a.nrows = nrows;
+  a.partial = dpart;
+  a.out = dout;
+  fl->pr = a;
+  fl->probe_built = true;
+  fl->gen++;
+  return LBM_OK;
+}
+
+int lbm_set_force_probe(lbm_fields* fl, const uint8_t* mask) {
+  if (!fl) return LBM_ERR_INVALID;
+  HIPCHK(fl->ctx, hipStreamSynchronize(fl->ctx->stream));
+  probe_release(fl);
+  fl->gen++;
+  if (!mask) {
+    fl->probe_host.clear();
+    return LBM_OK;
+  }
+  fl->probe_host.assign(mask, mask + fl->d.N);
+  return probe_build(fl);
+}
+
+int lbm_probed_force(lbm_fields* fl, double out[3]) {
+  if (!fl || !out) return LBM_ERR_INVALID;
+  out[0] = out[1] = out[2] = 0.0;
+  if (fl->probe_host.empty() || !fl->pr.out) return LBM_OK;
+  float v[3];
+  HIPCHK(fl->ctx, hipMemcpyAsync(v, fl->pr.out, sizeof v, hipMemcpyDeviceToHost, fl->ctx->stream));
+  HIPCHK(fl->ctx, hipStreamSynchronize(fl->ctx->stream));
+  for (int c = 0; c < 3; ++c) out[c] = v[c];
+  return LBM_OK;
+}
+
+static int probe_launch(lbm_fields* fl, int post) {
+  ProbeArgs a = fl->pr;
+  a.d = fl->d;
+  a.f = fl->f;
+  a.mask = fl->mask;
+  a.odd = post ? fl->parity ^ 1 : fl->parity;
+  a.wall_u = fl->wall_u;
+  HIPCHK(fl->ctx, launch_probe(fl->Q, fl->storage, post, a, fl->ctx->stream));
+  return LBM_OK;
+}
+
 static int check_steppable(lbm_fields* fl) {
   if (fl->uncovered)
     return fail(fl->ctx, LBM_ERR_UNSUPPORTED,
@@ -873,6 +1013,7 @@ static int check_steppable(lbm_fields* fl) {
                 fl->uncovered);
   if (fl->open_face_cells && !fl->scalars.empty())
     return fail(fl->ctx, LBM_ERR_UNSUPPORTED, "scalar distributions do not take open faces on this path");
+  if (!fl->probe_host.empty() && !fl->probe_built) return probe_build(fl);
   return LBM_OK;
 }
 
@@ -949,6 +1090,10 @@ static int hydro_step(lbm_fields* fl) {
     int rc = wale_enqueue(fl, fl->wale_cw2, fl->wale_nut_max, nullptr, nullptr);
     if (rc) return rc;
   }
+  if (fl->probe_built) {
+    int rc = probe_launch(fl, 0);
+    if (rc) return rc;
+  }
   HydroArgs a = hydro_args(fl);
   if (fl->timing) {
     if (fl->ev_used + 2 > fl->ev_pool.size()) {
@@ -966,6 +1111,10 @@ static int hydro_step(lbm_fields* fl) {
     fl->ev_used += 2;
   }
   if (a.nlist) HIPCHK(fl->ctx, launch_hydro_list(fl->Q, fl->storage, fl->collision, fl->parity, a, s));
+  if (fl->probe_built) {  // before the open faces, as the stream accumulates it
+    int rc = probe_launch(fl, 1);
+    if (rc) return rc;
+  }
   if (fl->open_face_cells) {
     int rc = open_faces_step(fl);
     if (rc) return rc;
@@ -1483,6 +1632,8 @@ int lbm_batch_create(lbm_ctx* ctx, lbm_fields* const* members, int n, lbm_batch*
     if (!fl->scalars.empty())
       return fail(ctx, LBM_ERR_UNSUPPORTED, "batch member %d carries scalar distributions", k);
     if (fl->parity != f0->parity) return fail(ctx, LBM_ERR_STATE, "batch members must be at the same step parity");
+    if (!fl->probe_host.empty())
+      return fail(ctx, LBM_ERR_UNSUPPORTED, "batch member %d has a force probe", k);
     if (fl->hp.force_on || fl->omega)
       return fail(ctx, LBM_ERR_UNSUPPORTED, "batch member %d has a body force or an omega field", k);
     if (fl->field_present && fl->field_uniform && !fl->ibm)

@@ -87,6 +87,25 @@ struct FaceArgs {
   float* stash_next[6];  // this step's, captured before the restore
 };
 
+// Force probe (lbm_probe.hip): links (i, q) with i - c_q a probed solid,
+// grouped by row in (row, x, q) order; cells = the distinct fluid cells.
+struct ProbeArgs {
+  Dom d;
+  const void* f;
+  const uint32_t* mask;
+  int odd;
+  const float* wall_u;  // N x 3 or null
+  const uint32_t* cells;
+  uint32_t ncells;
+  float* rho;  // ncells: pre-step density
+  const uint32_t* lcell;
+  const uint8_t* lq;
+  const uint32_t* row_start;  // nrows + 1
+  uint32_t nrows;
+  float* partial;  // 3 x nrows
+  float* out;      // 3: the step's probed force
+};
+
 // Host-side launchers (lbm_kernels.hip).  All enqueue on `s`.
 hipError_t launch_hydro(int Q, int storage, int collision, int odd, const HydroArgs& a, hipStream_t s);
 hipError_t launch_hydro_list(int Q, int storage, int collision, int odd, const HydroArgs& a, hipStream_t s);
@@ -103,6 +122,8 @@ hipError_t launch_adj_collide_stream(int storage, int odd, const Dom& d, const v
                                      const HydroParams& p, const double* lam_in, double* lam_out, double* omega_bar,
                                      double* Fbar, double* Gout, hipStream_t s);
 hipError_t launch_fill_f64(double* p, size_t n, double v, hipStream_t s);
+// probe passes (lbm_probe.hip): post = 0 before the step (densities), 1 after
+hipError_t launch_probe(int Q, int storage, int post, const ProbeArgs& a, hipStream_t s);
 // capture, restore and the per-face rules after a step (lbm_open.hip)
 hipError_t launch_open_faces(int Q, int storage, const FaceArgs& a, hipStream_t s);
 hipError_t launch_uncovered(const Dom& d, const uint8_t* solid, uint32_t open_bits, unsigned int* count,

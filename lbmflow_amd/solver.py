@@ -228,6 +228,23 @@ class Solver:
             om = np.ascontiguousarray(omega, np.float32).reshape(-1)
             self._chk(self.lib.lbm_set_omega_field(self.h, _fp(om)))
 
+    def set_force_probe(self, mask):
+        """``Solver::set_force_probe`` (solver.rs:2741-2760): solids whose
+        momentum-exchange force is accumulated each step (None removes)."""
+        if mask is None:
+            self._chk(self.lib.lbm_set_force_probe(self.h, None))
+            return
+        m = np.ascontiguousarray(mask, np.uint8).reshape(-1)
+        if m.size != self.n:
+            raise ValueError("probe mask has the wrong size")
+        self._chk(self.lib.lbm_set_force_probe(self.h, _u8p(m)))
+
+    def probed_force(self):
+        """``Solver::probed_force`` (solver.rs:5065-5071): the most recent step's."""
+        out = np.zeros(3)
+        self._chk(self.lib.lbm_probed_force(self.h, _dp(out)))
+        return out
+
     FACES = {"x-": 0, "x+": 1, "y-": 2, "y+": 3, "z-": 4, "z+": 5}
     FACE_KINDS = {"closed": 0, "velocity": 1, "pressure": 2, "outflow": 3, "convective": 4}
 

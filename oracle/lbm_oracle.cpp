@@ -207,6 +207,8 @@ struct Solver {
   std::vector<T> rho, ux, uy, uz;
   std::vector<uint8_t> solid;  // padded
   std::vector<T> wall_u;       // padded x3
+  std::vector<uint8_t> probe;  // padded, empty = no force probe
+  T probed[3] = {T(0), T(0), T(0)};
   bool has_ff = false;
   std::vector<T> ff;  // compact x3
   bool has_omega = false;
@@ -537,11 +539,14 @@ struct Solver {
   // stream_row, kernels.rs:547-622 (no probe: the momentum-exchange probe
   // is not on this path).
   void stream() {
-    const T six = T(6.0);
+    const T six = T(6.0), two = T(2.0);
     const int Q = L.Q;
+    const bool prb = !probe.empty();
+    std::vector<T> part(prb ? 3 * (size_t)ny * nz : 0, T(0));
 #pragma omp parallel for schedule(static)
     for (long r = 0; r < (long)ny * nz; ++r) {
       long y = r % ny, z = r / ny;
+      T pf[3] = {T(0), T(0), T(0)};
       for (long x = 0; x < nx; ++x) {
         size_t i = pidx(x, y, z);
         if (solid[i]) continue;
@@ -566,12 +571,38 @@ struct Solver {
             for (int d = 1; d < 3; ++d) cu = cu + cr[q][d] * wu[d];
             T fin = fout + six * wr[q] * rho_x * cu;
             ftmp[q * np + i] = fin;
+            if (prb && probe[si]) {  // momentum given to the wall (stream_row, kernels.rs:605-615)
+              const T ftot = fout + fin + two * wr[q];
+              pf[0] = pf[0] - cr[q][0] * ftot;
+              pf[1] = pf[1] - cr[q][1] * ftot;
+              pf[2] = pf[2] - cr[q][2] * ftot;
+            }
           } else {
             ftmp[q * np + i] = f[q * np + si];
           }
         }
       }
+      if (prb)
+        for (int a = 0; a < 3; ++a) part[3 * r + a] = pf[a];
     }
+    // deterministic total: row partials folded in row order (backend.rs:521-544)
+    probed[0] = probed[1] = probed[2] = T(0);
+    if (prb)
+      for (long r = 0; r < (long)ny * nz; ++r)
+        for (int a = 0; a < 3; ++a) probed[a] = probed[a] + part[3 * r + a];
+  }
+  // Solver::set_force_probe, solver.rs:2741-2760 (mask over global cells,
+  // exchanged into the halo like the solid mask)
+  void set_probe(const uint8_t* m) {
+    if (!m) {
+      probe.clear();
+      return;
+    }
+    probe.assign(np, 0);
+    for (long z = 0; z < nz; ++z)
+      for (long y = 0; y < ny; ++y)
+        for (long x = 0; x < nx; ++x) probe[pidx(x, y, z)] = m[cidx(x, y, z)];
+    wrap_halo(probe, 1, 1);
   }
 
   // moments_row via update_moments_impl, kernels.rs:627-668, backend.rs:858-916
@@ -1766,6 +1797,19 @@ void orc_set_omega_field(void* p, const double* om) {
       S->omega_field.resize(S->nc);
       for (size_t c = 0; c < S->nc; ++c) S->omega_field[c] = (TT)om[c];
     }
+  });
+}
+
+// Force probe over the solids marked in `mask` (compact, NULL clears) and
+// the momentum-exchange force of the most recent step.
+void orc_set_probe(void* p, const uint8_t* mask) {
+  Handle* h = (Handle*)p;
+  DISPATCH(h, S->set_probe(mask));
+}
+void orc_probed_force(void* p, double* out) {
+  Handle* h = (Handle*)p;
+  DISPATCH(h, {
+    for (int a = 0; a < 3; ++a) out[a] = (double)S->probed[a];
   });
 }
 

@@ -66,6 +66,8 @@ def lib():
         L.orc_set_omega_field.argtypes = [vp, d]
         L.orc_strain.argtypes = [vp, d]
         L.orc_set_face.argtypes = [vp, C.c_int, C.c_int, d]
+        L.orc_set_probe.argtypes = [vp, u8]
+        L.orc_probed_force.argtypes = [vp, d]
         L.orc_vgrad.argtypes = [vp, d]
         L.orc_wale.restype = C.c_long
         L.orc_wale.argtypes = [vp, C.c_double, C.c_double, d, d, d]
@@ -219,6 +221,19 @@ class OracleSolver:
         else:
             om = np.ascontiguousarray(om, np.float64)
             lib().orc_set_omega_field(self.h, _d(om))
+
+    def set_force_probe(self, mask):
+        """``Solver::set_force_probe`` (solver.rs:2741-2760) over a compact mask."""
+        if mask is None:
+            lib().orc_set_probe(self.h, None)
+        else:
+            self._probe = np.ascontiguousarray(mask, np.uint8).reshape(-1)
+            lib().orc_set_probe(self.h, _u8(self._probe))
+
+    def probed_force(self):
+        out = np.zeros(3)
+        lib().orc_probed_force(self.h, _d(out))
+        return out
 
     def set_face(self, face, kind, value=(0.0, 0.0, 0.0)):
         """Open-face BC of ``face``: value = u (velocity), (rho,) (pressure),
