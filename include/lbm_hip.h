@@ -173,6 +173,22 @@ int lbm_set_omega_field(lbm_fields* fl, const float* omega);
  * together with open faces (LBM_ERR_UNSUPPORTED). */
 int lbm_set_face_bc(lbm_fields* fl, int face, int kind, const double value[3]);
 
+/* VolumeSource / SourceKind (params.rs:262-283): after the open faces of
+ * every step, each fluid cell of the inclusive box lo..hi gains
+ * w_q q_cell (1 + 3 c.u + 4.5 (c.u)^2 - 1.5 u.u) with q_cell = q_lu / box
+ * cells (apply_volume_sources_impl, backend.rs:799-856): q_lu of mass per
+ * step, carrying u for a jet (negative q_lu: a sink).  Sources apply in
+ * order; lbm_set_volume_sources replaces the list (n = 0 clears). */
+#define LBM_SOURCE_MASS_FLOW 0
+#define LBM_SOURCE_JET 1
+typedef struct lbm_volume_source {
+  int lo[3], hi[3];
+  int kind;
+  double q_lu;
+  double u[3];
+} lbm_volume_source;
+int lbm_set_volume_sources(lbm_fields* fl, int n, const lbm_volume_source* src);
+
 /* Solver::set_force_probe (solver.rs:2741-2760): `mask` marks the solid
  * cells whose momentum-exchange force is accumulated every step (N bytes,
  * nonzero = probed; NULL removes the probe). */

@@ -47,6 +47,11 @@ class CkptInfo(C.Structure):
                 ("step", C.c_int64)]
 
 
+class VolumeSource(C.Structure):
+    _fields_ = [("lo", C.c_int * 3), ("hi", C.c_int * 3), ("kind", C.c_int), ("q_lu", C.c_double),
+                ("u", C.c_double * 3)]
+
+
 class WaleDiag(C.Structure):
     _fields_ = [("clipped_cells", C.c_int64), ("clipped_fraction", C.c_double),
                 ("max_nu_t_before_clipping", C.c_double), ("tau_eff_max", C.c_double), ("nu_t_max", C.c_double)]
@@ -84,6 +89,7 @@ SIGNATURES = {
     "lbm_set_wale": (_I, [_P, _I, _D, _D]),
     "lbm_set_face_bc": (_I, [_P, _I, _I, _pD]),
     "lbm_set_force_probe": (_I, [_P, C.POINTER(C.c_uint8)]),
+    "lbm_set_volume_sources": (_I, [_P, _I, C.POINTER(VolumeSource)]),
     "lbm_probed_force": (_I, [_P, _pD]),
     "lbm_gather_omega_field": (_I, [_P, _pF, _pI]),
     "lbm_stage_in": (_I, [_P, _pF]),

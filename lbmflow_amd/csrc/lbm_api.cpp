@@ -99,6 +99,8 @@ struct lbm_fields {
   int stash_sel = 0;
   unsigned int uncovered = 0;
   int g_stash = -1;
+  // volume sources, applied after the open faces in order
+  std::vector<SourceArgs> sources;
   // force probe: the probed-solid mask (compact, empty = none) and its link
   // list on the device, rebuilt after a mask change
   std::vector<uint8_t> probe_host;
@@ -970,6 +972,31 @@ static int probe_build(lbm_fields* fl) {
   return LBM_OK;
 }
 
+int lbm_set_volume_sources(lbm_fields* fl, int n, const lbm_volume_source* src) {
+  if (!fl || n < 0 || (n > 0 && !src)) return LBM_ERR_INVALID;
+  std::vector<SourceArgs> out;
+  const uint32_t ext[3] = {fl->d.nx, fl->d.ny, fl->d.nz};
+  for (int k = 0; k < n; ++k) {
+    const lbm_volume_source& v = src[k];
+    if (v.kind != LBM_SOURCE_MASS_FLOW && v.kind != LBM_SOURCE_JET)
+      return fail(fl->ctx, LBM_ERR_INVALID, "volume source %d: unknown kind %d", k, v.kind);
+    SourceArgs a{};
+    for (int c = 0; c < 3; ++c) {
+      if (v.lo[c] < 0 || v.hi[c] < v.lo[c] || (uint32_t)v.hi[c] >= ext[c])
+        return fail(fl->ctx, LBM_ERR_INVALID, "volume source %d: region outside the grid", k);
+      a.lo[c] = (uint32_t)v.lo[c];
+      a.hi[c] = (uint32_t)v.hi[c];
+      a.u[c] = v.kind == LBM_SOURCE_JET ? (float)v.u[c] : 0.0f;
+    }
+    if (!std::isfinite(v.q_lu)) return fail(fl->ctx, LBM_ERR_INVALID, "volume source %d: q_lu", k);
+    a.q_lu = (float)v.q_lu;
+    out.push_back(a);
+  }
+  fl->sources = out;
+  fl->gen++;
+  return LBM_OK;
+}
+
 int lbm_set_force_probe(lbm_fields* fl, const uint8_t* mask) {
   if (!fl) return LBM_ERR_INVALID;
   HIPCHK(fl->ctx, hipStreamSynchronize(fl->ctx->stream));
@@ -1118,6 +1145,14 @@ static int hydro_step(lbm_fields* fl) {
   if (fl->open_face_cells) {
     int rc = open_faces_step(fl);
     if (rc) return rc;
+  }
+  for (const SourceArgs& src : fl->sources) {
+    SourceArgs sa = src;
+    sa.d = fl->d;
+    sa.f = fl->f;
+    sa.mask = fl->mask;
+    sa.odd = fl->parity ^ 1;
+    HIPCHK(fl->ctx, launch_volume_source(fl->Q, fl->storage, sa, s));
   }
   fl->parity ^= 1;
   fl->time += 1;

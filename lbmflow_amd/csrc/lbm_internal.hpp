@@ -87,6 +87,19 @@ struct FaceArgs {
   float* stash_next[6];  // this step's, captured before the restore
 };
 
+// One volume source (lbm_open.hip): inclusive box, q_lu per step, velocity u
+// (zero for a mass flow).
+struct SourceArgs {
+  Dom d;
+  void* f;
+  const uint32_t* mask;
+  int odd;
+  uint32_t lo[3], hi[3];
+  float q_lu;
+  float u[3];
+};
+hipError_t launch_volume_source(int Q, int storage, const SourceArgs& a, hipStream_t s);
+
 // Force probe (lbm_probe.hip): links (i, q) with i - c_q a probed solid,
 // grouped by row in (row, x, q) order; cells = the distinct fluid cells.
 struct ProbeArgs {

@@ -394,6 +394,58 @@ __global__ __launch_bounds__(256) void k_face_bc(FaceArgs a, int face) {
   }
 }
 
+// ---- volume sources (apply_volume_sources_impl, backend.rs:799-856) ----------
+// after the open faces, per source in order: every fluid cell of the box
+// gains w_q q_cell (1 + 3 c.u + 4.5 (c.u)^2 - 1.5 u.u), q_cell = q_lu/count
+template <int Q, typename S>
+__global__ __launch_bounds__(256) void k_volume_source(SourceArgs a) {
+  const uint32_t ex = a.hi[0] - a.lo[0] + 1, ey = a.hi[1] - a.lo[1] + 1, ez = a.hi[2] - a.lo[2] + 1;
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= ex * ey * ez) return;
+  const uint32_t x = a.lo[0] + k % ex, y = a.lo[1] + (k / ex) % ey, z = a.lo[2] + k / (ex * ey);
+  const uint32_t i = (z * a.d.ny + y) * a.d.nx + x;
+  const uint32_t m = a.mask[i];
+  if (m & kMaskSolid) return;
+  const Nbr nb(a.d, x, y, z);
+  S* f = (S*)a.f;
+  const bool odd = a.odd != 0;
+  const float q_cell = a.q_lu / (float)((double)ex * ey * ez);
+  const float u0 = a.u[0], u1 = a.u[1], u2 = a.u[2];
+  float usq = u0 * u0;
+  usq = usq + u1 * u1;
+  usq = usq + u2 * u2;
+  sfor<0, Q>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    float cu = (float)lat_c(q, 0) * u0;  // every term kept, as the reference's loop
+    cu = cu + (float)lat_c(q, 1) * u1;
+    cu = cu + (float)lat_c(q, 2) * u2;
+    const float delta = lat_wf(Q, q) * q_cell * (1.0f + 3.0f * cu + 4.5f * cu * cu - 1.5f * usq);
+    write_pop<Q, S, q>(f, a.d.ps, nb, i, m, odd, read_pop<Q, S, q>(f, a.d.ps, nb, i, m, odd) + delta);
+  });
+}
+
+hipError_t launch_volume_source(int Q, int storage, const SourceArgs& a, hipStream_t s) {
+  const uint32_t n = (a.hi[0] - a.lo[0] + 1) * (a.hi[1] - a.lo[1] + 1) * (a.hi[2] - a.lo[2] + 1);
+  const dim3 g((n + 255) / 256), b(256);
+  if (storage == kF32) {
+    if (Q == 19)
+      hipLaunchKernelGGL((k_volume_source<19, float>), g, b, 0, s, a);
+    else
+      hipLaunchKernelGGL((k_volume_source<27, float>), g, b, 0, s, a);
+  } else if (storage == kF16) {
+    if (Q == 19)
+      hipLaunchKernelGGL((k_volume_source<19, _Float16>), g, b, 0, s, a);
+    else
+      hipLaunchKernelGGL((k_volume_source<27, _Float16>), g, b, 0, s, a);
+  } else {
+    if (Q == 19)
+      hipLaunchKernelGGL((k_volume_source<19, bf16s>), g, b, 0, s, a);
+    else
+      hipLaunchKernelGGL((k_volume_source<27, bf16s>), g, b, 0, s, a);
+  }
+  return hipGetLastError();
+}
+
 // fluid cells of non-periodic faces that no open face covers
 __global__ void k_uncovered(Dom d, const uint8_t* solid, uint32_t open_bits, unsigned int* count) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;

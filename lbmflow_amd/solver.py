@@ -228,6 +228,23 @@ class Solver:
             om = np.ascontiguousarray(omega, np.float32).reshape(-1)
             self._chk(self.lib.lbm_set_omega_field(self.h, _fp(om)))
 
+    def set_volume_sources(self, sources):
+        """``StepParams::sources`` (params.rs:262-283): a list of
+        (lo, hi, q_lu) mass flows and (lo, hi, q_lu, u) jets over inclusive
+        cell boxes."""
+        arr = (_L.VolumeSource * max(len(sources), 1))()
+        for k, src in enumerate(sources):
+            e = arr[k]
+            for c in range(3):
+                e.lo[c] = int(src[0][c])
+                e.hi[c] = int(src[1][c])
+            e.q_lu = float(src[2])
+            e.kind = 1 if len(src) > 3 else 0
+            if len(src) > 3:
+                for c in range(3):
+                    e.u[c] = float(src[3][c])
+        self._chk(self.lib.lbm_set_volume_sources(self.h, len(sources), arr))
+
     def set_force_probe(self, mask):
         """``Solver::set_force_probe`` (solver.rs:2741-2760): solids whose
         momentum-exchange force is accumulated each step (None removes)."""

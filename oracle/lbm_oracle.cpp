@@ -224,6 +224,15 @@ struct Solver {
   // open faces (FaceBC, params.rs:232-252), Face::ALL order XNeg..ZPos:
   // kind 0 closed, 1 Zou-He velocity, 2 Zou-He pressure, 3 outflow,
   // 4 convective
+  // volume sources (VolumeSource, params.rs:262-283): box lo..hi inclusive,
+  // kind 0 mass flow, 1 jet with velocity u
+  struct Src {
+    long lo[3], hi[3];
+    int kind;
+    T q_lu;
+    T u[3];
+  };
+  std::vector<Src> sources;
   int face_kind[6] = {0, 0, 0, 0, 0, 0};
   T face_u[6][3] = {};
   T face_rho[6] = {};
@@ -859,6 +868,32 @@ struct Solver {
         convective_face(face);
     }
   }
+  // apply_volume_sources_impl, backend.rs:799-856 (single part: global ==
+  // local coordinates), sources in order
+  void apply_volume_sources() {
+    const T one = T(1), three = T(3.0), f45 = T(4.5), f15 = T(1.5);
+    for (const Src& src : sources) {
+      const long count = (src.hi[0] - src.lo[0] + 1) * (src.hi[1] - src.lo[1] + 1) * (src.hi[2] - src.lo[2] + 1);
+      const T q_cell = src.q_lu / T((double)count);
+      T u[3] = {T(0), T(0), T(0)};
+      if (src.kind == 1)
+        for (int a = 0; a < 3; ++a) u[a] = src.u[a];
+      T usq = u[0] * u[0];
+      for (int a = 1; a < 3; ++a) usq = usq + u[a] * u[a];
+      for (long z = src.lo[2]; z <= src.hi[2]; ++z)
+        for (long y = src.lo[1]; y <= src.hi[1]; ++y)
+          for (long x = src.lo[0]; x <= src.hi[0]; ++x) {
+            const size_t i = pidx(x, y, z);
+            if (solid[i]) continue;
+            for (int q = 0; q < L.Q; ++q) {
+              T cu = T((double)L.c[q][0]) * u[0];
+              for (int a = 1; a < 3; ++a) cu = cu + T((double)L.c[q][a]) * u[a];
+              const T delta = T(L.w[q]) * q_cell * (one + three * cu + f45 * cu * cu - f15 * usq);
+              f[q * np + i] = f[q * np + i] + delta;
+            }
+          }
+    }
+  }
   bool any_open() const {
     for (int k : face_kind)
       if (k) return true;
@@ -875,6 +910,7 @@ struct Solver {
     stream();
     std::swap(f, ftmp);
     if (any_open()) apply_open_faces();
+    if (!sources.empty()) apply_volume_sources();
     update_moments();
   }
 
@@ -1796,6 +1832,28 @@ void orc_set_omega_field(void* p, const double* om) {
       S->has_omega = true;
       S->omega_field.resize(S->nc);
       for (size_t c = 0; c < S->nc; ++c) S->omega_field[c] = (TT)om[c];
+    }
+  });
+}
+
+// Volume source list: n entries of 10 doubles (lo[3], hi[3], kind, q_lu,
+// u[3] packed as lo0 lo1 lo2 hi0 hi1 hi2 kind q_lu u0 u1 u2 -> 11 values).
+void orc_set_sources(void* p, int n, const double* v) {
+  Handle* h = (Handle*)p;
+  DISPATCH(h, {
+    using TT = std::remove_reference_t<decltype(S->f[0])>;
+    S->sources.clear();
+    for (int k = 0; k < n; ++k) {
+      const double* e = v + 11 * k;
+      typename std::remove_reference_t<decltype(*S)>::Src src{};
+      for (int a = 0; a < 3; ++a) {
+        src.lo[a] = (long)e[a];
+        src.hi[a] = (long)e[3 + a];
+        src.u[a] = (TT)e[8 + a];
+      }
+      src.kind = (int)e[6];
+      src.q_lu = (TT)e[7];
+      S->sources.push_back(src);
     }
   });
 }

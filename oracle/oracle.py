@@ -67,6 +67,7 @@ def lib():
         L.orc_strain.argtypes = [vp, d]
         L.orc_set_face.argtypes = [vp, C.c_int, C.c_int, d]
         L.orc_set_probe.argtypes = [vp, u8]
+        L.orc_set_sources.argtypes = [vp, C.c_int, d]
         L.orc_probed_force.argtypes = [vp, d]
         L.orc_vgrad.argtypes = [vp, d]
         L.orc_wale.restype = C.c_long
@@ -221,6 +222,17 @@ class OracleSolver:
         else:
             om = np.ascontiguousarray(om, np.float64)
             lib().orc_set_omega_field(self.h, _d(om))
+
+    def set_volume_sources(self, sources):
+        """``StepParams::sources`` (params.rs:262-283): list of
+        (lo, hi, q_lu) mass flows or (lo, hi, q_lu, u) jets."""
+        rows = []
+        for src in sources:
+            lo, hi, q = src[0], src[1], src[2]
+            u = src[3] if len(src) > 3 else None
+            rows.append(list(lo) + list(hi) + [0.0 if u is None else 1.0, q] + list(u if u is not None else (0, 0, 0)))
+        v = np.ascontiguousarray(np.array(rows, np.float64).reshape(-1))
+        lib().orc_set_sources(self.h, len(rows), _d(v) if len(rows) else None)
 
     def set_force_probe(self, mask):
         """``Solver::set_force_probe`` (solver.rs:2741-2760) over a compact mask."""
