@@ -75,6 +75,7 @@ extern "C" {
 #define LBM_CAP_GRAVITY 0x800u       /* not set */
 #define LBM_CAP_PROBE 0x1000u        /* momentum-exchange force probe */
 #define LBM_CAP_WALE 0x2000u         /* WALE omega_plus producer (les.rs) */
+#define LBM_CAP_LOCALIZED 0x4000u    /* volume sources and face patches (supports_localized_features) */
 
 typedef struct lbm_ctx lbm_ctx;
 typedef struct lbm_fields lbm_fields;
@@ -154,8 +155,8 @@ int lbm_set_omega_field(lbm_fields* fl, const float* omega);
  * (lattice.rs:38-45).  Applied after every collide-stream step in face
  * order, after the populations skipped by the stream have been restored to
  * their previous post-collide values (apply_open_faces_impl,
- * backend.rs:602-705; kernels.rs:849-1299).  No inlet profiles or face
- * patches on this path. */
+ * backend.rs:602-705; kernels.rs:849-1299).  No inlet profiles on this
+ * path. */
 #define LBM_FACE_XNEG 0
 #define LBM_FACE_XPOS 1
 #define LBM_FACE_YNEG 2
@@ -168,10 +169,31 @@ int lbm_set_omega_field(lbm_fields* fl, const float* omega);
 #define LBM_FACE_OUTFLOW 3    /* zero-gradient copy from one cell inward */
 #define LBM_FACE_CONVECTIVE 4 /* value[0] = u_conv, mass pinned to the inner neighbour */
 /* GlobalSpec faces[face] = kind (params.rs:302-320): an open face on a
- * periodic axis is LBM_ERR_INVALID.  Stepping refuses fluid cells on a
+ * periodic axis, a velocity above MAX_SPEED = 0.3 (params.rs:11) or a
+ * non-positive density is LBM_ERR_INVALID.  Stepping refuses fluid cells on a
  * non-periodic face that no open face covers, and scalar distributions
  * together with open faces (LBM_ERR_UNSUPPORTED). */
 int lbm_set_face_bc(lbm_fields* fl, int face, int kind, const double value[3]);
+
+/* FacePatch (params.rs:286-296) and the patch arms of apply_open_faces_impl
+ * (backend.rs:602-766): each patch gives an inclusive rect lo..hi of face
+ * `face` (in-face coordinates over the two other axes in ascending order:
+ * x faces (y, z), y faces (x, z), z faces (x, y)) its own rule, kind and
+ * value as for lbm_set_face_bc; a LBM_FACE_CLOSED patch is a resting
+ * (zero-velocity Zou-He) lid.  A face holding patches takes the open-face
+ * pass even when its own rule is closed: its cells outside every patch then
+ * get the resting lid.  lbm_set_face_patches replaces the list (n = 0
+ * clears).  LBM_ERR_INVALID (validate_face_patches, solver.rs:690-740): bad
+ * face or kind, a rect outside its face or inverted, two rects of one face
+ * overlapping, a rule parameter out of range, a patch on a periodic axis,
+ * more than 16 patches. */
+typedef struct lbm_face_patch {
+  int face;
+  int lo[2], hi[2];
+  int kind;
+  double value[3];
+} lbm_face_patch;
+int lbm_set_face_patches(lbm_fields* fl, int n, const lbm_face_patch* patches);
 
 /* VolumeSource / SourceKind (params.rs:262-283): after the open faces of
  * every step, each fluid cell of the inclusive box lo..hi gains

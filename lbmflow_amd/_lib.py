@@ -52,6 +52,11 @@ class VolumeSource(C.Structure):
                 ("u", C.c_double * 3)]
 
 
+class FacePatch(C.Structure):
+    _fields_ = [("face", C.c_int), ("lo", C.c_int * 2), ("hi", C.c_int * 2), ("kind", C.c_int),
+                ("value", C.c_double * 3)]
+
+
 class WaleDiag(C.Structure):
     _fields_ = [("clipped_cells", C.c_int64), ("clipped_fraction", C.c_double),
                 ("max_nu_t_before_clipping", C.c_double), ("tau_eff_max", C.c_double), ("nu_t_max", C.c_double)]
@@ -90,6 +95,7 @@ SIGNATURES = {
     "lbm_set_face_bc": (_I, [_P, _I, _I, _pD]),
     "lbm_set_force_probe": (_I, [_P, C.POINTER(C.c_uint8)]),
     "lbm_set_volume_sources": (_I, [_P, _I, C.POINTER(VolumeSource)]),
+    "lbm_set_face_patches": (_I, [_P, _I, C.POINTER(FacePatch)]),
     "lbm_probed_force": (_I, [_P, _pD]),
     "lbm_gather_omega_field": (_I, [_P, _pF, _pI]),
     "lbm_stage_in": (_I, [_P, _pF]),

@@ -91,6 +91,8 @@ struct lbm_fields {
   // (two generations of unknowns x face cells), which generation is current,
   // and the fluid face cells no open face covers
   int face_kind[6] = {0, 0, 0, 0, 0, 0};
+  std::vector<FacePatchArgs> patches;  // rect overrides (closed ones as zero velocity)
+  int patched[6] = {0, 0, 0, 0, 0, 0};  // faces holding at least one patch
   float face_u[6][3] = {};
   float face_rho[6] = {};
   float face_uconv[6] = {};
@@ -287,11 +289,31 @@ static int normalize_parity(lbm_fields* fl) {
   return LBM_OK;
 }
 
+// faces with an open-face pass: an open rule, or patches (a closed face
+// with patches is a zero-velocity lid outside them)
 uint32_t open_bits(const lbm_fields* fl) {
   uint32_t b = 0;
   for (int k = 0; k < 6; ++k)
-    if (fl->face_kind[k]) b |= 1u << k;
+    if (fl->face_kind[k] || fl->patched[k]) b |= 1u << k;
   return b;
+}
+
+// the face's two stash generations, zeroed (the reference's second buffer)
+static int ensure_stash(lbm_fields* fl, int face) {
+  if (fl->stash[face][0]) return LBM_OK;
+  const int ax = face / 2;
+  const uint32_t ext[3] = {fl->d.nx, fl->d.ny, fl->d.nz};
+  const uint32_t t1 = ax == 0 ? 1 : 0, t2 = ax == 2 ? 1 : 2;
+  const size_t bytes = (size_t)ext[t1] * ext[t2] * (fl->Q == 27 ? 9 : 5) * sizeof(float);
+  hipStream_t s = fl->ctx->stream;
+  for (int g = 0; g < 2; ++g) {
+    int rc = dev_alloc(fl, (void**)&fl->stash[face][g], bytes);
+    if (rc) return rc;
+    HIPCHK(fl->ctx, hipMemsetAsync(fl->stash[face][g], 0, bytes, s));
+  }
+  fl->stash_bytes[face] = bytes;
+  HIPCHK(fl->ctx, hipStreamSynchronize(s));
+  return LBM_OK;
 }
 
 // Fluid cells on a non-periodic face that no open face rule covers (their
@@ -458,7 +480,8 @@ int lbm_stream_copy(lbm_ctx* ctx, size_t bytes, int iterations, double* gbps) {
 int lbm_capabilities(const lbm_ctx* ctx, uint32_t* flags) {
   if (!ctx || !flags) return LBM_ERR_INVALID;
   *flags = LBM_CAP_TRT | LBM_CAP_CENTRAL_MOMENT | LBM_CAP_F16_STORAGE | LBM_CAP_BF16_STORAGE | LBM_CAP_FORCE_FIELD |
-           LBM_CAP_OMEGA_FIELD | LBM_CAP_ROTATING_IBM | LBM_CAP_SCALAR | LBM_CAP_PERIODIC | LBM_CAP_WALE | LBM_CAP_OPEN_FACES | LBM_CAP_PROBE;
+           LBM_CAP_OMEGA_FIELD | LBM_CAP_ROTATING_IBM | LBM_CAP_SCALAR | LBM_CAP_PERIODIC | LBM_CAP_WALE | LBM_CAP_OPEN_FACES | LBM_CAP_PROBE |
+           LBM_CAP_LOCALIZED;
   return LBM_OK;
 }
 
@@ -1045,6 +1068,30 @@ static int check_steppable(lbm_fields* fl) {
 }
 
 // ---- open faces ------------------------------------------------------------
+// validate_face_bc / validate_velocity (solver.rs:745-780) and the open-axis
+// extent rule (solver.rs:566-575) for one rule on `face`
+static int face_rule_check(lbm_fields* fl, int face, int kind, const double* value) {
+  if (kind == LBM_FACE_VELOCITY) {
+    double sq = 0.0;
+    for (int a = 0; a < 3; ++a) {
+      if (!std::isfinite(value[a])) return fail(fl->ctx, LBM_ERR_INVALID, "face velocity must be finite");
+      sq += value[a] * value[a];
+    }
+    if (!(std::sqrt(sq) <= kMaxSpeed))
+      return fail(fl->ctx, LBM_ERR_INVALID, "prescribed speed %g exceeds the low-Mach limit %g", std::sqrt(sq),
+                  kMaxSpeed);
+  }
+  if (kind == LBM_FACE_PRESSURE && !(value[0] > 0.0 && std::isfinite(value[0])))
+    return fail(fl->ctx, LBM_ERR_INVALID, "face density must be finite and positive");
+  if (kind == LBM_FACE_CONVECTIVE && !(value[0] >= 0.0 && std::isfinite(value[0])))
+    return fail(fl->ctx, LBM_ERR_INVALID, "convective speed must be finite and non-negative");
+  if (kind != LBM_FACE_CLOSED && fl->Q != 19 && fl->Q != 27) return LBM_ERR_UNSUPPORTED;
+  const uint32_t ext[3] = {fl->d.nx, fl->d.ny, fl->d.nz};
+  if (kind != LBM_FACE_CLOSED && ext[face / 2] < 3)
+    return fail(fl->ctx, LBM_ERR_INVALID, "an open face needs at least 3 cells along its axis");
+  return LBM_OK;
+}
+
 int lbm_set_face_bc(lbm_fields* fl, int face, int kind, const double value[3]) {
   if (!fl || face < 0 || face > 5 || kind < LBM_FACE_CLOSED || kind > LBM_FACE_CONVECTIVE)
     return LBM_ERR_INVALID;
@@ -1052,35 +1099,65 @@ int lbm_set_face_bc(lbm_fields* fl, int face, int kind, const double value[3]) {
   if (kind != LBM_FACE_CLOSED && fl->d.per[ax])
     return fail(fl->ctx, LBM_ERR_INVALID, "face %d is on a periodic axis and cannot be open", face);
   if (kind != LBM_FACE_CLOSED && kind != LBM_FACE_OUTFLOW && !value) return LBM_ERR_INVALID;
-  if (kind == LBM_FACE_VELOCITY)
-    for (int a = 0; a < 3; ++a)
-      if (!std::isfinite(value[a])) return fail(fl->ctx, LBM_ERR_INVALID, "face velocity must be finite");
-  if (kind == LBM_FACE_PRESSURE && !(value[0] > 0.0 && std::isfinite(value[0])))
-    return fail(fl->ctx, LBM_ERR_INVALID, "face density must be finite and positive");
-  if (kind == LBM_FACE_CONVECTIVE && !(value[0] >= 0.0 && std::isfinite(value[0])))
-    return fail(fl->ctx, LBM_ERR_INVALID, "convective speed must be finite and non-negative");
-  if (kind != LBM_FACE_CLOSED && fl->Q != 19 && fl->Q != 27) return LBM_ERR_UNSUPPORTED;
-  const uint32_t ext[3] = {fl->d.nx, fl->d.ny, fl->d.nz};
-  if (kind != LBM_FACE_CLOSED && ext[ax] < 3)
-    return fail(fl->ctx, LBM_ERR_INVALID, "an open face needs at least 3 cells along its axis");
-  const uint32_t t1 = ax == 0 ? 1 : 0, t2 = ax == 2 ? 1 : 2;
-  const size_t nf = (size_t)ext[t1] * ext[t2];
-  const size_t bytes = nf * (fl->Q == 27 ? 9 : 5) * sizeof(float);
-  hipStream_t s = fl->ctx->stream;
-  if (kind != LBM_FACE_CLOSED && !fl->stash[face][0]) {
-    for (int g = 0; g < 2; ++g) {
-      int rc = dev_alloc(fl, (void**)&fl->stash[face][g], bytes);
-      if (rc) return rc;
-      // the stale slots start as the reference's zeroed second buffer
-      HIPCHK(fl->ctx, hipMemsetAsync(fl->stash[face][g], 0, bytes, s));
-    }
-    fl->stash_bytes[face] = bytes;
-    HIPCHK(fl->ctx, hipStreamSynchronize(s));
+  int rc = face_rule_check(fl, face, kind, value);
+  if (rc) return rc;
+  if (kind != LBM_FACE_CLOSED) {
+    rc = ensure_stash(fl, face);
+    if (rc) return rc;
   }
   fl->face_kind[face] = kind;
   for (int a = 0; a < 3; ++a) fl->face_u[face][a] = kind == LBM_FACE_VELOCITY ? (float)value[a] : 0.0f;
   fl->face_rho[face] = kind == LBM_FACE_PRESSURE ? (float)value[0] : 0.0f;
   fl->face_uconv[face] = kind == LBM_FACE_CONVECTIVE ? (float)value[0] : 0.0f;
+  fl->gen++;
+  return count_uncovered(fl);
+}
+
+// FacePatch list (params.rs:286-296) with validate_face_patches
+// (solver.rs:690-740): face in range, rect inside the face and not inverted,
+// rule parameters in range, no two rects of one face overlapping; a patch on
+// a periodic axis is refused.
+int lbm_set_face_patches(lbm_fields* fl, int n, const lbm_face_patch* patches) {
+  if (!fl || n < 0 || (n && !patches)) return LBM_ERR_INVALID;
+  if (n > kMaxFacePatches) return fail(fl->ctx, LBM_ERR_INVALID, "at most %d face patches", kMaxFacePatches);
+  const uint32_t ext[3] = {fl->d.nx, fl->d.ny, fl->d.nz};
+  std::vector<FacePatchArgs> list;
+  for (int k = 0; k < n; ++k) {
+    const lbm_face_patch& p = patches[k];
+    if (p.face < 0 || p.face > 5 || p.kind < LBM_FACE_CLOSED || p.kind > LBM_FACE_CONVECTIVE)
+      return fail(fl->ctx, LBM_ERR_INVALID, "patch %d: bad face or kind", k);
+    const int ax = p.face / 2, t1 = ax == 0 ? 1 : 0, t2 = ax == 2 ? 1 : 2;
+    if (fl->d.per[ax]) return fail(fl->ctx, LBM_ERR_INVALID, "patch %d lies on a periodic axis", k);
+    if (p.lo[0] < 0 || p.lo[1] < 0 || p.lo[0] > p.hi[0] || p.lo[1] > p.hi[1] || (uint32_t)p.hi[0] >= ext[t1] ||
+        (uint32_t)p.hi[1] >= ext[t2])
+      return fail(fl->ctx, LBM_ERR_INVALID, "patch %d: rect outside face %d", k, p.face);
+    int rc = face_rule_check(fl, p.face, p.kind, p.value);
+    if (rc) return rc;
+    if (fl->Q != 19 && fl->Q != 27) return LBM_ERR_UNSUPPORTED;
+    for (int j = 0; j < k; ++j) {
+      const lbm_face_patch& o = patches[j];
+      if (o.face == p.face && p.lo[0] <= o.hi[0] && o.lo[0] <= p.hi[0] && p.lo[1] <= o.hi[1] && o.lo[1] <= p.hi[1])
+        return fail(fl->ctx, LBM_ERR_INVALID, "patches %d and %d overlap on face %d", j, k, p.face);
+    }
+    FacePatchArgs a{};
+    a.face = p.face;
+    for (int c = 0; c < 2; ++c) {
+      a.lo[c] = (uint32_t)p.lo[c];
+      a.hi[c] = (uint32_t)p.hi[c];
+    }
+    a.kind = p.kind == LBM_FACE_CLOSED ? kFaceVelocity : p.kind;  // a closed patch: a resting lid
+    for (int c = 0; c < 3; ++c) a.u[c] = p.kind == LBM_FACE_VELOCITY ? (float)p.value[c] : 0.0f;
+    a.rho = p.kind == LBM_FACE_PRESSURE ? (float)p.value[0] : 0.0f;
+    a.uconv = p.kind == LBM_FACE_CONVECTIVE ? (float)p.value[0] : 0.0f;
+    list.push_back(a);
+  }
+  for (const FacePatchArgs& a : list) {
+    int rc = ensure_stash(fl, a.face);
+    if (rc) return rc;
+  }
+  fl->patches = list;
+  for (int k = 0; k < 6; ++k) fl->patched[k] = 0;
+  for (const FacePatchArgs& a : list) fl->patched[a.face] = 1;
   fl->gen++;
   return count_uncovered(fl);
 }
@@ -1094,9 +1171,10 @@ static int open_faces_step(lbm_fields* fl) {
   a.p = fl->hp;
   a.F_cur = fl->field_present ? fl->F_cur : nullptr;
   for (int k = 0; k < 6; ++k) {
-    if (!fl->face_kind[k]) continue;
+    if (!fl->face_kind[k] && !fl->patched[k]) continue;
     a.faces[a.nfaces++] = k;
-    a.kind[k] = fl->face_kind[k];
+    // a closed face with patches: zero-velocity Zou-He outside them
+    a.kind[k] = fl->face_kind[k] ? fl->face_kind[k] : kFaceVelocity;
     for (int c = 0; c < 3; ++c) a.u[k][c] = fl->face_u[k][c];
     a.rho[k] = fl->face_rho[k];
     a.uconv[k] = fl->face_uconv[k];
@@ -1104,6 +1182,8 @@ static int open_faces_step(lbm_fields* fl) {
     a.stash_next[k] = fl->stash[k][fl->stash_sel ^ 1];
   }
   if (!a.nfaces) return LBM_OK;
+  a.npatch = (int)fl->patches.size();
+  for (int p = 0; p < a.npatch; ++p) a.patch[p] = fl->patches[p];
   HIPCHK(fl->ctx, launch_open_faces(fl->Q, fl->storage, a, fl->ctx->stream));
   fl->stash_sel ^= 1;
   return LBM_OK;

@@ -70,6 +70,20 @@ struct HydroArgs {
 // Open faces (lbm_open.hip): FaceBC kinds of params.rs:232-252, faces in
 // Face::ALL order (XNeg, XPos, YNeg, YPos, ZNeg, ZPos).
 constexpr int kFaceClosed = 0, kFaceVelocity = 1, kFacePressure = 2, kFaceOutflow = 3, kFaceConvective = 4;
+// MAX_SPEED (params.rs:11): the low-Mach limit on prescribed speeds, lattice units
+constexpr double kMaxSpeed = 0.3;
+// One face patch: inclusive rect lo..hi over the face's tangents (ascending
+// axis order) with its own rule (a closed patch arrives as a zero velocity).
+constexpr int kMaxFacePatches = 16;
+struct FacePatchArgs {
+  int face;
+  uint32_t lo[2], hi[2];
+  int kind;
+  float u[3];
+  float rho;
+  float uconv;
+};
+
 struct FaceArgs {
   Dom d;
   void* f;
@@ -85,6 +99,8 @@ struct FaceArgs {
   float uconv[6];
   float* stash_cur[6];   // per face: unknowns x face cells, previous step's post-collide values
   float* stash_next[6];  // this step's, captured before the restore
+  int npatch;            // rect overrides, at most one per face cell
+  FacePatchArgs patch[kMaxFacePatches];
 };
 
 // One volume source (lbm_open.hip): inclusive box, q_lu per step, velocity u

@@ -23,6 +23,11 @@
 //            which is the reference's post-swap state of the skipped slots;
 //   BC       (one launch per face, Face::ALL order): the face's rule, on
 //            the logical populations read and written in place.
+// Face patches give the cells of their rect their own rule; a face with
+// patches is open even when its own rule is closed, and its other cells then
+// take a zero-velocity Zou-He lid (backend.rs:602-766).  Within one face no
+// rule reads another face cell, so the reference's base-then-patches passes
+// are one pass with a per-cell rule.
 // When every open face lies on one axis the faces share no cell and no
 // thread reads what another writes, so one launch does all three per cell.
 // The stash is double-buffered and starts at zero (the reference's ftmp).
@@ -248,7 +253,27 @@ __device__ __forceinline__ void bc_cell(const FaceArgs& a, uint32_t coord) {
       fv[q] = read_pop<Q, S, q>(f, a.d.ps, nb, i, m, odd);
     }
   });
-  const int kind = a.kind[FACE];
+  // the cell's rule: the patch whose rect holds it, else the face's own
+  // (patches on one face never overlap)
+  int kind = a.kind[FACE];
+  float ub[3] = {a.u[FACE][0], a.u[FACE][1], a.u[FACE][2]};
+  float rho_bc = a.rho[FACE], lam = a.uconv[FACE];
+  if (a.npatch) {
+    const uint32_t ext[3] = {a.d.nx, a.d.ny, a.d.nz};
+    const uint32_t c1 = coord % ext[t1], c2 = coord / ext[t1];
+    for (int p = 0; p < a.npatch; ++p) {
+      const FacePatchArgs& pt = a.patch[p];
+      if (pt.face == FACE && c1 >= pt.lo[0] && c1 <= pt.hi[0] && c2 >= pt.lo[1] && c2 <= pt.hi[1]) {
+        kind = pt.kind;
+        ub[0] = pt.u[0];
+        ub[1] = pt.u[1];
+        ub[2] = pt.u[2];
+        rho_bc = pt.rho;
+        lam = pt.uconv;
+        break;
+      }
+    }
+  }
   if (kind == kFaceVelocity || kind == kFacePressure) {
     float f0[3] = {a.p.F[0], a.p.F[1], a.p.F[2]};
     if (a.F_cur && (m & kMaskForced)) {  // force_independent_at: field nonzero only on the force list
@@ -256,8 +281,6 @@ __device__ __forceinline__ void bc_cell(const FaceArgs& a, uint32_t coord) {
       f0[1] = f0[1] + a.F_cur[3 * (size_t)i + 1];
       f0[2] = f0[2] + a.F_cur[3 * (size_t)i + 2];
     }
-    const float* ub = a.u[FACE];
-    const float rho_bc = a.rho[FACE];
     const float nsign = (float)F::nin, one = 1.0f, two = 2.0f, half = 0.5f;
     float r, vn, vt1, vt2;
     if constexpr (Q == 27) {  // zou_he_face_d3q27, kernels.rs:958-1083
@@ -345,7 +368,6 @@ __device__ __forceinline__ void bc_cell(const FaceArgs& a, uint32_t coord) {
     return;
   }
   // convective
-  const float lam = a.uconv[FACE];
   const float inv = 1.0f / (1.0f + lam);
   float fj[Q];
   sfor<0, Q>([&](auto qc) {

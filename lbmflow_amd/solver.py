@@ -276,6 +276,24 @@ class Solver:
             v[:vv.size] = vv
         self._chk(self.lib.lbm_set_face_bc(self.h, f, self.FACE_KINDS[kind], _dp(v)))
 
+    def set_face_patches(self, patches):
+        """``GlobalSpec::face_patches`` (FacePatch, params.rs:286-296): a list
+        of (face, (lo1, lo2), (hi1, hi2), kind, value) with inclusive rects
+        over the face's two other axes in ascending order, kind and value as
+        for :meth:`set_face_bc`; replaces the list."""
+        arr = (_L.FacePatch * max(len(patches), 1))()
+        for k, (face, lo, hi, kind, value) in enumerate(patches):
+            e = arr[k]
+            e.face = self.FACES[face] if isinstance(face, str) else int(face)
+            e.lo[0], e.lo[1] = int(lo[0]), int(lo[1])
+            e.hi[0], e.hi[1] = int(hi[0]), int(hi[1])
+            e.kind = self.FACE_KINDS[kind]
+            if value is not None:
+                vv = np.atleast_1d(np.asarray(value, np.float64))
+                for c in range(vv.size):
+                    e.value[c] = float(vv[c])
+        self._chk(self.lib.lbm_set_face_patches(self.h, len(patches), arr))
+
     def wale_update(self, cw: float = WALE_CW, tau_eff_max: float | None = None, want_nu_t: bool = False):
         """One ``WaleLes::update`` (les.rs:120-211): install the WALE omega
         field of the current state.  Returns the diagnostics dict

@@ -14,6 +14,7 @@
 #include <xmmintrin.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -237,6 +238,21 @@ struct Solver {
   T face_u[6][3] = {};
   T face_rho[6] = {};
   T face_uconv[6] = {};
+  // one face rule: kind as above with its parameters
+  struct Rule {
+    int kind;
+    T u[3];
+    T rho;
+    T uconv;
+  };
+  // FacePatch (params.rs:286-296): inclusive in-face rect lo..hi over the
+  // face's tangents (ascending axis order) with its own rule
+  struct Patch {
+    int face;
+    long lo[2], hi[2];
+    Rule bc;
+  };
+  std::vector<Patch> patches;
 
   size_t pidx(long x, long y, long z) const {
     return ((size_t)(z + 1) * py + (size_t)(y + 1)) * px + (size_t)(x + 1);
@@ -676,13 +692,27 @@ struct Solver {
     }
     return g;
   }
-  // for_face_cells_selected with FaceCellSelection::All (kernels.rs:742-766):
-  // t2 outer, t1 inner
+  // FaceCellSelection (kernels.rs:718-740): kind 0 All, 1 Rect lo..hi,
+  // 2 Excluding every rect of `ex`
+  struct Sel {
+    int kind = 0;
+    long lo[2] = {0, 0}, hi[2] = {0, 0};
+    const std::vector<std::array<long, 4>>* ex = nullptr;
+    bool operator()(long c1, long c2) const {
+      if (kind == 1) return lo[0] <= c1 && c1 <= hi[0] && lo[1] <= c2 && c2 <= hi[1];
+      if (kind == 2)
+        for (const auto& r : *ex)
+          if (r[0] <= c1 && c1 <= r[2] && r[1] <= c2 && c2 <= r[3]) return false;
+      return true;
+    }
+  };
+  // for_face_cells_selected (kernels.rs:742-766): t2 outer, t1 inner
   template <class F>
-  void for_face_cells(const FaceGeo& g, F&& body) {
+  void for_face_cells(const FaceGeo& g, const Sel& sel, F&& body) {
     const long n[3] = {nx, ny, nz};
     for (long c2 = 0; c2 < n[g.t2]; ++c2)
       for (long c1 = 0; c1 < n[g.t1]; ++c1) {
+        if (!sel(c1, c2)) continue;
         long pos[3];
         pos[g.a] = g.fixed;
         pos[g.t1] = c1;
@@ -694,14 +724,14 @@ struct Solver {
   // zou_he_face_3d (D3Q19, kernels.rs:1085-1219) and zou_he_face_d3q27
   // (kernels.rs:958-1083); kind 1 velocity, 2 pressure.  Gravity is not on
   // this path (g = 0); F0 = params force + field (force_independent_at).
-  void zou_he_face(int face) {
+  void zou_he_face(int face, const Rule& bc, const Sel& sel) {
     const FaceGeo g = face_geo(face);
     const int a = g.a, t1 = g.t1, t2 = g.t2;
     const T nsign = T(g.nin[a]);
     const T half = T(0.5), two = T(2.0), one = T(1);
-    const bool vel = face_kind[face] == 1;
-    const T* ub = face_u[face];
-    const T rho_bc = face_rho[face];
+    const bool vel = bc.kind == 1;
+    const T* ub = bc.u;
+    const T rho_bc = bc.rho;
     if (L.Q == 27) {
       double s2a = 0.0, s2b = 0.0;
       for (int k = 0; k < g.nunk; ++k) {
@@ -710,7 +740,7 @@ struct Solver {
         s2b += L.w[q] * (double)L.c[q][t2] * (double)L.c[q][t2];
       }
       const T s2_t1 = T(s2a), s2_t2 = T(s2b), six = T(6.0);
-      for_face_cells(g, [&](const long* pos) {
+      for_face_cells(g, sel, [&](const long* pos) {
         const size_t i = pidx(pos[0], pos[1], pos[2]);
         if (solid[i]) return;
         T f0[3];
@@ -758,7 +788,7 @@ struct Solver {
     const int q_mp = dir_of(-e1[0] + e2[0], -e1[1] + e2[1], -e1[2] + e2[2]);
     const int q_mm = dir_of(-e1[0] - e2[0], -e1[1] - e2[1], -e1[2] - e2[2]);
     const T c13 = T(1.0 / 3.0), c16 = T(1.0 / 6.0);
-    for_face_cells(g, [&](const long* pos) {
+    for_face_cells(g, sel, [&](const long* pos) {
       const size_t i = pidx(pos[0], pos[1], pos[2]);
       if (solid[i]) return;
       T f0[3];
@@ -817,9 +847,9 @@ struct Solver {
     }
   }
   // outflow_face_selected, kernels.rs:1221-1248
-  void outflow_face(int face) {
+  void outflow_face(int face, const Sel& sel) {
     const FaceGeo g = face_geo(face);
-    for_face_cells(g, [&](const long* pos) {
+    for_face_cells(g, sel, [&](const long* pos) {
       const size_t i = pidx(pos[0], pos[1], pos[2]);
       const size_t j = pidx(pos[0] + g.nin[0], pos[1] + g.nin[1], pos[2] + g.nin[2]);
       if (solid[i] || solid[j]) return;
@@ -827,14 +857,13 @@ struct Solver {
     });
   }
   // convective_face_selected, kernels.rs:1254-1299
-  void convective_face(int face) {
+  void convective_face(int face, T lam, const Sel& sel) {
     const FaceGeo g = face_geo(face);
-    const T lam = face_uconv[face];
     const T inv = T(1) / (T(1) + lam);
     double ws = 0.0;
     for (int k = 0; k < g.nunk; ++k) ws += L.w[g.unk[k]];
     const T wsum = T(ws);
-    for_face_cells(g, [&](const long* pos) {
+    for_face_cells(g, sel, [&](const long* pos) {
       const size_t i = pidx(pos[0], pos[1], pos[2]);
       const size_t j = pidx(pos[0] + g.nin[0], pos[1] + g.nin[1], pos[2] + g.nin[2]);
       if (solid[i] || solid[j]) return;
@@ -855,17 +884,48 @@ struct Solver {
       }
     });
   }
-  // apply_open_faces_impl, backend.rs:602-705 (no face patches or inlet
-  // profiles on this path): the faces in Face::ALL order
+  void apply_rule(int face, const Rule& bc, const Sel& sel) {
+    if (bc.kind == 0) {  // a Closed patch: impermeable lid, zero-velocity Zou-He
+      Rule z{1, {T(0), T(0), T(0)}, T(0), T(0)};
+      zou_he_face(face, z, sel);
+    } else if (bc.kind == 1 || bc.kind == 2) {
+      zou_he_face(face, bc, sel);
+    } else if (bc.kind == 3) {
+      outflow_face(face, sel);
+    } else if (bc.kind == 4) {
+      convective_face(face, bc.uconv, sel);
+    }
+  }
+  // apply_open_faces_impl, backend.rs:602-766 (single part: the patch rects
+  // are already local; no inlet profiles on this path): the faces in
+  // Face::ALL order; on each, the base rule over the cells outside every
+  // patch (zero-velocity Zou-He when the base is closed and the face has
+  // patches), then each patch's rule over its rect, in patch order
   void apply_open_faces() {
     for (int face = 0; face < 6; ++face) {
+      std::vector<std::array<long, 4>> ex;
+      std::vector<const Patch*> mine;
+      for (const Patch& pt : patches)
+        if (pt.face == face) {
+          mine.push_back(&pt);
+          ex.push_back({pt.lo[0], pt.lo[1], pt.hi[0], pt.hi[1]});
+        }
       const int k = face_kind[face];
-      if (k == 1 || k == 2)
-        zou_he_face(face);
-      else if (k == 3)
-        outflow_face(face);
-      else if (k == 4)
-        convective_face(face);
+      if (!k && mine.empty()) continue;
+      Sel excl;
+      excl.kind = 2;
+      excl.ex = &ex;
+      Rule base{k, {face_u[face][0], face_u[face][1], face_u[face][2]}, face_rho[face], face_uconv[face]};
+      apply_rule(face, base, excl);  // kind 0 here: the closed base with patches
+      for (const Patch* pt : mine) {
+        Sel r;
+        r.kind = 1;
+        r.lo[0] = pt->lo[0];
+        r.lo[1] = pt->lo[1];
+        r.hi[0] = pt->hi[0];
+        r.hi[1] = pt->hi[1];
+        apply_rule(face, pt->bc, r);
+      }
     }
   }
   // apply_volume_sources_impl, backend.rs:799-856 (single part: global ==
@@ -897,7 +957,7 @@ struct Solver {
   bool any_open() const {
     for (int k : face_kind)
       if (k) return true;
-    return false;
+    return !patches.empty();
   }
 
   // Backend::run_span default body, backend.rs:266-328 (single part,
@@ -1887,6 +1947,53 @@ int orc_set_face(void* p, int face, int kind, const double* v) {
       S->face_rho[face] = kind == 2 ? (TT)v[0] : TT(0);
       S->face_uconv[face] = kind == 4 ? (TT)v[0] : TT(0);
     }
+  });
+  return rc;
+}
+
+// Face patches (FacePatch, params.rs:286-296; validate_face_patches,
+// solver.rs:690-740): n rows of 9 doubles {face, lo1, lo2, hi1, hi2, kind,
+// v[3]} with v as for orc_set_face; replaces the list.  Returns -1 for a
+// face out of range, a rect outside the face or inverted, overlapping rects
+// on one face, or a patch on a periodic axis.
+int orc_set_patches(void* p, int n, const double* rows) {
+  Handle* h = (Handle*)p;
+  int rc = 0;
+  DISPATCH(h, {
+    using SS = std::remove_reference_t<decltype(*S)>;
+    using TT = std::remove_reference_t<decltype(S->f[0])>;
+    std::vector<typename SS::Patch> list;
+    const long ext[3] = {S->nx, S->ny, S->nz};
+    for (int k = 0; k < n && !rc; ++k) {
+      const double* r = rows + 9 * k;
+      typename SS::Patch pt{};
+      pt.face = (int)r[0];
+      const int kind = (int)r[5];
+      if (pt.face < 0 || pt.face > 5 || kind < 0 || kind > 4 || S->periodic[pt.face / 2]) {
+        rc = -1;
+        break;
+      }
+      const int a = pt.face / 2, t1 = a == 0 ? 1 : 0, t2 = a == 2 ? 1 : 2;
+      for (int c = 0; c < 2; ++c) {
+        pt.lo[c] = (long)r[1 + c];
+        pt.hi[c] = (long)r[3 + c];
+      }
+      if (pt.lo[0] < 0 || pt.lo[1] < 0 || pt.lo[0] > pt.hi[0] || pt.lo[1] > pt.hi[1] || pt.hi[0] >= ext[t1] ||
+          pt.hi[1] >= ext[t2]) {
+        rc = -1;
+        break;
+      }
+      pt.bc.kind = kind;
+      for (int c = 0; c < 3; ++c) pt.bc.u[c] = kind == 1 ? (TT)r[6 + c] : TT(0);
+      pt.bc.rho = kind == 2 ? (TT)r[6] : TT(0);
+      pt.bc.uconv = kind == 4 ? (TT)r[6] : TT(0);
+      for (const auto& o : list)
+        if (o.face == pt.face && pt.lo[0] <= o.hi[0] && o.lo[0] <= pt.hi[0] && pt.lo[1] <= o.hi[1] &&
+            o.lo[1] <= pt.hi[1])
+          rc = -1;
+      list.push_back(pt);
+    }
+    if (!rc) S->patches = list;
   });
   return rc;
 }

@@ -66,6 +66,7 @@ def lib():
         L.orc_set_omega_field.argtypes = [vp, d]
         L.orc_strain.argtypes = [vp, d]
         L.orc_set_face.argtypes = [vp, C.c_int, C.c_int, d]
+        L.orc_set_patches.argtypes = [vp, C.c_int, d]
         L.orc_set_probe.argtypes = [vp, u8]
         L.orc_set_sources.argtypes = [vp, C.c_int, d]
         L.orc_probed_force.argtypes = [vp, d]
@@ -255,6 +256,18 @@ class OracleSolver:
         v[:vv.size] = vv
         if lib().orc_set_face(self.h, face, kind, _d(v)) != 0:
             raise ValueError("open face on a periodic axis or bad face/kind")
+
+    def set_face_patches(self, patches):
+        """Face patches (FacePatch, params.rs:286-296): a list of
+        (face, (lo1, lo2), (hi1, hi2), kind, value), rects inclusive over the
+        face's tangents in ascending axis order; replaces the list."""
+        rows = np.zeros((max(len(patches), 1), 9))
+        for k, (face, lo, hi, kind, value) in enumerate(patches):
+            vv = np.atleast_1d(np.asarray(value if value is not None else 0.0, np.float64))
+            rows[k, :6] = (face, lo[0], lo[1], hi[0], hi[1], kind)
+            rows[k, 6:6 + vv.size] = vv
+        if lib().orc_set_patches(self.h, len(patches), _d(rows)) != 0:
+            raise ValueError("face patch out of range, overlapping, or on a periodic axis")
 
     def strain(self):
         """Strain rate per cell, [Sxx, Syy, Szz, Sxy, Sxz, Syz] (solver.rs:5393-5447)."""
