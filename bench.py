@@ -343,14 +343,15 @@ def workload_les(args):
     s.close()
     ms, ms0 = 1e3 * dt / args.steps, 1e3 * dt0 / args.steps
     S = STORAGE_BYTES[args.storage]
-    # the closure's algorithmic bytes: per fluid cell the moment pass reads
-    # Q populations and the mask and writes 4 moments, the WALE pass reads Q
-    # populations, the cell's moments and mask and writes omega (neighbour
-    # velocities come from L2); per solid cell both passes read the mask and
-    # write 4 moments and omega
+    # the closure's algorithmic bytes: per fluid cell the local pass reads Q
+    # populations and the mask and writes the velocity and 3 strain
+    # components, the gradient pass reads the mask, the cell's velocity and
+    # strain and writes omega (neighbour velocities come from L2); per solid
+    # cell the local pass reads the mask and writes 6 zeros, the gradient pass
+    # reads the mask and writes omega
     n_total = args.size ** 3
-    b_cell = (Q * S + 4 + 16) + (Q * S + 16 + 4 + 4)
-    b_solid = (4 + 16) + (4 + 4)
+    b_cell = (Q * S + 4 + 24) + (4 + 24 + 4)
+    b_solid = (4 + 24) + (4 + 4)
     closure_bytes = fluid * b_cell + (n_total - fluid) * b_solid
     wale_ms = ms - ms0
     print(f"[bench] les: {ms:.3f} ms/step (plain {ms0:.3f}), closure {wale_ms:.3f} ms, "

@@ -814,7 +814,7 @@ static int wale_buffers(lbm_fields* fl) {
     fl->gen++;
   }
   if (!fl->wale_mom) {
-    int rc = dev_alloc(fl, (void**)&fl->wale_mom, 4 * n * 4);
+    int rc = dev_alloc(fl, (void**)&fl->wale_mom, 6 * n * 4);
     if (rc) return rc;
   }
   if (!fl->wale_stats) {
@@ -824,16 +824,14 @@ static int wale_buffers(lbm_fields* fl) {
   return LBM_OK;
 }
 
-// Cached moments of the current state, then the omega_plus field; launches
-// only (graph-capturable: the statistics words are cleared by a kernel).
+// Cached velocity and strain of the current state, then the omega_plus
+// field; launches only (graph-capturable: the statistics words are cleared
+// by a kernel).
 static int wale_enqueue(lbm_fields* fl, float cw2, float nut_max, float* nut, uint32_t* stats) {
   hipStream_t s = fl->ctx->stream;
-  float* m = fl->wale_mom;
-  const size_t n = fl->d.N;
-  HIPCHK(fl->ctx, launch_moments(fl->Q, fl->storage, fl->parity, fl->d, fl->f, fl->mask, fl->F_mom, fl->hp, m, m + n,
-                                 m + 2 * n, m + 3 * n, s));
   HIPCHK(fl->ctx, launch_wale(fl->Q, fl->storage, fl->parity, fl->d, fl->f, fl->mask, fl->wall_u,
-                              fl->field_present ? fl->F_cur : nullptr, m, fl->hp, (float)(1.0 / fl->params.omega_p),
+                              fl->field_present ? fl->F_cur : nullptr, fl->F_mom, fl->wale_mom, fl->hp,
+                              (float)(1.0 / fl->params.omega_p),
                               cw2, (float)wale_nu(fl), nut_max, fl->omega, nut, stats, s));
   return LBM_OK;
 }

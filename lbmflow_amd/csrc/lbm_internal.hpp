@@ -160,8 +160,9 @@ hipError_t launch_uncovered(const Dom& d, const uint8_t* solid, uint32_t open_bi
 // WALE omega_plus field from the state and its cached moments `mom` (4 x N:
 // rho, ux, uy, uz) (lbm_les.hip).  `stats` (2 words, cleared here by a
 // kernel): clipped cell count, bits of the largest raw nu_t.
+// work: 6 x N floats (cached velocity, off-diagonal strain)
 hipError_t launch_wale(int Q, int storage, int odd, const Dom& d, const void* f, const uint32_t* mask,
-                       const float* wall_u, const float* F_cur, const float* mom, const HydroParams& p,
+                       const float* wall_u, const float* F_cur, const float* F_mom, float* work, const HydroParams& p,
                        float tau_eff, float cw_delta_sq, float nu0, float nut_max, float* omega, float* nut,
                        uint32_t* stats, hipStream_t s);
 hipError_t launch_build_mask(int Q, const Dom& d, const uint8_t* solid, const float* wall_u, uint32_t* mask,

@@ -112,6 +112,32 @@ def test_force_field_enters_the_stress():
     _compare(orc, hip, "forced TGV")
 
 
+@pytest.mark.parametrize("periodic", [(1, 1, 1), (0, 1, 0)])
+def test_tall_column_spans_several_z_chunks(periodic):
+    """A domain taller than the gradient pass's z chunk (64 cells) with x and
+    y extents that fill no tile: the chunk seams, the wrapped z neighbours
+    of a periodic axis and the one-sided ends of a closed one, with an
+    obstacle column crossing the seams."""
+    dims = (70, 5, 150)
+    nx, ny, nz = dims
+    solid = np.zeros((nz, ny, nx), np.uint8)
+    solid[60:70, 2, 30:33] = 1
+    solid[125:130, 1:3, 10] = 1
+    if not periodic[0]:
+        solid[:, :, 0] = solid[:, :, nx - 1] = 1
+    if not periodic[2]:
+        solid[0, :, :] = solid[nz - 1, :, :] = 1
+    solid = solid.reshape(-1)
+    rho, u = cases.tgv_field(dims, u0=0.03)
+    orc, hip = cases.pair(19, dims, nu=0.01, periodic=periodic, solid=solid)
+    orc.init_with(rho, u)
+    hip.init_with(rho, u)
+    orc.step(3)
+    hip.run(3)
+    nut, _ = _compare(orc, hip, f"tall column {periodic}")
+    assert np.sum(nut > 0) > 1000
+
+
 def test_per_step_closure_follows_the_update_loop():
     """set_wale: every step recomputes omega from the pre-collision state,
     as WaleLes::update before each Solver::step does."""
