@@ -7,10 +7,14 @@
 // expressions keep the reference's operand order and grouping; the build
 // uses -ffp-contract=off so no multiply-add is fused behind our back.
 //
-// Pins (see oracle/README-pins in tests/test_oracle_*.py):
+// Pins (tests/test_oracle_*.py):
 //   lattice invariants, omegas, equilibrium fixed point, CM rest fixed point,
 //   uniform-u uniformity, closed-box mass conservation, TGV decay, geometry
-//   counts/volumes, QOI closed forms, IBM single-marker exactness.
+//   counts/volumes, QOI closed forms, IBM single-marker exactness; WALE
+//   closed forms; open faces (imposed velocity / density, fixed points,
+//   channel flux and profile); volume-source mass and momentum; face
+//   patches (the reference's impinging-jet and full-face acceptance tests,
+//   exact per-rect rules); force-probe momentum balance.
 #include <xmmintrin.h>
 
 #include <algorithm>
