@@ -478,7 +478,7 @@ __device__ __forceinline__ void slot_st(S* p, float v) {
 template <int Q, typename S, int KIND, bool ODD, bool FON, bool WRAP, bool FILL, bool PLAIN = false>
 __device__ __forceinline__ void cell_step(const HydroArgs& a, uint32_t i, uint32_t x, uint32_t y, uint32_t z,
                                           const uint32_t* mask_ptr, uint32_t skip, const float Fm[3],
-                                          const float Fc[3]) {
+                                          const float Fc[3], uint32_t xlo = 0u, uint32_t xhi = 0u) {
   constexpr bool fon = FON;
   const Slots<S, WRAP> sl(a, i, x, y, z);
   // The mask word and the Q populations are loaded together: the loads do
@@ -500,13 +500,16 @@ __device__ __forceinline__ void cell_step(const HydroArgs& a, uint32_t i, uint32
   // cell instead of two.
 #pragma unroll
   for (int q = 0; q < Q; ++q) asm volatile("" : "+v"(fv[q]));
-  // Even steps: a solid cell of an active tile runs along with its wave and
-  // stores zeros into its own (dead) slots, so the tile's stores are whole
-  // cache lines issued by the same instructions.
-  const bool dead = FILL && !ODD && (m & kMaskSolid) != 0;
+  // A solid cell of an active tile runs along with its wave and stores zeros
+  // into dead slots (even steps: its own; odd steps, fp32 storage: those of
+  // its targets that no fluid cell writes), so the tile's stores cover whole
+  // cache lines.  With 16-bit storage the odd-step fill speeds up the next
+  // even step about as much as it slows the odd one, so it is left out.
+  constexpr bool kOddFill = sizeof(S) == 4;
+  const bool dead = FILL && (!ODD || kOddFill) && (m & kMaskSolid) != 0;
   if (m & skip & (dead ? ~kMaskSolid : ~0u)) return;
   if constexpr (ODD) {
-    if (m & kMaskLinks) {  // wall links: the bounced population is in the own slot
+    if (!dead && (m & kMaskLinks)) {  // wall links: the bounced population is in the own slot
       sfor<1, Q>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
         if ((m >> q) & 1u) fv[q] = slot_ld<S, Q, q>(sl.template at<q, 0, 0, 0>());
@@ -562,7 +565,7 @@ __device__ __forceinline__ void cell_step(const HydroArgs& a, uint32_t i, uint32
   // even steps: the bounced population goes to the plain target as well
   if constexpr (!ODD || PLAIN) {
     sfor<0, Q>(store_plain);
-  } else {
+  } else if constexpr (!FILL || !kOddFill) {
     // one exec-masked pair of stores per direction for the whole wave
     sfor<0, Q>([&](auto qc) {
       constexpr int q = decltype(qc)::value;
@@ -571,6 +574,24 @@ __device__ __forceinline__ void cell_step(const HydroArgs& a, uint32_t i, uint32
         slot_st<S, Q, q>(sl.template at<o, 0, 0, 0>(), fv[q]);
       else
         store_plain(qc);
+    });
+  } else {
+    // Odd steps of an active fp32 tile.  A wall-link lane stores the bounced
+    // population in its own slot; a solid lane writes zeros to each of its
+    // targets that no fluid cell writes (mask bit q clear), only inside the
+    // tile's live x range [xlo, xhi], so no store reaches a line outside the
+    // tile's own.  (Also writing a link lane's plain target, a dead slot,
+    // measured slower.)
+    sfor<0, Q>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      constexpr int o = lat_opp(q);
+      constexpr int cx = lat_c(q, 0);
+      const bool inr = cx == 0 || ((uint32_t)(x + cx) >= xlo && (uint32_t)(x + cx) <= xhi);
+      const bool link = !dead && ((m >> o) & 1u);
+      const bool plain = dead ? (!((m >> q) & 1u) && inr) : !link;
+      if (link) slot_st<S, Q, q>(sl.template at<o, 0, 0, 0>(), fv[q]);
+      if (plain)
+        slot_st<S, Q, q>(sl.template at<q, lat_c(q, 0), lat_c(q, 1), lat_c(q, 2)>(), dead ? 0.0f : fv[q]);
     });
   }
 }
@@ -589,11 +610,16 @@ __global__ __launch_bounds__(256) void k_hydro(HydroArgs a) {
   if (x >= a.d.nx || (te & ((threadIdx.x & 32) ? kTileDeadHi : kTileDeadLo))) return;
   const uint32_t y = row % a.d.ny, z = row / a.d.ny;
   const uint32_t i = row * a.d.nx + x;
-  if (te & kTilePlain)
+  if (te & kTilePlain) {
     cell_step<Q, S, KIND, ODD, FON, WRAP, true, true>(a, i, x, y, z, nullptr, 0u, a.p.F, a.p.F);
-  else
+  } else {
+    // the tile's live x range (an all-solid half is not visited)
+    const uint32_t x0 = seg * 64;
+    const uint32_t xlo = (te & kTileDeadLo) ? x0 + 32 : x0;
+    const uint32_t xhi = min((te & kTileDeadHi) ? x0 + 31 : x0 + 63, a.d.nx - 1);
     cell_step<Q, S, KIND, ODD, FON, WRAP, true, false>(a, i, x, y, z, a.mask + i, kMaskSolid | kMaskForced,
-                                                       a.p.F, a.p.F);
+                                                       a.p.F, a.p.F, xlo, xhi);
+  }
 }
 
 // Cells carrying a per-cell force: the velocity uses the force of the cached
@@ -646,11 +672,15 @@ __global__ __launch_bounds__(256) void k_hydro_batch(const HydroArgs* __restrict
     if (x < a.d.nx && !(te & ((threadIdx.x & 32) ? kTileDeadHi : kTileDeadLo))) {
       const uint32_t y = row % a.d.ny, z = row / a.d.ny;
       const uint32_t i = row * a.d.nx + x;
-      if (te & kTilePlain)
+      if (te & kTilePlain) {
         cell_step<Q, S, KIND, ODD, false, false, true, true>(a, i, x, y, z, nullptr, 0u, a.p.F, a.p.F);
-      else
+      } else {
+        const uint32_t x0 = seg * 64;
+        const uint32_t xlo = (te & kTileDeadLo) ? x0 + 32 : x0;
+        const uint32_t xhi = min((te & kTileDeadHi) ? x0 + 31 : x0 + 63, a.d.nx - 1);
         cell_step<Q, S, KIND, ODD, false, false, true, false>(a, i, x, y, z, a.mask + i, kMaskSolid | kMaskForced,
-                                                              a.p.F, a.p.F);
+                                                              a.p.F, a.p.F, xlo, xhi);
+      }
     }
   }
 }
@@ -870,12 +900,25 @@ __global__ void k_build_mask(Dom d, const uint8_t* solid, const float* wall_u, u
                              unsigned int* err) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= d.N) return;
-  if (solid[i]) {
-    mask[i] = kMaskSolid;
-    return;
-  }
   uint32_t x, y, z;
   cell_xyz(d, i, x, y, z);
+  if (solid[i]) {
+    // bit q: the odd-step store target of direction q (i + c_q, wrapped or
+    // linear, as either addressing of the collide-stream kernel reaches it)
+    // is a fluid cell, which writes that slot itself
+    const Nbr nb(d, x, y, z);
+    uint32_t m = kMaskSolid;
+    sfor<1, Q>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      constexpr int cx = lat_c(q, 0), cy = lat_c(q, 1), cz = lat_c(q, 2);
+      const uint32_t jw = i + (uint32_t)nb.template off<cx, cy, cz>();
+      const int64_t jl = (int64_t)i + cx + cy * (int64_t)d.nx + cz * (int64_t)d.nx * d.ny;
+      const bool fluid_target = !solid[jw] || (jl >= 0 && jl < (int64_t)d.N && !solid[jl]);
+      if (fluid_target) m |= 1u << q;
+    });
+    mask[i] = m;
+    return;
+  }
   const bool edge = (!d.per[0] && (x == 0 || x == d.nx - 1)) || (!d.per[1] && (y == 0 || y == d.ny - 1)) ||
                     (!d.per[2] && (z == 0 || z == d.nz - 1));
   if (edge) atomicAdd(err, 1u);
