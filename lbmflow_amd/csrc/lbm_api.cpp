@@ -62,7 +62,8 @@ struct lbm_fields {
   uint32_t* tiles = nullptr;
   size_t tiles_cap = 0;
   uint32_t ntiles = 0;
-  std::vector<uint32_t> tiles_host;
+  std::vector<uint32_t> tiles_host;  // the even-step visit order
+  uint32_t* tiles_odd = nullptr;     // the same tiles in the odd-step visit order
   uint32_t* flist = nullptr;
   size_t flist_cap = 0;
   // reduction scratch
@@ -228,41 +229,50 @@ int rebuild_tiles(lbm_fields* fl) {
   HIPCHK(fl->ctx, hipStreamSynchronize(s));
   dev_free(fl, dflags, nt);
   // Visit order: slabs of zb planes, y-rows within a slab, then the planes of
-  // the slab, then the x-segments.  Odd steps write a row's slots from the
-  // row itself (bounce-back) and from its upstream rows; keeping z-neighbour
-  // rows a few tiles apart lets one XCD's L2 merge those writes into whole
-  // lines before they reach memory.  That pays for 16-bit storage (a 64-cell
-  // tile is one 128-byte line per plane); with fp32 the plain row order
-  // (zb = 1) streams each plane more sequentially and measures faster.
-  uint32_t zb = fl->storage == kF32 ? 1 : 16;
-  if (const char* e = getenv("LBM_TILE_ZBLOCK")) zb = (uint32_t)std::max(1, atoi(e));
-  std::vector<uint32_t> list;
-  list.reserve(nt);
+  // the slab, then the x-segments, one order per parity.  Even steps read
+  // and write each cell's own slots and stream best in plain row order
+  // (zb = 1).  Odd steps write a row's slots from the row itself
+  // (bounce-back) and from its upstream rows; keeping z-neighbour rows a
+  // few tiles apart lets one XCD's L2 merge those writes into whole lines
+  // before they reach memory (512^3 D3Q27 tank, odd step: fp32 4.68 ms at
+  // zb = 1, 4.48 at zb = 32; fp16 3.71 at zb = 1, 3.11 at zb = 16).
+  uint32_t zb_even = 1, zb_odd = fl->storage == kF32 ? 32 : 16;
+  if (const char* e = getenv("LBM_TILE_ZBLOCK")) zb_even = zb_odd = (uint32_t)std::max(1, atoi(e));
   const uint32_t ny = fl->d.ny, nz = fl->d.nz;
-  for (uint32_t z0 = 0; z0 < nz; z0 += zb)
-    for (uint32_t y = 0; y < ny; ++y)
-      for (uint32_t z = z0; z < std::min(nz, z0 + zb); ++z)
-        for (uint32_t sg = 0; sg < nseg; ++sg) {
-          const size_t t = ((size_t)z * ny + y) * nseg + sg;
-          const uint8_t fg = flags[t];
-          if (!(fg & 1)) continue;
-          uint32_t e = (uint32_t)t | ((fg & 2) ? kTilePlain : 0u);
-          // an all-solid half of an fp32 tile is one whole line per plane
-          // nobody needs; with 16-bit storage it is half a line, and
-          // leaving it out (a 64-byte partial write) measured slower than
-          // keeping its zero stores (2.83 vs 2.75 ms on the 512^3 tank)
-          if (fl->storage == kF32) e |= ((fg & 4) ? kTileDeadLo : 0u) | ((fg & 8) ? kTileDeadHi : 0u);
-          list.push_back(e);
-        }
+  auto build = [&](uint32_t zb) {
+    std::vector<uint32_t> list;
+    list.reserve(nt);
+    for (uint32_t z0 = 0; z0 < nz; z0 += zb)
+      for (uint32_t y = 0; y < ny; ++y)
+        for (uint32_t z = z0; z < std::min(nz, z0 + zb); ++z)
+          for (uint32_t sg = 0; sg < nseg; ++sg) {
+            const size_t t = ((size_t)z * ny + y) * nseg + sg;
+            const uint8_t fg = flags[t];
+            if (!(fg & 1)) continue;
+            uint32_t e = (uint32_t)t | ((fg & 2) ? kTilePlain : 0u);
+            // an all-solid half of an fp32 tile is one whole line per plane
+            // nobody needs; with 16-bit storage it is half a line, and
+            // leaving it out (a 64-byte partial write) measured slower than
+            // keeping its zero stores (2.83 vs 2.75 ms on the 512^3 tank)
+            if (fl->storage == kF32) e |= ((fg & 4) ? kTileDeadLo : 0u) | ((fg & 8) ? kTileDeadHi : 0u);
+            list.push_back(e);
+          }
+    return list;
+  };
+  std::vector<uint32_t> list = build(zb_even);
+  const std::vector<uint32_t> list_odd = build(zb_odd);
   if (list.size() > fl->tiles_cap) {
     dev_free(fl, fl->tiles, fl->tiles_cap * 4);
-    fl->tiles = nullptr;
+    dev_free(fl, fl->tiles_odd, fl->tiles_cap * 4);
+    fl->tiles = fl->tiles_odd = nullptr;
     fl->tiles_cap = std::max<size_t>(list.size(), 1);
     rc = dev_alloc(fl, (void**)&fl->tiles, fl->tiles_cap * 4);
+    if (!rc) rc = dev_alloc(fl, (void**)&fl->tiles_odd, fl->tiles_cap * 4);
     if (rc) return rc;
   }
   if (!list.empty()) {
     HIPCHK(fl->ctx, hipMemcpyAsync(fl->tiles, list.data(), list.size() * 4, hipMemcpyHostToDevice, s));
+    HIPCHK(fl->ctx, hipMemcpyAsync(fl->tiles_odd, list_odd.data(), list_odd.size() * 4, hipMemcpyHostToDevice, s));
     HIPCHK(fl->ctx, hipStreamSynchronize(s));
   }
   fl->ntiles = (uint32_t)list.size();
@@ -367,7 +377,7 @@ HydroArgs hydro_args(lbm_fields* fl) {
   HydroArgs a{};
   a.d = fl->d;
   a.t = fl->t;
-  a.tiles = fl->tiles;
+  a.tiles = fl->parity ? fl->tiles_odd : fl->tiles;
   a.ntiles = fl->ntiles;
   a.nseg = (fl->d.nx + 63) / 64;
   a.f = fl->f;
@@ -570,7 +580,7 @@ void lbm_fields_free(lbm_fields* fl) {
   if (fl->gexec) (void)hipGraphExecDestroy(fl->gexec);
   if (fl->graph) (void)hipGraphDestroy(fl->graph);
   for (void* p : {(void*)fl->f_alloc, (void*)fl->mask, (void*)fl->solid, (void*)fl->wall_u, (void*)fl->omega,
-                  (void*)fl->F_cur, (void*)fl->F_mom, (void*)fl->flist, (void*)fl->partials, (void*)fl->tiles,
+                  (void*)fl->F_cur, (void*)fl->F_mom, (void*)fl->flist, (void*)fl->partials, (void*)fl->tiles, (void*)fl->tiles_odd,
                   (void*)fl->ring, (void*)fl->ring_pos, (void*)fl->wale_mom, (void*)fl->wale_stats})
     if (p) (void)hipFree(p);
   for (int k = 0; k < 6; ++k)
