@@ -137,8 +137,9 @@ size_t lbm_fields_device_bytes(const lbm_fields* fl);
  * cell, nonzero = solid) and the per-cell wall velocity (meaningful on
  * solids).  Either pointer may be NULL (all fluid / resting walls).
  * A domain with a fluid cell on a non-periodic face cannot be stepped
- * unless an open-face rule covers that face (lbm_set_face_bc): lbm_run_span
- * fails with LBM_ERR_UNSUPPORTED until the masks or a face rule close it. */
+ * unless an open-face rule (lbm_set_face_bc) or face patches
+ * (lbm_set_face_patches) cover that face: lbm_run_span fails with
+ * LBM_ERR_UNSUPPORTED until the masks, a face rule or patches close it. */
 int lbm_set_masks(lbm_fields* fl, const uint8_t* solid, const float* wall_u);
 int lbm_read_solid(lbm_fields* fl, uint8_t* solid);
 /* Half-way bounce-back link list (implicit in stream_row, kernels.rs:572-603):
