@@ -512,9 +512,25 @@ __device__ __forceinline__ void cell_step(const HydroArgs& a, uint32_t i, uint32
   if (m & skip & (dead ? ~kMaskSolid : ~0u)) return;
   if constexpr (ODD) {
     if (!dead && (m & kMaskLinks)) {  // wall links: the bounced population is in the own slot
+      // Every reload is issued before any is used: a 16-bit value converted
+      // inside its own conditional made each load wait for the one before
+      // (one memory latency per link direction).  So each direction issues
+      // one unconditional wave-wide load: the own slot q when some lane of
+      // the wave needs it, else the own slot 0 the step has just read (an L1
+      // hit); the choice is wave-uniform, no branch, no merge to wait for.
+      S raw[Q];
       sfor<1, Q>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        if ((m >> q) & 1u) fv[q] = slot_ld<S, Q, q>(sl.template at<q, 0, 0, 0>());
+        const bool need = __builtin_amdgcn_ballot_w64(((m >> q) & 1u) != 0u) != 0u;
+        raw[q] = *(need ? sl.template at<q, 0, 0, 0>() : sl.template at<0, 0, 0, 0>());
+      });
+      sfor<1, Q>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        if ((m >> q) & 1u) {
+          float v = Store<S>::ld(&raw[q], 0);
+          if constexpr (Store<S>::kScaled) v = v * lat_wf(Q, q);
+          fv[q] = v;
+        }
       });
     }
   }
