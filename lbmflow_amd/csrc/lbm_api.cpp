@@ -237,7 +237,10 @@ int rebuild_tiles(lbm_fields* fl) {
   // before they reach memory (512^3 D3Q27 tank, odd step: fp32 4.68 ms at
   // zb = 1, 4.48 at zb = 32; fp16 3.71 at zb = 1, 3.11 at zb = 16).
   uint32_t zb_even = 1, zb_odd = fl->storage == kF32 ? 32 : 16;
-  if (const char* e = getenv("LBM_TILE_ZBLOCK")) zb_even = zb_odd = (uint32_t)std::max(1, atoi(e));
+  if (const char* e = getenv("LBM_TILE_ZBLOCK")) {  // "n" (both parities) or "even,odd"
+    zb_even = zb_odd = (uint32_t)std::max(1, atoi(e));
+    if (const char* c = strchr(e, ',')) zb_odd = (uint32_t)std::max(1, atoi(c + 1));
+  }
   const uint32_t ny = fl->d.ny, nz = fl->d.nz;
   auto build = [&](uint32_t zb) {
     std::vector<uint32_t> list;
