@@ -110,7 +110,7 @@ struct lbm_fields {
   bool probe_built = false;
   ProbeArgs pr{};
   std::vector<std::pair<void*, size_t>> pr_allocs;
-  // WALE LES: cached-moment scratch (4 x N), diagnostics words, and the
+  // WALE LES: velocity and strain work array (6 x N), diagnostics words, and the
   // per-step mode's coefficients
   float* wale_mom = nullptr;
   uint32_t* wale_stats = nullptr;
