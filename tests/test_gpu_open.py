@@ -65,6 +65,31 @@ def test_channel_bitexact(Q, outlet):
         assert_state(orc, hip, solid, f"D3Q{Q} outlet {KIND[outlet]} step {done}")
 
 
+@pytest.mark.parametrize("outlet", [O.FACE_PRESSURE, O.FACE_CONVECTIVE])
+def test_d3q27_obstacles_next_to_open_faces_bitexact(outlet):
+    """D3Q27 fp32 (the odd-step dead-slot fill is active) with solid blocks
+    one cell from the inlet and touching the outlet face: solid lanes of
+    active tiles under the open faces' wrapped addressing, next to the
+    stale-slot capture and restore."""
+    dims = (70, 10, 6)
+    nx, ny, nz = dims
+    solid = np.zeros((nz, ny, nx), np.uint8)
+    solid[:, 0, :] = solid[:, ny - 1, :] = 1
+    solid[1:4, 3:6, 1:4] = 1             # one cell from the inlet face
+    solid[2:5, 4:8, nx - 3:nx] = 1       # reaching the outlet face
+    solid[:, 2:4, 30:34] = 1             # a pillar mid-channel
+    solid = solid.reshape(-1)
+    orc, hip = cases.pair(27, dims, collision="trt", nu=0.05, periodic=(0, 0, 1), solid=solid)
+    out_val = {O.FACE_PRESSURE: (1.001,), O.FACE_CONVECTIVE: (0.05,)}[outlet]
+    set_faces(orc, hip, [(O.XNEG, O.FACE_VELOCITY, (0.03, 0.002, 0.0)), (O.XPOS, outlet, out_val)])
+    done = 0
+    for k in (1, 1, 5, 12):
+        orc.step(k)
+        hip.run(k)
+        done += k
+        assert_state(orc, hip, solid, f"D3Q27 obstacles, outlet {KIND[outlet]} step {done}")
+
+
 def test_open_edges_and_corners_bitexact():
     """Four open faces meeting at edges (z periodic, no walls): every rule
     at once, in face order, over cells shared by two faces."""
