@@ -176,6 +176,15 @@ int lbm_set_omega_field(lbm_fields* fl, const float* omega);
  * together with open faces (LBM_ERR_UNSUPPORTED). */
 int lbm_set_face_bc(lbm_fields* fl, int face, int kind, const double value[3]);
 
+/* Solver::set_inlet_profile (solver.rs:3322-3350): a per-cell velocity on
+ * a LBM_FACE_VELOCITY face, u[3] per face cell in face order c2 * n1 + c1
+ * (t1, t2 = the face's other axes ascending; n1 = extent of t1), replacing
+ * the face's uniform u in its own Zou-He rule (not on its patches).  NULL
+ * clears.  LBM_ERR_INVALID on a face that is not a velocity face or a
+ * non-finite value.  The profile is used while the face stays a velocity
+ * face. */
+int lbm_set_inlet_profile(lbm_fields* fl, int face, const double* values);
+
 /* FacePatch (params.rs:286-296) and the patch arms of apply_open_faces_impl
  * (backend.rs:602-766): each patch gives an inclusive rect lo..hi of face
  * `face` (in-face coordinates over the two other axes in ascending order:

@@ -96,6 +96,7 @@ SIGNATURES = {
     "lbm_set_force_probe": (_I, [_P, C.POINTER(C.c_uint8)]),
     "lbm_set_volume_sources": (_I, [_P, _I, C.POINTER(VolumeSource)]),
     "lbm_set_face_patches": (_I, [_P, _I, C.POINTER(FacePatch)]),
+    "lbm_set_inlet_profile": (_I, [_P, _I, _pD]),
     "lbm_probed_force": (_I, [_P, _pD]),
     "lbm_gather_omega_field": (_I, [_P, _pF, _pI]),
     "lbm_stage_in": (_I, [_P, _pF]),

@@ -93,6 +93,7 @@ struct lbm_fields {
   // and the fluid face cells no open face covers
   int face_kind[6] = {0, 0, 0, 0, 0, 0};
   std::vector<FacePatchArgs> patches;  // rect overrides (closed ones as zero velocity)
+  float* prof[6] = {};                 // inlet profiles: u[3] per face cell, or null
   int patched[6] = {0, 0, 0, 0, 0, 0};  // faces holding at least one patch
   float face_u[6][3] = {};
   float face_rho[6] = {};
@@ -586,9 +587,11 @@ void lbm_fields_free(lbm_fields* fl) {
                   (void*)fl->F_cur, (void*)fl->F_mom, (void*)fl->flist, (void*)fl->partials, (void*)fl->tiles, (void*)fl->tiles_odd,
                   (void*)fl->ring, (void*)fl->ring_pos, (void*)fl->wale_mom, (void*)fl->wale_stats})
     if (p) (void)hipFree(p);
-  for (int k = 0; k < 6; ++k)
+  for (int k = 0; k < 6; ++k) {
     for (int g = 0; g < 2; ++g)
       if (fl->stash[k][g]) (void)hipFree(fl->stash[k][g]);
+    if (fl->prof[k]) (void)hipFree(fl->prof[k]);
+  }
   for (auto& pa : fl->pr_allocs) (void)hipFree(pa.first);
   delete fl;
 }
@@ -1173,6 +1176,37 @@ int lbm_set_face_patches(lbm_fields* fl, int n, const lbm_face_patch* patches) {
   return count_uncovered(fl);
 }
 
+// Solver::set_inlet_profile (solver.rs:3322-3350): u[3] per cell of a
+// velocity face in face order (c2 * n1 + c1); NULL clears.
+int lbm_set_inlet_profile(lbm_fields* fl, int face, const double* values) {
+  if (!fl || face < 0 || face > 5) return LBM_ERR_INVALID;
+  if (fl->face_kind[face] != LBM_FACE_VELOCITY)
+    return fail(fl->ctx, LBM_ERR_INVALID, "set_inlet_profile: face %d is not a velocity face", face);
+  const uint32_t ext[3] = {fl->d.nx, fl->d.ny, fl->d.nz};
+  const int ax = face / 2;
+  const size_t nf = (size_t)ext[ax == 0 ? 1 : 0] * ext[ax == 2 ? 1 : 2];
+  if (!values) {
+    dev_free(fl, fl->prof[face], 3 * nf * sizeof(float));
+    fl->prof[face] = nullptr;
+    fl->gen++;
+    return LBM_OK;
+  }
+  std::vector<float> h(3 * nf);
+  for (size_t k = 0; k < 3 * nf; ++k) {
+    if (!std::isfinite(values[k])) return fail(fl->ctx, LBM_ERR_INVALID, "inlet profile value %zu is not finite", k);
+    h[k] = (float)values[k];
+  }
+  if (!fl->prof[face]) {
+    int rc = dev_alloc(fl, (void**)&fl->prof[face], 3 * nf * sizeof(float));
+    if (rc) return rc;
+  }
+  hipStream_t s = fl->ctx->stream;
+  HIPCHK(fl->ctx, hipMemcpyAsync(fl->prof[face], h.data(), 3 * nf * sizeof(float), hipMemcpyHostToDevice, s));
+  HIPCHK(fl->ctx, hipStreamSynchronize(s));
+  fl->gen++;
+  return LBM_OK;
+}
+
 static int open_faces_step(lbm_fields* fl) {
   FaceArgs a{};
   a.d = fl->d;
@@ -1189,6 +1223,7 @@ static int open_faces_step(lbm_fields* fl) {
     for (int c = 0; c < 3; ++c) a.u[k][c] = fl->face_u[k][c];
     a.rho[k] = fl->face_rho[k];
     a.uconv[k] = fl->face_uconv[k];
+    a.prof[k] = fl->face_kind[k] == LBM_FACE_VELOCITY ? fl->prof[k] : nullptr;
     a.stash_cur[k] = fl->stash[k][fl->stash_sel];
     a.stash_next[k] = fl->stash[k][fl->stash_sel ^ 1];
   }

@@ -101,6 +101,7 @@ struct FaceArgs {
   float* stash_next[6];  // this step's, captured before the restore
   int npatch;            // rect overrides, at most one per face cell
   FacePatchArgs patch[kMaxFacePatches];
+  const float* prof[6];  // per face: u[3] per face cell (inlet profile) or null
 };
 
 // One volume source (lbm_open.hip): inclusive box, q_lu per step, velocity u

@@ -258,6 +258,7 @@ __device__ __forceinline__ void bc_cell(const FaceArgs& a, uint32_t coord) {
   int kind = a.kind[FACE];
   float ub[3] = {a.u[FACE][0], a.u[FACE][1], a.u[FACE][2]};
   float rho_bc = a.rho[FACE], lam = a.uconv[FACE];
+  bool own_rule = true;
   if (a.npatch) {
     const uint32_t ext[3] = {a.d.nx, a.d.ny, a.d.nz};
     const uint32_t c1 = coord % ext[t1], c2 = coord / ext[t1];
@@ -270,9 +271,18 @@ __device__ __forceinline__ void bc_cell(const FaceArgs& a, uint32_t coord) {
         ub[2] = pt.u[2];
         rho_bc = pt.rho;
         lam = pt.uconv;
+        own_rule = false;
         break;
       }
     }
+  }
+  // the inlet profile replaces u on the face's own velocity rule
+  // (profile.map_or(u, |p| p[coord]), kernels.rs:1021-1034, 1164-1180)
+  if (own_rule && kind == kFaceVelocity && a.prof[FACE]) {
+    const float* pu = a.prof[FACE] + 3 * (size_t)coord;
+    ub[0] = pu[0];
+    ub[1] = pu[1];
+    ub[2] = pu[2];
   }
   if (kind == kFaceVelocity || kind == kFacePressure) {
     float f0[3] = {a.p.F[0], a.p.F[1], a.p.F[2]};

@@ -276,6 +276,31 @@ class Solver:
             v[:vv.size] = vv
         self._chk(self.lib.lbm_set_face_bc(self.h, f, self.FACE_KINDS[kind], _dp(v)))
 
+    def set_inlet_profile(self, face, values):
+        """``Solver::set_inlet_profile`` (solver.rs:3322-3350): ``values`` of
+        shape (face cells, 3) in face order (c2 * n1 + c1, t1 and t2 the
+        face's other axes ascending) on a velocity face; None clears."""
+        f = self.FACES[face] if isinstance(face, str) else int(face)
+        if values is None:
+            self._chk(self.lib.lbm_set_inlet_profile(self.h, f, None))
+            return
+        v = np.ascontiguousarray(values, np.float64).reshape(-1)
+        ext = self.dims
+        ax = f // 2
+        t1, t2 = (1, 2) if ax == 0 else ((0, 2) if ax == 1 else (0, 1))
+        if v.size != 3 * ext[t1] * ext[t2]:
+            raise ValueError("profile must cover the whole face")
+        self._chk(self.lib.lbm_set_inlet_profile(self.h, f, _dp(v)))
+
+    def set_inlet_profile_with(self, face, profile):
+        """``Solver::set_inlet_profile_with`` (solver.rs:3352-3366):
+        ``profile(c1, c2)`` at every face cell's tangent coordinates."""
+        f = self.FACES[face] if isinstance(face, str) else int(face)
+        ax = f // 2
+        t1, t2 = (1, 2) if ax == 0 else ((0, 2) if ax == 1 else (0, 1))
+        vals = [profile(c1, c2) for c2 in range(self.dims[t2]) for c1 in range(self.dims[t1])]
+        self.set_inlet_profile(f, np.asarray(vals, np.float64))
+
     def set_face_patches(self, patches):
         """``GlobalSpec::face_patches`` (FacePatch, params.rs:286-296): a list
         of (face, (lo1, lo2), (hi1, hi2), kind, value) with inclusive rects

@@ -257,6 +257,9 @@ struct Solver {
     Rule bc;
   };
   std::vector<Patch> patches;
+  // inlet profiles (FieldSet::inlet_profiles, fields.rs:267): per velocity
+  // face, u[3] per face cell in face order (c2 * n1 + c1); empty = none
+  std::vector<T> profile[6];
 
   size_t pidx(long x, long y, long z) const {
     return ((size_t)(z + 1) * py + (size_t)(y + 1)) * px + (size_t)(x + 1);
@@ -711,12 +714,14 @@ struct Solver {
     }
   };
   // for_face_cells_selected (kernels.rs:742-766): t2 outer, t1 inner
+  long coord_ = 0;  // face coordinate of the cell for_face_cells visits
   template <class F>
   void for_face_cells(const FaceGeo& g, const Sel& sel, F&& body) {
     const long n[3] = {nx, ny, nz};
     for (long c2 = 0; c2 < n[g.t2]; ++c2)
       for (long c1 = 0; c1 < n[g.t1]; ++c1) {
         if (!sel(c1, c2)) continue;
+        coord_ = c2 * n[g.t1] + c1;
         long pos[3];
         pos[g.a] = g.fixed;
         pos[g.t1] = c1;
@@ -728,7 +733,7 @@ struct Solver {
   // zou_he_face_3d (D3Q19, kernels.rs:1085-1219) and zou_he_face_d3q27
   // (kernels.rs:958-1083); kind 1 velocity, 2 pressure.  Gravity is not on
   // this path (g = 0); F0 = params force + field (force_independent_at).
-  void zou_he_face(int face, const Rule& bc, const Sel& sel) {
+  void zou_he_face(int face, const Rule& bc, const Sel& sel, const T* prof = nullptr) {
     const FaceGeo g = face_geo(face);
     const int a = g.a, t1 = g.t1, t2 = g.t2;
     const T nsign = T(g.nin[a]);
@@ -763,7 +768,9 @@ struct Solver {
         }
         const T closure = s0 + two * sneg + one;
         T r, vn, vt1, vt2;
-        zh_solve(vel, ub, rho_bc, closure, f0, a, t1, t2, nsign, r, vn, vt1, vt2);
+        // profile.map_or(u, |p| p[coord]) (kernels.rs:1021-1034)
+        const T* uc = (vel && prof) ? prof + 3 * coord_ : ub;
+        zh_solve(vel, uc, rho_bc, closure, f0, a, t1, t2, nsign, r, vn, vt1, vt2);
         const T c_t1 = r * vt1 * (one - six * s2_t1) - q0a;
         const T c_t2 = r * vt2 * (one - six * s2_t2) - q0b;
         for (int k = 0; k < g.nunk; ++k) {
@@ -802,7 +809,8 @@ struct Solver {
       const T sneg = F(L.opp[q_n]) + F(L.opp[q_p1]) + F(L.opp[q_m1]) + F(L.opp[q_p2]) + F(L.opp[q_m2]);
       const T closure = s0 + two * sneg + one;
       T r, vn, vt1, vt2;
-      zh_solve(vel, ub, rho_bc, closure, f0, a, t1, t2, nsign, r, vn, vt1, vt2);
+      const T* uc = (vel && prof) ? prof + 3 * coord_ : ub;  // kernels.rs:1164-1180
+      zh_solve(vel, uc, rho_bc, closure, f0, a, t1, t2, nsign, r, vn, vt1, vt2);
       const T qt1 = F(q_t1) - F(q_mt1) + F(q_pp) + F(q_pm) - F(q_mp) - F(q_mm);
       const T qt2 = F(q_t2) - F(q_mt2) + F(q_pp) - F(q_pm) + F(q_mp) - F(q_mm);
       const T n1 = c13 * r * vt1 - half * qt1;
@@ -888,12 +896,12 @@ struct Solver {
       }
     });
   }
-  void apply_rule(int face, const Rule& bc, const Sel& sel) {
+  void apply_rule(int face, const Rule& bc, const Sel& sel, const T* prof = nullptr) {
     if (bc.kind == 0) {  // a Closed patch: impermeable lid, zero-velocity Zou-He
       Rule z{1, {T(0), T(0), T(0)}, T(0), T(0)};
       zou_he_face(face, z, sel);
     } else if (bc.kind == 1 || bc.kind == 2) {
-      zou_he_face(face, bc, sel);
+      zou_he_face(face, bc, sel, prof);
     } else if (bc.kind == 3) {
       outflow_face(face, sel);
     } else if (bc.kind == 4) {
@@ -920,7 +928,9 @@ struct Solver {
       excl.kind = 2;
       excl.ex = &ex;
       Rule base{k, {face_u[face][0], face_u[face][1], face_u[face][2]}, face_rho[face], face_uconv[face]};
-      apply_rule(face, base, excl);  // kind 0 here: the closed base with patches
+      // kind 0 here: the closed base with patches; the inlet profile only
+      // on the face's own velocity rule (backend.rs:664-675)
+      apply_rule(face, base, excl, profile[face].empty() ? nullptr : profile[face].data());
       for (const Patch* pt : mine) {
         Sel r;
         r.kind = 1;
@@ -1998,6 +2008,30 @@ int orc_set_patches(void* p, int n, const double* rows) {
       list.push_back(pt);
     }
     if (!rc) S->patches = list;
+  });
+  return rc;
+}
+
+// Inlet profile of a velocity face (Solver::set_inlet_profile,
+// solver.rs:3322-3350): values = 3 x (face cells) in face order
+// (c2 * n1 + c1), or NULL to clear.  Returns -1 unless the face is a
+// velocity face.
+int orc_set_inlet_profile(void* p, int face, const double* values) {
+  Handle* h = (Handle*)p;
+  int rc = 0;
+  DISPATCH(h, {
+    using TT = std::remove_reference_t<decltype(S->f[0])>;
+    if (face < 0 || face > 5 || S->face_kind[face] != 1) {
+      rc = -1;
+    } else if (!values) {
+      S->profile[face].clear();
+    } else {
+      const long ext[3] = {S->nx, S->ny, S->nz};
+      const int a = face / 2, t1 = a == 0 ? 1 : 0, t2 = a == 2 ? 1 : 2;
+      const long nf = ext[t1] * ext[t2];
+      S->profile[face].assign(3 * nf, TT(0));
+      for (long k = 0; k < 3 * nf; ++k) S->profile[face][k] = (TT)values[k];
+    }
   });
   return rc;
 }

@@ -67,6 +67,7 @@ def lib():
         L.orc_strain.argtypes = [vp, d]
         L.orc_set_face.argtypes = [vp, C.c_int, C.c_int, d]
         L.orc_set_patches.argtypes = [vp, C.c_int, d]
+        L.orc_set_inlet_profile.argtypes = [vp, C.c_int, d]
         L.orc_set_probe.argtypes = [vp, u8]
         L.orc_set_sources.argtypes = [vp, C.c_int, d]
         L.orc_probed_force.argtypes = [vp, d]
@@ -256,6 +257,18 @@ class OracleSolver:
         v[:vv.size] = vv
         if lib().orc_set_face(self.h, face, kind, _d(v)) != 0:
             raise ValueError("open face on a periodic axis or bad face/kind")
+
+    def set_inlet_profile(self, face, values):
+        """Solver::set_inlet_profile (solver.rs:3322-3350): values (face
+        cells, 3) in face order (c2 * n1 + c1) on a velocity face; None
+        clears."""
+        if values is None:
+            rc = lib().orc_set_inlet_profile(self.h, face, None)
+        else:
+            v = np.ascontiguousarray(values, np.float64).reshape(-1)
+            rc = lib().orc_set_inlet_profile(self.h, face, _d(v))
+        if rc != 0:
+            raise ValueError("inlet profile on a face that is not a velocity face")
 
     def set_face_patches(self, patches):
         """Face patches (FacePatch, params.rs:286-296): a list of

@@ -201,3 +201,59 @@ def test_refusals():
     with pytest.raises(L.LbmError):
         s.run(1)
     s.close()
+
+
+def duct_profile(n1, n2, umax=0.03):
+    c1 = np.arange(n1) + 0.5
+    c2 = np.arange(n2) + 0.5
+    v = np.zeros((n2, n1, 3))
+    v[:, :, 0] = umax * np.outer(4 * c2 * (n2 - c2) / n2**2, 4 * c1 * (n1 - c1) / n1**2)
+    v[:, :, 2] = 0.002 * np.outer(np.ones(n2), np.linspace(-1, 1, n1))
+    return v.reshape(-1, 3).astype(np.float32).astype(np.float64)
+
+
+@pytest.mark.parametrize("Q", [19, 27])
+def test_inlet_profile_bitexact(Q):
+    """set_inlet_profile (solver.rs:3322-3350): a duct profile on the inlet,
+    with a patch on the same face taking its own rule over its rect."""
+    dims = (20, 10, 6)
+    nx, ny, nz = dims
+    solid = channel_solid(dims)
+    orc, hip = cases.pair(Q, dims, collision="trt", nu=0.05, periodic=(0, 0, 1), solid=solid)
+    set_faces(orc, hip, [(O.XNEG, O.FACE_VELOCITY, (0.01, 0.0, 0.0)), (O.XPOS, O.FACE_PRESSURE, (1.0,))])
+    prof = duct_profile(ny, nz)
+    orc.set_inlet_profile(O.XNEG, prof)
+    hip.set_inlet_profile("x-", prof)
+    patch = [(O.XNEG, (3, 1), (4, 3), O.FACE_VELOCITY, (0.02, 0.0, 0.0))]
+    orc.set_face_patches(patch)
+    hip.set_face_patches([("x-", (3, 1), (4, 3), "velocity", (0.02, 0.0, 0.0))])
+    done = 0
+    for k in (1, 1, 6, 11):
+        orc.step(k)
+        hip.run(k)
+        done += k
+        assert_state(orc, hip, solid, f"D3Q{Q} inlet profile step {done}")
+    hip.set_inlet_profile("x-", None)
+    orc.set_inlet_profile(O.XNEG, None)
+    orc.step(3)
+    hip.run(3)
+    assert_state(orc, hip, solid, f"D3Q{Q} profile cleared")
+
+
+def test_inlet_profile_refusals():
+    dims = (10, 6, 4)
+    s = L.Solver(19, dims, nu=0.05, periodic=(False, True, True))
+    s.set_face_bc("x-", "pressure", (1.0,))
+    with pytest.raises(L.LbmError):  # not a velocity face
+        s.set_inlet_profile("x-", np.zeros((24, 3)))
+    s.set_face_bc("x-", "velocity", (0.01, 0.0, 0.0))
+    with pytest.raises(ValueError):  # wrong size
+        s.set_inlet_profile("x-", np.zeros((23, 3)))
+    bad = np.zeros((24, 3))
+    bad[5, 0] = np.nan
+    with pytest.raises(L.LbmError):
+        s.set_inlet_profile("x-", bad)
+    s.set_inlet_profile_with("x-", lambda c1, c2: (0.01 * (c1 + 1) / 6, 0.0, 0.0))
+    s.set_face_bc("x+", "outflow")
+    s.run(4)
+    s.close()

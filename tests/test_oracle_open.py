@@ -115,3 +115,56 @@ def test_open_face_on_periodic_axis_is_refused():
     o, _ = channel(19, (6, 6, 6), periodic=(1, 0, 0))
     with pytest.raises(ValueError):
         o.set_face(O.XNEG, O.FACE_OUTFLOW)
+
+
+def duct_profile(n1, n2, umax=0.03):
+    """u_x(c1, c2) = umax f(c1) g(c2) on an x face, face order c2 * n1 + c1."""
+    c1 = np.arange(n1) + 0.5
+    c2 = np.arange(n2) + 0.5
+    f = 4 * c1 * (n1 - c1) / n1**2
+    g = 4 * c2 * (n2 - c2) / n2**2
+    v = np.zeros((n2, n1, 3))
+    v[:, :, 0] = umax * np.outer(g, f)
+    v[:, :, 1] = 0.002 * np.outer(g, np.ones(n1))
+    return v.reshape(-1, 3)
+
+
+@pytest.mark.parametrize("Q", [19, 27])
+@pytest.mark.parametrize("force", [(0.0, 0.0, 0.0), (2e-6, -1e-6, 5e-7)])
+def test_inlet_profile_is_imposed_per_cell(Q, force):
+    """set_inlet_profile (solver.rs:3322-3350; kernels.rs:1021-1034,
+    1164-1180): each face cell of the velocity face takes its own u."""
+    dims = (12, 8, 6)
+    nx, ny, nz = dims
+    o, _ = channel(Q, dims, force=force, walls=False, periodic=(0, 1, 1))
+    prof = duct_profile(ny, nz)
+    o.set_face(O.XNEG, O.FACE_VELOCITY, (0.01, 0.0, 0.0))
+    o.set_inlet_profile(O.XNEG, prof)
+    o.set_face(O.XPOS, O.FACE_PRESSURE, (1.0,))
+    want = prof.reshape(nz, ny, 3)
+    for _ in range(3):
+        o.step(7)
+        _, a, b, c = face_moments(o, dims, 0)
+        for k, m in enumerate((a, b, c)):
+            np.testing.assert_allclose(m, want[:, :, k], rtol=0, atol=1e-15)
+
+
+def test_constant_profile_equals_the_plain_face():
+    dims = (10, 6, 5)
+    runs = []
+    for with_profile in (False, True):
+        o, _ = channel(19, dims, walls=False, periodic=(0, 1, 1))
+        o.set_face(O.XNEG, O.FACE_VELOCITY, (0.02, 0.001, -0.003))
+        if with_profile:
+            o.set_inlet_profile(O.XNEG, np.tile((0.02, 0.001, -0.003), (dims[1] * dims[2], 1)))
+        o.set_face(O.XPOS, O.FACE_OUTFLOW)
+        o.step(15)
+        runs.append(o.get_f())
+    assert np.array_equal(runs[0], runs[1])
+
+
+def test_inlet_profile_needs_a_velocity_face():
+    o, _ = channel(19, (8, 6, 4), walls=False, periodic=(0, 1, 1))
+    o.set_face(O.XNEG, O.FACE_PRESSURE, (1.0,))
+    with pytest.raises(ValueError):
+        o.set_inlet_profile(O.XNEG, np.zeros((24, 3)))
