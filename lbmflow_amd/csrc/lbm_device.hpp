@@ -175,9 +175,11 @@ template <int Q, typename S, int q>
 __device__ __forceinline__ float read_pop(const S* base, size_t ps, const Nbr& nb, uint32_t i, uint32_t m,
                                           bool odd) {
   constexpr int o = lat_opp(q);
-  if (!odd || ((m >> q) & 1u)) return pop_ld<S, Q, q>(base + (size_t)q * ps, i);
-  const uint32_t s = i + (uint32_t)nb.template off<-lat_c(q, 0), -lat_c(q, 1), -lat_c(q, 2)>();
-  return pop_ld<S, Q, q>(base + (size_t)o * ps, s);
+  // one load from a selected address: with 16-bit storage a load converted
+  // inside either arm of a branch makes the caller's loads wait one by one
+  const bool own = !odd || ((m >> q) & 1u);
+  const uint32_t s = own ? i : i + (uint32_t)nb.template off<-lat_c(q, 0), -lat_c(q, 1), -lat_c(q, 2)>();
+  return pop_ld<S, Q, q>(base + (size_t)(own ? q : o) * ps, s);
 }
 
 // write population q of cell i where read_pop finds it (the inverse map)
