@@ -100,6 +100,7 @@ struct IbmState {
   double* du = nullptr;      // U x 3
   double* spread = nullptr;  // U x 3
   double* imp = nullptr;     // M x 3
+  double* imp_next = nullptr;  // M x 3: the next sweep's impulses, computed with this sweep's slip
   double* mrep = nullptr;    // M x 3
   double* mtq = nullptr;     // M x 3
   double* mslip = nullptr;   // M
@@ -209,11 +210,15 @@ __global__ void k_ibm_spread(uint32_t U, const uint32_t* uc_off, const uint32_t*
   }
 }
 
-// per marker: represented force, torque contribution, post-sweep slip
+// per marker: represented force, torque contribution, post-sweep slip; with
+// imp_next, also the next sweep's impulse from the same interpolated
+// velocity (what k_ibm_impulse of the next sweep computes, operation for
+// operation), which k_ibm_reduce installs if the instance goes on
 __global__ void k_ibm_marker_diag(int64_t M, const uint32_t* mk_off, const uint32_t* mk_inst, const uint32_t* pt_u,
                                   const double* pt_w, const double* pos, const double* target, const double* unow,
                                   const double* du, const double* imp, const double* cen, double* mrep, double* mtq,
-                                  double* mslip, uint8_t* mact, const IbmDevDiag* diag) {
+                                  double* mslip, uint8_t* mact, const IbmDevDiag* diag, const double* rho,
+                                  const double* ksum, double relax, double* imp_next) {
   const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= M) return;
   const uint32_t k = mk_inst[m];
@@ -223,11 +228,18 @@ __global__ void k_ibm_marker_diag(int64_t M, const uint32_t* mk_off, const uint3
     const double w = pt_w[p];
     for (int a = 0; a < 3; ++a) rep[a] += imp[3 * m + a] * w;
   }
+  double mob = 0.0;
   for (uint32_t p = mk_off[m]; p < mk_off[m + 1]; ++p) {
     const uint32_t u = pt_u[p];
     const double w = pt_w[p];
     for (int a = 0; a < 3; ++a) um[a] += w * (unow[3 * (size_t)u + a] + du[3 * (size_t)u + a]);
+    if (imp_next) mob += w * ksum[u] / fmax(rho[u], 1.0e-30);
   }
+  if (imp_next)
+    for (int a = 0; a < 3; ++a) {
+      const double slip = target[3 * m + a] - um[a];
+      imp_next[3 * m + a] = mob == 0.0 ? 0.0 : relax * slip / mob;
+    }
   const double r[3] = {pos[3 * m] - cen[3 * k], pos[3 * m + 1] - cen[3 * k + 1], pos[3 * m + 2] - cen[3 * k + 2]};
   mtq[3 * m + 0] = r[1] * rep[2] - r[2] * rep[1];
   mtq[3 * m + 1] = r[2] * rep[0] - r[0] * rep[2];
@@ -242,7 +254,8 @@ __global__ void k_ibm_marker_diag(int64_t M, const uint32_t* mk_off, const uint3
 // convergence test
 __global__ __launch_bounds__(256) void k_ibm_reduce(const uint32_t* moff, const double* mrep, const double* mtq,
                                                     const double* mslip, const uint8_t* mact, const double* weight,
-                                                    int iter, const double* scale, double tol, IbmDevDiag* diags) {
+                                                    int iter, const double* scale, double tol, IbmDevDiag* diags,
+                                                    const double* imp_next, double* imp) {
   __shared__ double sh[9][256];
   __shared__ int conv;
   const int k = blockIdx.x;
@@ -271,24 +284,30 @@ __global__ __launch_bounds__(256) void k_ibm_reduce(const uint32_t* moff, const 
                                     : sh[j][threadIdx.x] + sh[j][threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x != 0) return;
-  for (int a = 0; a < 3; ++a) {
-    diag->torque[a] -= sh[a][0];
-    diag->marker_force[a] += sh[3 + a][0];
-    diag->fluid_force[a] += sh[3 + a][0];
+  if (threadIdx.x == 0) {
+    for (int a = 0; a < 3; ++a) {
+      diag->torque[a] -= sh[a][0];
+      diag->marker_force[a] += sh[3 + a][0];
+      diag->fluid_force[a] += sh[3 + a][0];
+    }
+    diag->iterations = iter + 1;
+    diag->slip_max = sh[6][0];
+    diag->slip_rms = sh[8][0] > 0.0 ? sqrt(sh[7][0] / sh[8][0]) : 0.0;
+    diag->slip_max_rel = diag->slip_max / scale[k];
+    diag->slip_rms_rel = diag->slip_rms / scale[k];
+    const double* ff = diag->fluid_force;
+    const double* mf = diag->marker_force;
+    const double e[3] = {ff[0] - mf[0], ff[1] - mf[1], ff[2] - mf[2]};
+    const double ne = sqrt(0.0 + e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
+    const double nm = sqrt(0.0 + mf[0] * mf[0] + mf[1] * mf[1] + mf[2] * mf[2]);
+    diag->mom_err = ne / fmax(nm, 1.0e-12);
+    if (diag->slip_max_rel <= tol) diag->converged = 1;
+    conv = diag->converged;
   }
-  diag->iterations = iter + 1;
-  diag->slip_max = sh[6][0];
-  diag->slip_rms = sh[8][0] > 0.0 ? sqrt(sh[7][0] / sh[8][0]) : 0.0;
-  diag->slip_max_rel = diag->slip_max / scale[k];
-  diag->slip_rms_rel = diag->slip_rms / scale[k];
-  const double* ff = diag->fluid_force;
-  const double* mf = diag->marker_force;
-  const double e[3] = {ff[0] - mf[0], ff[1] - mf[1], ff[2] - mf[2]};
-  const double ne = sqrt(0.0 + e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
-  const double nm = sqrt(0.0 + mf[0] * mf[0] + mf[1] * mf[1] + mf[2] * mf[2]);
-  diag->mom_err = ne / fmax(nm, 1.0e-12);
-  if (diag->slip_max_rel <= tol) diag->converged = 1;
+  // an instance that goes on takes the next sweep's impulses
+  __syncthreads();
+  if (imp_next && !conv)
+    for (int64_t e = 3 * (int64_t)moff[k] + threadIdx.x; e < 3 * (int64_t)moff[k + 1]; e += 256) imp[e] = imp_next[e];
 }
 
 // force_field += T(spread)
@@ -365,7 +384,7 @@ void ibm_destroy(IbmState* s) {
   if (!s) return;
   void* ptrs[] = {s->moff,    s->cen,    s->scale,   s->dom,    s->pos,    s->weight, s->target, s->mk_off,
                   s->mk_inst, s->pt_u,   s->pt_m,    s->pt_w,   s->uc_cell, s->uc_inst, s->uc_off, s->uc_ent,
-                  s->ksum,    s->unow,   s->rho,     s->du,     s->spread, s->imp,    s->mrep,   s->mtq,
+                  s->ksum,    s->unow,   s->rho,     s->du,     s->spread, s->imp,    s->imp_next, s->mrep, s->mtq,
                   s->mslip,   s->mact,   s->diag,    s->imp_hist, s->du_hist, s->a_T, s->a_um, s->a_mob,
                   s->a_du,    s->a_unow, s->a_rho};
   for (void* p : ptrs)
@@ -439,6 +458,7 @@ static IbmState* ibm_build(const std::vector<const IbmHost*>& parts, const lbm_i
   chain(scratch(&s->du, 3 * (size_t)s->U));
   chain(scratch(&s->spread, 3 * (size_t)s->U));
   chain(scratch(&s->imp, 3 * (size_t)s->M));
+  chain(scratch(&s->imp_next, 3 * (size_t)s->M));
   chain(scratch(&s->mrep, 3 * (size_t)s->M));
   chain(scratch(&s->mtq, 3 * (size_t)s->M));
   chain(scratch(&s->mslip, s->M));
@@ -643,8 +663,10 @@ static int ibm_enqueue_impl(const FieldsView& v, IbmState* s, void* ring, uint32
     if (record && s->U)
       hipLaunchKernelGGL(k_copy_d, dim3(nblk(3 * (size_t)s->U)), dim3(256), 0, st,
                          s->du_hist + 3 * (size_t)s->U * it, s->du, 3 * (size_t)s->U);
-    hipLaunchKernelGGL(k_ibm_impulse, dim3(gm), dim3(256), 0, st, s->M, s->mk_off, s->mk_inst, s->pt_u, s->pt_w,
-                       s->target, s->unow, s->du, s->rho, s->ksum, s->cfg.relaxation, s->imp, s->diag);
+    if (it == 0)  // later sweeps take the impulses the previous sweep's diagnostics computed
+      hipLaunchKernelGGL(k_ibm_impulse, dim3(gm), dim3(256), 0, st, s->M, s->mk_off, s->mk_inst, s->pt_u, s->pt_w,
+                         s->target, s->unow, s->du, s->rho, s->ksum, s->cfg.relaxation, s->imp, s->diag);
+    double* nxt = it + 1 < s->cfg.max_iterations ? s->imp_next : nullptr;
     if (record)
       hipLaunchKernelGGL(k_copy_d, dim3(nblk(3 * (size_t)s->M)), dim3(256), 0, st,
                          s->imp_hist + 3 * (size_t)s->M * it, s->imp, 3 * (size_t)s->M);
@@ -653,9 +675,9 @@ static int ibm_enqueue_impl(const FieldsView& v, IbmState* s, void* ring, uint32
                          s->pt_w, s->imp, s->rho, s->spread, s->du, s->diag);
     hipLaunchKernelGGL(k_ibm_marker_diag, dim3(gm), dim3(256), 0, st, s->M, s->mk_off, s->mk_inst, s->pt_u, s->pt_w,
                        s->pos, s->target, s->unow, s->du, s->imp, s->cen, s->mrep, s->mtq, s->mslip, s->mact,
-                       s->diag);
+                       s->diag, s->rho, s->ksum, s->cfg.relaxation, nxt);
     hipLaunchKernelGGL(k_ibm_reduce, dim3(s->K), dim3(256), 0, st, s->moff, s->mrep, s->mtq, s->mslip, s->mact,
-                       s->weight, it, s->scale, s->cfg.slip_tolerance, s->diag);
+                       s->weight, it, s->scale, s->cfg.slip_tolerance, s->diag, nxt, s->imp);
     AUXCHK(v, hipGetLastError());
   }
   if (s->U) {
