@@ -2217,16 +2217,23 @@ int lbm_adjoint_torque(lbm_fields* fl, int steps, int window, int snapshots, dou
   revolve(0, steps, 0, snapshots, plan);
   int64_t advanced = steps;
   const double tq_bar = 1.0 / window;
+  // the snapshot slot the device state still equals (the schedule restores
+  // the slot it has just saved; that copy back is skipped), -1 for none
+  int held = -1;
   for (size_t a = 0; a < plan.size(); a += 2) {
     const int op = plan[a], arg = plan[a + 1];
     if (op == ACT_RESTORE) {
-      ADJ_TRY(snap_restore(fl, snap[arg]));
+      if (arg != held) ADJ_TRY(snap_restore(fl, snap[arg]));
+      held = arg;
     } else if (op == ACT_SAVE) {
       ADJ_TRY(snap_save(fl, snap[arg]));
+      held = arg;
     } else if (op == ACT_ADVANCE) {
+      held = -1;
       for (int k = 0; k < arg; ++k) ADJ_TRY(ibm_step(fl));
       advanced += arg;
     } else {  // adjoint of step `arg`; the solver holds the state before it
+      held = -1;
       if (!fl->flist_host.empty())
         HIPCHK(fl->ctx, launch_copy_force_list(fl->flist, (uint32_t)fl->flist_host.size(), fl->F_cur, nullptr, s));
       fl->field_present = false;
