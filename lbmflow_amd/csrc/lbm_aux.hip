@@ -121,10 +121,14 @@ struct IbmState {
 // u_now = (m + F/2) / rho at the stencil cells, rho the cached density
 // (solver.rs:3040-3084: f64 momentum of the populations, force without the
 // just-cleared field).
+// With reset_K > 0 the first reset_K threads also clear the instances'
+// diagnostics (no other kernel of the gather reads them).
 template <int Q, typename S>
 __global__ void k_ibm_gather(const IbmDomain* dom, int odd, const uint32_t* uc_cell, const uint32_t* uc_inst,
-                             uint32_t U, int field_present, double* unow, double* rho, double* du, double* spread) {
+                             uint32_t U, int field_present, double* unow, double* rho, double* du, double* spread,
+                             IbmDevDiag* diag, uint32_t reset_K) {
   const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+  if (u < reset_K) memset(diag + u, 0, sizeof(IbmDevDiag));
   if (u >= U) return;
   const IbmDomain& D = dom[uc_inst[u]];
   const Dom d = D.d;
@@ -601,9 +605,9 @@ IbmDomain ibm_domain_of(const FieldsView& v) {
 }
 
 template <int Q, typename S>
-static void gather_q(const FieldsView& v, IbmState* s) {
+static void gather_q(const FieldsView& v, IbmState* s, uint32_t reset_K) {
   hipLaunchKernelGGL((k_ibm_gather<Q, S>), dim3(nblk(s->U)), dim3(256), 0, v.stream, s->dom, v.parity, s->uc_cell,
-                     s->uc_inst, s->U, v.field_present ? 1 : 0, s->unow, s->rho, s->du, s->spread);
+                     s->uc_inst, s->U, v.field_present ? 1 : 0, s->unow, s->rho, s->du, s->spread, s->diag, reset_K);
 }
 
 static void diag_to_abi(const IbmDevDiag& h, lbm_ibm_diag* out) {
@@ -644,20 +648,24 @@ static int ibm_enqueue_impl(const FieldsView& v, IbmState* s, void* ring, uint32
       return LBM_ERR_STATE;
     }
   hipStream_t st = v.stream;
+  // the gather clears the diagnostics when it has a thread per instance
+  const uint32_t reset_K = s->U >= (uint32_t)s->K ? (uint32_t)s->K : 0u;
   if (s->U) {
     if (v.Q == 19) {
-      if (v.storage == kF32) gather_q<19, float>(v, s);
-      else if (v.storage == kF16) gather_q<19, _Float16>(v, s);
-      else gather_q<19, bf16s>(v, s);
+      if (v.storage == kF32) gather_q<19, float>(v, s, reset_K);
+      else if (v.storage == kF16) gather_q<19, _Float16>(v, s, reset_K);
+      else gather_q<19, bf16s>(v, s, reset_K);
     } else {
-      if (v.storage == kF32) gather_q<27, float>(v, s);
-      else if (v.storage == kF16) gather_q<27, _Float16>(v, s);
-      else gather_q<27, bf16s>(v, s);
+      if (v.storage == kF32) gather_q<27, float>(v, s, reset_K);
+      else if (v.storage == kF16) gather_q<27, _Float16>(v, s, reset_K);
+      else gather_q<27, bf16s>(v, s, reset_K);
     }
     AUXCHK(v, hipGetLastError());
   }
-  hipLaunchKernelGGL(k_ibm_reset, dim3(nblk(s->K, 64)), dim3(64), 0, st, s->diag, s->K);
-  AUXCHK(v, hipGetLastError());
+  if (!s->U || !reset_K) {
+    hipLaunchKernelGGL(k_ibm_reset, dim3(nblk(s->K, 64)), dim3(64), 0, st, s->diag, s->K);
+    AUXCHK(v, hipGetLastError());
+  }
   const uint32_t gm = nblk((size_t)s->M), gu = nblk(s->U);
   for (int it = 0; it < s->cfg.max_iterations; ++it) {
     if (record && s->U)
