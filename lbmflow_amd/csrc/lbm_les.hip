@@ -12,6 +12,9 @@
 //     (gather_velocity_gradient, solver.rs:5490-5581), then
 //     nu_t = (Cw Delta)^2 (Sd:Sd)^(3/2) / ((S:S)^(5/2) + (Sd:Sd)^(5/4)),
 //     clipped to nu_t_max when asked, omega = 1/(3 (nu0 + nu_t) + 1/2).
+//     The second pass marches (x, y) columns along z with the z - 1, z,
+//     z + 1 velocities in registers, so each velocity is read from memory
+//     once; x and y neighbours come from the tile's own rows in L2.
 // Bytes per fluid cell: 4Q + 4 (mask) + 24 in the first pass; 4 + 24 + the
 // neighbours' velocities (mostly from L2) + 4 in the second, against two
 // full population reads for a moment pass followed by a strain pass.
