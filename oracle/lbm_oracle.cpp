@@ -1284,10 +1284,17 @@ struct Solver {
     long x, y, z;
     double w;
   };
-  // marker_stencil, rotating_ibm.rs:163-224
-  static void marker_stencil(const double p[3], const int dims[3], int radius, std::vector<SP>& out) {
+  // marker_stencil, rotating_ibm.rs:163-224.  `d` is the lattice's
+  // dimension: axes a >= d take the single point (0, 1.0) (rotating_ibm.rs:
+  // 176-179), the reference's 2D case, used here by one-cell-deep boxes
+  // that restate the reference's D2Q9 IBM tests.
+  static void marker_stencil(const double p[3], const int dims[3], int radius, std::vector<SP>& out, int d = 3) {
     std::vector<std::pair<long, double>> axes[3];
     for (int a = 0; a < 3; ++a) {
+      if (a >= d) {
+        axes[a].push_back({0, 1.0});
+        continue;
+      }
       long lo = (long)std::floor(p[a]) - radius + 1;
       long hi = (long)std::floor(p[a]) + radius;
       for (long i = lo; i <= hi; ++i) {
@@ -1318,7 +1325,7 @@ struct Solver {
   // diag layout: iterations, slip_max, slip_rms, slip_max_rel, slip_rms_rel,
   // torque[3], fluid_force[3], marker_force[3], momentum_error_rel
   int apply_ibm(int nm, const double* pos, const double* weight, const double center[3], const double omega[3],
-                int max_iter, double tol, int radius, double relax, double* diag_out) {
+                int max_iter, double tol, int radius, double relax, double* diag_out, int stencil_dim = 3) {
     int dims[3] = {nx, ny, nz};
     size_t n = nc;
     std::vector<double> u_now(3 * n, 0.0);
@@ -1359,7 +1366,7 @@ struct Solver {
     double scale = std::max(max_target, 1.0);
     std::vector<std::vector<SP>> st(nm);
     for (int m = 0; m < nm; ++m) {
-      marker_stencil(&pos[3 * m], dims, radius, st[m]);
+      marker_stencil(&pos[3 * m], dims, radius, st[m], stencil_dim);
       if (st[m].empty()) return -1;
     }
     std::vector<double> ksum(n, 0.0);
@@ -2087,10 +2094,11 @@ long orc_wale(void* p, double cw, double tau_eff_max, double* omega, double* nut
 }
 
 int orc_apply_ibm(void* p, int nm, const double* pos, const double* weight, const double center[3],
-                  const double omega[3], int max_iter, double tol, int radius, double relax, double* diag) {
+                  const double omega[3], int max_iter, double tol, int radius, double relax, double* diag,
+                  int stencil_dim) {
   Handle* h = (Handle*)p;
   int r = 0;
-  DISPATCH(h, r = S->apply_ibm(nm, pos, weight, center, omega, max_iter, tol, radius, relax, diag));
+  DISPATCH(h, r = S->apply_ibm(nm, pos, weight, center, omega, max_iter, tol, radius, relax, diag, stencil_dim));
   return r;
 }
 

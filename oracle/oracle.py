@@ -74,7 +74,7 @@ def lib():
         L.orc_vgrad.argtypes = [vp, d]
         L.orc_wale.restype = C.c_long
         L.orc_wale.argtypes = [vp, C.c_double, C.c_double, d, d, d]
-        L.orc_apply_ibm.argtypes = [vp, C.c_int, d, d, d, d, C.c_int, C.c_double, C.c_int, C.c_double, d]
+        L.orc_apply_ibm.argtypes = [vp, C.c_int, d, d, d, d, C.c_int, C.c_double, C.c_int, C.c_double, d, C.c_int]
         L.orc_scalar_enable.argtypes = [vp, C.c_double, C.c_int]
         L.orc_scalar_set.argtypes = [vp, C.c_int, d]
         L.orc_scalar_get.argtypes = [vp, C.c_int, d]
@@ -301,14 +301,17 @@ class OracleSolver:
         k = lib().orc_wale(self.h, cw, 0.0 if tau_eff_max is None else tau_eff_max, _d(om), _d(nt), _d(mx))
         return om, nt, int(k), float(mx[0])
 
-    def apply_ibm(self, pos, weight, center, omega, max_iter=3, tol=1e-2, radius=1, relax=1.0):
+    def apply_ibm(self, pos, weight, center, omega, max_iter=3, tol=1e-2, radius=1, relax=1.0, stencil_dim=3):
+        """Solver::apply_rotating_ibm (solver.rs:3016-3282).  stencil_dim=2
+        is the reference's 2D marker stencil (z axis a single point of
+        weight 1) for one-cell-deep boxes."""
         pos = np.ascontiguousarray(pos, np.float64).reshape(-1)
         weight = np.ascontiguousarray(weight, np.float64)
         c = np.ascontiguousarray(center, np.float64)
         o = np.ascontiguousarray(omega, np.float64)
         diag = np.zeros(15)
         rc = lib().orc_apply_ibm(self.h, len(weight), _d(pos), _d(weight), _d(c), _d(o), max_iter, tol, radius,
-                                 relax, _d(diag))
+                                 relax, _d(diag), int(stencil_dim))
         if rc != 0:
             raise ValueError("IBM marker stencil is outside the domain")
         return ibm_diag_dict(diag)

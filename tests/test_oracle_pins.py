@@ -412,6 +412,38 @@ def test_ibm_single_marker_is_corrected_exactly():
     assert d["momentum_error_rel"] < 1e-14
 
 
+def test_ibm_single_marker_impulse_closed_form_at_nonunit_density():
+    """solver.rs:3040-3208 at rho != 1 with a body force and a force field
+    already present: the interpolated velocity is u_now = (m + F/2) / rho
+    with F = global force + force field; a marker on a cell centre (hat
+    kernel: one stencil cell, W = 1) gets mobility 1 / rho, so its impulse is
+    rho (U - u_now) and lands in the force field of that cell; a marker
+    between cells is corrected exactly in one sweep (du = impulse W / rho,
+    the same 1 / rho as the mobility)."""
+    n = 6
+    g = np.array([1.0e-4, -2.0e-4, 0.5e-4])
+    s = O.OracleSolver(19, (n, n, n), prec=8, periodic=(1, 1, 1), collision=O.TRT, nu=0.05, force=g)
+    rng = np.random.default_rng(5)
+    s.init_with(1.2 + 0.05 * rng.random(n ** 3), 0.01 * rng.standard_normal((n ** 3, 3)))
+    ff0 = 1e-4 * rng.standard_normal((n ** 3, 3))
+    s.set_force_field(ff0)
+    rho = s.moments()[0]
+    c, _, _, _ = O.lattice(19)
+    m = s.get_f().T @ c.astype(float)  # sum_q c_q f_q (deviations: sum_q c_q w_q = 0)
+    cell = (3 * n + 2) * n + 4  # (x, y, z) = (4, 2, 3)
+    centre, om = np.array([1.0, 0.5, 3.0]), np.array([0.002, -0.001, 0.003])
+    p = np.array([4.0, 2.0, 3.0])
+    d = s.apply_ibm(p[None], np.ones(1), centre, om, max_iter=1, radius=1)
+    target = np.cross(om, p - centre)
+    u_now = (m[cell] + 0.5 * (g + ff0[cell])) / rho[cell]
+    got = s.force_field()[cell] - ff0[cell]
+    np.testing.assert_allclose(got, rho[cell] * (target - u_now), rtol=1e-13, atol=1e-19)
+    assert d["slip_max"] < 1e-16
+    # between cells, at the same non-unit density: one sweep is exact
+    d = s.apply_ibm(np.array([[2.3, 3.6, 1.45]]), np.ones(1), centre, om, max_iter=1, radius=2)
+    assert d["slip_max"] < 1e-16
+
+
 def test_ibm_torque_flips_with_omega_sign_and_force_matches_marker_force():
     pos, w, vel, c = O.radial_impeller((24,) * 3, 1.0, 20.0, 20.0, 8.0, 8.0, 6, 0.0, 0.01)
     outs = []
@@ -468,10 +500,14 @@ def test_scalar_uniform_stays_uniform_and_mass_is_conserved(d3q7):
     assert abs(s.scalar_get(k).sum() - m0) < 1e-12 * m0
 
 
-def test_d3q7_gaussian_spreads_at_2Dt():
+@pytest.mark.parametrize("d3q7", [False, True])
+def test_scalar_gaussian_spreads_at_2Dt(d3q7):
+    """Diffusivity of the BGK scalar: omega = 1 / (3 D + 1/2) on the hydro
+    lattice (solver.rs:2113-2205), 1 / (4 D + 1/2) on D3Q7 (c_s^2 = 1/4):
+    a Gaussian's variance grows by 2 D t per axis."""
     n, D, sig0 = 32, 0.05, 2.5
     s = O.OracleSolver(19, (n, n, n), prec=8, periodic=(1, 1, 1), collision=O.TRT, nu=0.05)
-    k = s.scalar_enable(D, True)
+    k = s.scalar_enable(D, d3q7)
     x, y, z = cases.coords((n, n, n))
     r2 = (x - 15.5) ** 2 + (y - 15.5) ** 2 + (z - 15.5) ** 2
     s.scalar_set(k, np.exp(-r2 / (2 * sig0 ** 2)))
@@ -512,3 +548,161 @@ def test_compartment_cv_partitions_fluid():
     v = np.linspace(1.0, 2.0, n ** 3)
     out = O.compartment_cv((n, n, n), v, solid, imp_z=4.0)
     assert sum(cnt for _, cnt in out.values()) == (solid == 0).sum()
+
+
+# ---- restated reference tests: moving walls, advection, IBM sweeps ------------------
+
+@pytest.mark.parametrize("Q,coll", [(19, "trt"), (27, "trt"), (27, "cm")])
+def test_moving_wall_plane_couette_is_exact(Q, coll):
+    """Half-way bounce-back with a moving wall (stream_row, kernels.rs:595-603):
+    plane Couette between a resting wall and a lid moving in +x has the exact
+    linear profile u(y) = U (y - 1/2) / (ny - 2) at steady state (walls on
+    the half-way planes).  The sign and the factor 6 w rho c.u of the wall
+    term both show in the profile."""
+    nx, ny, nz, U = 3, 10, 1, 0.01
+    u = np.zeros((6, 3))
+    u[O.YPOS] = [U, 0.0, 0.0]
+    solid, wall = O.build_wall_rims((nx, ny, nz), [0, 0, 1, 1, 0, 0], u)
+    s = O.OracleSolver(Q, (nx, ny, nz), prec=8, periodic=(1, 0, 1), collision=cases.COLL[coll], nu=1.0 / 6.0,
+                       solid=solid, wall_u=wall)
+    s.step(3000)
+    _, ux, uy, uz = (m.reshape(nz, ny, nx)[0] for m in s.moments())
+    exact = U * (np.arange(1, ny - 1) - 0.5) / (ny - 2)
+    # TRT: exact to round-off; central moment: its order-2 rate carries
+    # omega_s (1 - 0.16 |u|^2) (kernels.rs:481-490), a viscosity that varies
+    # with |u| by O(u^2) and bends the profile by a few 1e-6 of U
+    tol = 1e-10 if coll != "cm" else 1e-4
+    for col in range(nx):
+        assert np.abs(ux[1:ny - 1, col] - exact).max() < tol * U
+    assert np.abs(uy[1:ny - 1]).max() < 1e-13 and np.abs(uz[1:ny - 1]).max() < 1e-13
+
+
+def _centroid_drift(Q_scalar_d3q7, u0):
+    """solver.rs:5996-6033 restated with its drift made exact: a slab of
+    tracer in a uniform periodic stream; the tracer starts at its
+    equilibrium (set_scalar_concentration, solver.rs:2946-2956), so the
+    summed first moment of h is u0 * total at every step and the centroid
+    moves exactly u0 per step."""
+    dims = (40, 40, 40)
+    s = O.OracleSolver(19, dims, prec=8, periodic=(1, 1, 1), collision=O.TRT, nu=0.08)
+    n = int(np.prod(dims))
+    s.init_with(np.ones(n), np.tile(u0, (n, 1)))
+    k = s.scalar_enable(0.01, Q_scalar_d3q7)
+    x, y, z = cases.coords(dims)
+    c0 = ((np.abs(x - 20) <= 1) & (np.abs(y - 20) <= 1) & (np.abs(z - 20) <= 1)).astype(np.float64)
+    s.scalar_set(k, c0)
+    steps = 10
+    s.step(steps)
+    c1 = s.scalar_get(k)
+
+    def centroid(c, coord):
+        return (c * coord).sum() / c.sum()
+
+    return [(centroid(c1, a) - centroid(c0, a)) / steps for a in (x, y, z)], c1
+
+
+@pytest.mark.parametrize("d3q7", [False, True])
+def test_scalar_advects_with_the_stream(d3q7):
+    u0 = np.array([0.04, -0.01, 0.02])
+    drift, c1 = _centroid_drift(d3q7, u0)
+    np.testing.assert_allclose(drift, u0, rtol=1e-10, atol=1e-14)
+    # no tracer crossed a periodic seam in 10 steps (the moments above are unwrapped)
+    x, y, z = cases.coords((40, 40, 40))
+    far = (np.abs(x - 20) > 12) | (np.abs(y - 20) > 12) | (np.abs(z - 20) > 12)
+    assert np.abs(c1[far]).max() < 1e-30
+
+
+def _circle_2d(center, radius, omega_z, n_markers):
+    """RotatingBody::circle_2d (rotating_ibm.rs:55-75): n markers on a circle
+    in the z = 0 plane, weight = arc length 2 pi R / n."""
+    th = 2 * np.pi * np.arange(n_markers) / n_markers
+    pos = np.stack([center[0] + radius * np.cos(th), center[1] + radius * np.sin(th), 0 * th], 1)
+    w = np.full(n_markers, 2 * np.pi * radius / n_markers)
+    return pos, w, np.array([center[0], center[1], 0.0]), np.array([0.0, 0.0, omega_z])
+
+
+def _plane_solver(nx, ny, solid=None, wall=None, periodic=(1, 1, 1)):
+    """A one-cell-deep periodic-z D3Q19 TRT box, nu = 1/6: with u_z = 0 its
+    populations summed over c_z evolve as D2Q9 (the reference tests' 2D
+    lattice), and with the 2D marker stencil the reference's 2D IBM bands
+    apply."""
+    return O.OracleSolver(19, (nx, ny, 1), prec=8, periodic=periodic, collision=O.TRT, nu=1.0 / 6.0, solid=solid,
+                          wall_u=wall)
+
+
+def test_ibm_iterations_reduce_slip_and_conserve_momentum():
+    """rotating_ibm.rs:71-106: 96 markers (kernel radius 2: overlapping
+    stencils, so the mobility's row sum differs from the diagonal), one
+    sweep against sixteen."""
+    pos, w, c, om = _circle_2d((24.0, 24.0), 8.0, 0.003, 96)
+    single = _plane_solver(48, 48).apply_ibm(pos, w, c, om, max_iter=1, tol=0.0, radius=2, stencil_dim=2)
+    multi = _plane_solver(48, 48).apply_ibm(pos, w, c, om, max_iter=16, tol=0.0, radius=2, stencil_dim=2)
+    assert multi["iterations"] > 1
+    assert multi["slip_max_rel"] < 0.5 * single["slip_max_rel"]
+    assert multi["slip_max_rel"] < 6.0e-4
+    assert multi["slip_rms_rel"] < 4.0e-4
+    assert multi["momentum_error_rel"] < 1.0e-3
+
+
+def test_ibm_couette_matches_native_moving_wall():
+    """rotating_ibm.rs:176-218: a marker line moving at U one cell below a
+    resting wall drives nearly the profile the native moving wall drives."""
+    nx, ny, top = 48, 34, 0.002
+    u = np.zeros((6, 3))
+    u[O.YPOS] = [top, 0.0, 0.0]
+    solid, wall = O.build_wall_rims((nx, ny, 1), [0, 0, 1, 1, 0, 0], u)
+    native = _plane_solver(nx, ny, solid, wall, periodic=(1, 0, 1))
+    native.step(3000)
+    solid0, wall0 = O.build_wall_rims((nx, ny, 1), [0, 0, 1, 1, 0, 0], np.zeros((6, 3)))
+    ibm = _plane_solver(nx, ny, solid0, wall0, periodic=(1, 0, 1))
+    yl = float(ny - 3)
+    omega = 1.0e-6
+    pos = np.stack([np.arange(nx, dtype=float), np.full(nx, yl), np.zeros(nx)], 1)
+    c = np.array([0.0, yl + top / omega, 0.0])
+    for _ in range(800):
+        ibm.clear_force_field()
+        last = ibm.apply_ibm(pos, np.ones(nx), c, np.array([0.0, 0.0, omega]), max_iter=1, tol=1.0, radius=1,
+                             relax=0.05, stencil_dim=2)
+        ibm.step()
+    a = native.moments()[1].reshape(ny, nx)[:, nx // 2]
+    b = ibm.moments()[1].reshape(ny, nx)[:, nx // 2]
+    sl = slice(2, ny - 3)
+    l2 = np.sqrt(((b[sl] - a[sl]) ** 2).sum() / (a[sl] ** 2).sum())
+    assert last["slip_max_rel"] < 2.0e-3
+    assert l2 < 0.75
+    assert np.abs(b[sl] - a[sl]).max() / top < 0.65
+    # both profiles rise toward the moving boundary (the sign of the drive)
+    assert a[ny - 3] > a[ny // 2] > a[3] > 0 and b[ny - 4] > b[ny // 2] > b[3] > 0
+
+
+def test_ibm_taylor_couette_torque_and_profile():
+    """rotating_ibm.rs:222-309: inner cylinder of markers (r_i = 10) turning
+    at Omega inside a resting half-way wall (r_o = 30); the torque is within
+    10 % of 4 pi nu Omega r_i^2 r_o^2 / (r_o^2 - r_i^2) and the azimuthal
+    profile within the reference's bands of A r + B / r."""
+    nx = ny = 80
+    cx = cy = 40.0
+    r_i, r_o, omega = 10.0, 30.0, 0.00015
+    x, y, _ = cases.coords((nx, ny, 1))
+    r = np.hypot(x - cx, y - cy)
+    solid = (r > r_o + 0.5).astype(np.uint8)
+    sim = _plane_solver(nx, ny, solid, np.zeros((nx * ny, 3)))
+    pos, w, c, om = _circle_2d((cx, cy), r_i, omega, 160)
+    for _ in range(800):
+        sim.clear_force_field()
+        last = sim.apply_ibm(pos, w, c, om, max_iter=4, tol=1.0e-3, radius=2, relax=1.0, stencil_dim=2)
+        sim.step()
+    _, ux, uy, _ = sim.moments()
+    ro = r_o + 1.0
+    A = -omega * r_i ** 2 / (ro ** 2 - r_i ** 2)
+    B = omega * r_i ** 2 * ro ** 2 / (ro ** 2 - r_i ** 2)
+    band = (r >= r_i + 4.0) & (r <= r_o - 5.0)
+    ut = (-ux * (y - cy) + uy * (x - cx))[band] / r[band]
+    exact = A * r[band] + B / r[band]
+    l2 = np.sqrt(((ut - exact) ** 2).sum() / (exact ** 2).sum())
+    torque_ref = 4 * np.pi * (1.0 / 6.0) * omega * r_i ** 2 * ro ** 2 / (ro ** 2 - r_i ** 2)
+    assert last["slip_max_rel"] < 1.0e-5
+    assert last["momentum_error_rel"] < 1.0e-6
+    assert abs(abs(last["torque"][2]) - torque_ref) / torque_ref < 0.10
+    assert l2 < 7.0e-2
+    assert np.abs(ut - exact).max() / (omega * r_i) < 6.0e-2
