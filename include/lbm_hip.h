@@ -302,6 +302,16 @@ int64_t lbm_time(const lbm_fields* fl);
 /* Backend::read_moments, backend.rs:361-362 (compact; solids keep the
  * values of the last explicit write, as in the reference). */
 int lbm_read_moments(lbm_fields* fl, float* rho, float* ux, float* uy, float* uz);
+/* No counterpart in the reference.  lbm_read_moments for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): the same kernel and the same results, but
+ * the device staging buffer is a fixed array inside the library (one per
+ * GPU, held until the process ends) instead of a per-call hipMalloc.  At most
+ * LBM_NOALLOC_MAX_CELLS cells (64 MiB of staging); larger domains return
+ * LBM_ERR_INVALID.  The array is shared by every call on one GPU: calls that
+ * overlap on one GPU (two contexts on two host threads) race. */
+#define LBM_NOALLOC_MAX_CELLS 4194304
+int lbm_read_moments_noalloc(lbm_fields* fl, float* rho, float* ux, float* uy, float* uz);
 /* Backend::reduce, backend.rs:351-359: f64 accumulation over fluid cells. */
 int lbm_reduce(lbm_fields* fl, int kind, int axis, double* out);
 

@@ -113,6 +113,7 @@ SIGNATURES = {
     "lbm_force_list_size": (C.c_int64, [_P]),
     "lbm_time": (C.c_int64, [_P]),
     "lbm_read_moments": (_I, [_P, _pF, _pF, _pF, _pF]),
+    "lbm_read_moments_noalloc": (_I, [_P, _pF, _pF, _pF, _pF]),
     "lbm_reduce": (_I, [_P, _I, _I, _pD]),
     "lbm_geom_wall_rims": (_I, [_P, _pI, _pD]),
     "lbm_geom_stirred_tank": (_I, [_P, _D, _D, _D, _I, _D, _D]),

@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -120,7 +121,43 @@ struct lbm_fields {
   size_t bytes = 0;
 };
 
+// Staging array of lbm_read_moments_noalloc: rho, ux, uy, uz of at most
+// LBM_NOALLOC_MAX_CELLS cells.  Each GPU holds its own copy at its own
+// address; host code never passes this name to a kernel or a copy (in host
+// code it is a stand-in, not a device address) but the per-device address
+// that lbm_ctx_create resolves with hipGetSymbolAddress.
+static __device__ __align__(256) float scratch[4 * (size_t)LBM_NOALLOC_MAX_CELLS];
+
 namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+float* g_scratch_ptr[kMaxDevices] = {};
+hipError_t g_scratch_err[kMaxDevices] = {};
+bool g_scratch_done[kMaxDevices] = {};
+
+// Once per device, with that device current: the device address of
+// `scratch`.  A failure is recorded and reported by the entry point.
+void resolve_scratch(int device) {
+  if (device < 0 || device >= kMaxDevices) return;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_done[device]) return;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  g_scratch_err[device] = e;
+  g_scratch_ptr[device] = e == hipSuccess ? (float*)p : nullptr;
+  g_scratch_done[device] = true;
+}
+
+float* scratch_of(int device, hipError_t* err) {
+  if (device < 0 || device >= kMaxDevices) {
+    *err = hipErrorInvalidDevice;
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  *err = g_scratch_done[device] ? g_scratch_err[device] : hipErrorNotInitialized;
+  return g_scratch_ptr[device];
+}
 
 int fail(lbm_ctx* c, int code, const char* fmt, ...) {
   char buf[512];
@@ -441,6 +478,7 @@ int lbm_ctx_create(int device, lbm_ctx** out) {
   if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return LBM_ERR_DEVICE;
   if (device < 0 || device >= n) return LBM_ERR_INVALID;
   if (hipSetDevice(device) != hipSuccess) return LBM_ERR_DEVICE;
+  resolve_scratch(device);
   lbm_ctx* c = new (std::nothrow) lbm_ctx();
   if (!c) return LBM_ERR_NOMEM;
   c->device = device;
@@ -1446,6 +1484,28 @@ int lbm_read_moments(lbm_fields* fl, float* rho, float* ux, float* uy, float* uz
   HIPCHK(fl->ctx, hipMemcpyAsync(uz, buf + 3 * n, n * 4, hipMemcpyDeviceToHost, s));
   HIPCHK(fl->ctx, hipStreamSynchronize(s));
   dev_free(fl, buf, 4 * n * 4);
+  return LBM_OK;
+}
+
+int lbm_read_moments_noalloc(lbm_fields* fl, float* rho, float* ux, float* uy, float* uz) {
+  if (!fl || !rho || !ux || !uy || !uz) return LBM_ERR_INVALID;
+  const size_t n = fl->d.N;
+  if (n > (size_t)LBM_NOALLOC_MAX_CELLS)
+    return fail(fl->ctx, LBM_ERR_INVALID, "lbm_read_moments_noalloc: %zu cells exceed the staging maximum of %d cells",
+                n, LBM_NOALLOC_MAX_CELLS);
+  hipError_t se = hipSuccess;
+  float* buf = scratch_of(fl->ctx->device, &se);
+  if (!buf)
+    return fail(fl->ctx, LBM_ERR_DEVICE, "lbm_read_moments_noalloc: no staging address for device %d: %s",
+                fl->ctx->device, hipGetErrorString(se));
+  hipStream_t s = fl->ctx->stream;
+  HIPCHK(fl->ctx, launch_moments(fl->Q, fl->storage, fl->parity, fl->d, fl->f, fl->mask, fl->F_mom, fl->hp, buf,
+                                 buf + n, buf + 2 * n, buf + 3 * n, s));
+  HIPCHK(fl->ctx, hipMemcpyAsync(rho, buf, n * 4, hipMemcpyDeviceToHost, s));
+  HIPCHK(fl->ctx, hipMemcpyAsync(ux, buf + n, n * 4, hipMemcpyDeviceToHost, s));
+  HIPCHK(fl->ctx, hipMemcpyAsync(uy, buf + 2 * n, n * 4, hipMemcpyDeviceToHost, s));
+  HIPCHK(fl->ctx, hipMemcpyAsync(uz, buf + 3 * n, n * 4, hipMemcpyDeviceToHost, s));
+  HIPCHK(fl->ctx, hipStreamSynchronize(s));
   return LBM_OK;
 }
 
