@@ -425,6 +425,13 @@ def main():
 
     s.run_with_ibm(args.warmup, graph=False)
     dt, k_avg_ms, tq, diag = timed_span(s, args.steps)
+    # the same span replayed from the captured two-step HIP graph
+    s.run_with_ibm(2, graph=True)  # capture (not timed)
+    s.synchronize()
+    t0 = time.perf_counter()
+    s.run_with_ibm(args.steps, graph=True)
+    s.synchronize()
+    dt_graph = time.perf_counter() - t0
     mass = s.total_mass_f64()
 
     Q = args.lattice
@@ -436,7 +443,8 @@ def main():
     cells_main = fluid - n_list
     achieved = cells_main * B / (k_avg_ms * 1e-3) / 1e9
     print(f"[bench] {args.storage}: {ms_per_step:.3f} ms/step, k_hydro {k_avg_ms:.3f} ms, "
-          f"{achieved:.0f} GB/s algorithmic", file=sys.stderr, flush=True)
+          f"{achieved:.0f} GB/s algorithmic; graph-replayed span {1e3 * dt_graph / args.steps:.3f} ms/step",
+          file=sys.stderr, flush=True)
 
     copy_gbps = s.ctx.stream_copy_gbps(4 << 30, 20)
     prof = profile_name(args.size, args.lattice, args.collision, args.storage)
@@ -489,6 +497,8 @@ def main():
                    "fluid_cells": int(fluid), "parallelism": "single GPU"},
         "mlups_total": round(mlups_total, 2),
         "mlups_fluid": round(mlups_fluid, 2),
+        "graph_span": {"ms_per_step": round(1e3 * dt_graph / args.steps, 4),
+                       "mlups_fluid": round(fluid * args.steps / dt_graph / 1e6, 2)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None if traffic is None else round(traffic),

@@ -415,7 +415,12 @@ typedef struct lbm_ckpt_info {
 } lbm_ckpt_info;
 int lbm_checkpoint_write_part(lbm_fields* fl, const char* path, lbm_ckpt_info* info);
 /* Installs the state of `path` (masks, parameters, IBM body must already be
- * those of the saved run) and sets the step counter to `step`. */
+ * those of the saved run) and sets the step counter to `step`.  On entry
+ * info->mask_hash and info->payload_hash hold the manifest's values for the
+ * part: the current masks are checked against the first before the file is
+ * read (CKPT_MASK_MISMATCH, solver.rs:4954-4962) and the file's payload hash
+ * against the second (CKPT_PAYLOAD_CORRUPT) before any state is installed.
+ * On return info holds the file's values. */
 int lbm_checkpoint_read_part(lbm_fields* fl, const char* path, int64_t step, lbm_ckpt_info* info);
 /* FNV-1a 64 (offset basis 0xcbf29ce484222325, prime 0x100000001b3). */
 uint64_t lbm_fnv1a64(const void* data, size_t n);

@@ -47,9 +47,12 @@ def _f32(v: float) -> bytes:
     return struct.pack("<f", v)
 
 
-def spec_hash(Q: int, dims, nu: float, collision: str, magic: float, omega_shear, periodic, force) -> int:
-    """spec_hash (solver.rs:883-945) for a closed-face scenario with no
-    volume sources or face patches, fp32 run (forces hashed as f32)."""
+def spec_hash(Q: int, dims, nu: float, collision: str, magic: float, omega_shear, periodic, force,
+              sources=()) -> int:
+    """spec_hash (solver.rs:883-945) for a closed-face scenario with no face
+    patches, fp32 run (reals hashed as f32).  `sources` are the volume
+    sources in order, each (lo, hi, q_lu) for a mass flow or
+    (lo, hi, q_lu, u) for a jet (SourceKind ids 1 and 2)."""
     data = _u64(3) + _u64(Q)
     for v in dims:
         data += _u64(int(v))
@@ -66,6 +69,15 @@ def spec_hash(Q: int, dims, nu: float, collision: str, magic: float, omega_shear
         data += _u64(0)
     for v in force:
         data += _f32(float(v))
+    for src in sources:
+        for v in list(src[0]) + list(src[1]):
+            data += _u64(int(v))
+        if len(src) > 3:
+            data += _u64(2) + _f32(float(src[2]))
+            for v in src[3]:
+                data += _f32(float(v))
+        else:
+            data += _u64(1) + _f32(float(src[2]))
     return fnv1a64(data)
 
 
@@ -94,7 +106,8 @@ def _lattice(Q: int) -> str:
 
 
 def _solver_spec_hash(s) -> int:
-    return spec_hash(s.Q, s.dims, s.nu, s.collision, s.magic, s.params.omega_shear, s.periodic, s.force)
+    return spec_hash(s.Q, s.dims, s.nu, s.collision, s.magic, s.params.omega_shear, s.periodic, s.force,
+                     getattr(s, "sources", ()))
 
 
 def save(solver, directory: str) -> dict:
@@ -148,6 +161,11 @@ def restore(solver, directory: str) -> int:
         raise CheckpointError("CKPT_DECOMP_MISMATCH", "checkpoint decomposition differs from this run")
     rank = m["ranks"][0]
     info = _L.CkptInfo()
+    # the library checks the current masks against mask_hash before it reads
+    # the file, and the file's payload against payload_hash, before any state
+    # is installed
+    info.mask_hash = _parse_hash(rank.get("mask_hash"), "mask_hash")
+    info.payload_hash = _parse_hash(rank.get("payload_hash"), "payload_hash")
     rc = solver.lib.lbm_checkpoint_read_part(solver.h, os.path.join(directory, rank["file"]).encode(),
                                              int(m["step"]), C.byref(info))
     if rc != 0:
@@ -155,6 +173,13 @@ def restore(solver, directory: str) -> int:
         msg = msg.decode() if msg else ""
         code = msg.split(":", 1)[0] if msg.startswith("CKPT_") else "CKPT_IO"
         raise CheckpointError(code, msg)
-    if _hs(info.payload_hash) != rank["payload_hash"]:
-        raise CheckpointError("CKPT_PAYLOAD_CORRUPT", "rank payload hash differs from the manifest")
     return int(m["step"])
+
+
+def _parse_hash(s, what: str) -> int:
+    if not isinstance(s, str) or not s.startswith("fnv1a64:") or len(s) != 24:
+        raise CheckpointError("CKPT_MANIFEST_INVALID", f"{what} is not an fnv1a64 hash string: {s!r}")
+    try:
+        return int(s[8:], 16)
+    except ValueError:
+        raise CheckpointError("CKPT_MANIFEST_INVALID", f"{what} is not an fnv1a64 hash string: {s!r}") from None

@@ -195,6 +195,19 @@ void dev_free(lbm_fields* fl, void* p, size_t bytes) {
 
 size_t N3(const lbm_fields* fl) { return (size_t)fl->d.N * 3; }
 
+// Destroys a captured step graph once the launches already queued on `s`
+// (replays of that graph among them) have finished: an executable graph is
+// not destroyed under a replay still in flight.
+hipError_t drop_graph(hipStream_t s, hipGraphExec_t* gexec, hipGraph_t* graph) {
+  hipError_t e = hipSuccess;
+  if (*gexec || *graph) e = hipStreamSynchronize(s);
+  if (*gexec) (void)hipGraphExecDestroy(*gexec);
+  if (*graph) (void)hipGraphDestroy(*graph);
+  *gexec = nullptr;
+  *graph = nullptr;
+  return e;
+}
+
 // Plane padding in elements; LBM_PLANE_PAD overrides the default (tuning).
 uint64_t plane_pad_elems() {
   const char* e = getenv("LBM_PLANE_PAD");
@@ -1368,6 +1381,8 @@ int lbm_run_ibm_span(lbm_fields* fl, int steps, int use_graph, double* torque, l
   if (rc) return rc;
   hipStream_t s = fl->ctx->stream;
   if (fl->ring_cap < (uint32_t)steps) {
+    // the captured steps write the old ring: retire them before it goes
+    HIPCHK(fl->ctx, drop_graph(s, &fl->gexec, &fl->graph));
     if (fl->ring) dev_free(fl, fl->ring, (size_t)fl->ring_cap * ibm_ring_entry_bytes());
     fl->ring = nullptr;
     fl->ring_cap = 0;
@@ -1388,10 +1403,7 @@ int lbm_run_ibm_span(lbm_fields* fl, int steps, int use_graph, double* torque, l
     const bool stale = !fl->gexec || fl->g_parity != fl->parity || fl->g_gen != fl->gen ||
                        fl->g_stash != fl->stash_sel || memcmp(&a, &fl->g_args, sizeof a) != 0;
     if (stale) {
-      if (fl->gexec) (void)hipGraphExecDestroy(fl->gexec);
-      if (fl->graph) (void)hipGraphDestroy(fl->graph);
-      fl->gexec = nullptr;
-      fl->graph = nullptr;
+      HIPCHK(fl->ctx, drop_graph(s, &fl->gexec, &fl->graph));
       const int parity0 = fl->parity;
       const int64_t time0 = fl->time;
       HIPCHK(fl->ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
@@ -1958,15 +1970,13 @@ int lbm_batch_run_span(lbm_batch* b, int steps, int use_graph, double* torque) {
   const size_t K = b->mem.size();
   hipStream_t s = b->ctx->stream;
   if (b->ibm && b->ring_cap < (uint32_t)steps) {
+    // the captured steps write the old ring: retire them before it goes
+    HIPCHK(b->ctx, drop_graph(s, &b->gexec, &b->graph));
     if (b->ring) (void)hipFree(b->ring);
     b->ring = nullptr;
     b->ring_cap = 0;
     HIPCHK(b->ctx, hipMalloc(&b->ring, (size_t)steps * K * ibm_ring_entry_bytes()));
     b->ring_cap = (uint32_t)steps;
-    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
-    if (b->graph) (void)hipGraphDestroy(b->graph);
-    b->gexec = nullptr;
-    b->graph = nullptr;
   }
   if (b->ibm && !b->ring_pos) HIPCHK(b->ctx, hipMalloc((void**)&b->ring_pos, 256));
   if (b->ibm) HIPCHK(b->ctx, launch_fill_f32((float*)b->ring_pos, 1, 0.0f, s));
@@ -1974,10 +1984,7 @@ int lbm_batch_run_span(lbm_batch* b, int steps, int use_graph, double* torque) {
   if (use_graph && steps >= 2) {
     const int parity = b->mem[0]->parity;
     if (!b->gexec || b->g_parity != parity || b->g_grid != b->grid) {
-      if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
-      if (b->graph) (void)hipGraphDestroy(b->graph);
-      b->gexec = nullptr;
-      b->graph = nullptr;
+      HIPCHK(b->ctx, drop_graph(s, &b->gexec, &b->graph));
       std::vector<int64_t> t0;
       for (lbm_fields* fl : b->mem) t0.push_back(fl->time);
       HIPCHK(b->ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
@@ -2233,11 +2240,21 @@ int lbm_adjoint_torque(lbm_fields* fl, int steps, int window, int snapshots, dou
   Snapshot fin;
   double *lamA = nullptr, *lamB = nullptr, *obar = nullptr, *Fbar = nullptr, *G0 = nullptr, *G1 = nullptr,
          *wbar = nullptr;
+  // a failure after the forward pass puts the final state back (best
+  // effort) before every snapshot and adjoint buffer is freed
+  bool reversing = false;
   auto cleanup = [&]() {
+    const std::string err = fl->ctx->err;  // the failure being reported
+    if (reversing && fin.f) (void)snap_restore(fl, fin);
+    (void)hipStreamSynchronize(s);
+    fl->ctx->err = err;
     for (Snapshot& sn : snap) snap_free(sn);
     snap_free(fin);
     for (double* p : {lamA, lamB, obar, Fbar, G0, G1, wbar})
       if (p) (void)hipFree(p);
+  };
+  auto hip_rc = [&](hipError_t e, const char* what) -> int {
+    return e == hipSuccess ? LBM_OK : fail(fl->ctx, LBM_ERR_DEVICE, "%s: %s", what, hipGetErrorString(e));
   };
 #define ADJ_TRY(expr)   \
   do {                  \
@@ -2247,6 +2264,7 @@ int lbm_adjoint_torque(lbm_fields* fl, int steps, int window, int snapshots, dou
       return _r;        \
     }                   \
   } while (0)
+#define ADJ_HIP(expr) ADJ_TRY(hip_rc((expr), #expr))
   // forward pass: J
   ADJ_TRY(snap_save(fl, snap[0]));
   std::vector<double> tq(3 * (size_t)steps);
@@ -2260,6 +2278,7 @@ int lbm_adjoint_torque(lbm_fields* fl, int steps, int window, int snapshots, dou
     return LBM_OK;
   }
   // reverse pass
+  reversing = true;
   auto dalloc = [&](double** p, size_t n) -> int {
     if (hipMalloc((void**)p, n * sizeof(double)) != hipSuccess)
       return fail(fl->ctx, LBM_ERR_NOMEM, "adjoint buffer of %zu doubles", n);
@@ -2295,11 +2314,11 @@ int lbm_adjoint_torque(lbm_fields* fl, int steps, int window, int snapshots, dou
     } else {  // adjoint of step `arg`; the solver holds the state before it
       held = -1;
       if (!fl->flist_host.empty())
-        HIPCHK(fl->ctx, launch_copy_force_list(fl->flist, (uint32_t)fl->flist_host.size(), fl->F_cur, nullptr, s));
+        ADJ_HIP(launch_copy_force_list(fl->flist, (uint32_t)fl->flist_host.size(), fl->F_cur, nullptr, s));
       fl->field_present = false;
       ADJ_TRY(ibm_enqueue_recorded(view_of(fl), fl->ibm));
       fl->field_present = true;
-      HIPCHK(fl->ctx, launch_adj_collide_stream(fl->storage, fl->parity, fl->d, fl->f, fl->mask, fl->omega,
+      ADJ_HIP(launch_adj_collide_stream(fl->storage, fl->parity, fl->d, fl->f, fl->mask, fl->omega,
                                                 fl->F_cur, fl->F_mom, fl->hp, lamA, lamB, wbar, Fbar, G1, s));
       ADJ_TRY(ibm_adjoint(view_of(fl), fl->ibm, Fbar, G0, arg >= steps - window ? tq_bar : 0.0, lamB, obar));
       std::swap(lamA, lamB);
@@ -2308,11 +2327,11 @@ int lbm_adjoint_torque(lbm_fields* fl, int steps, int window, int snapshots, dou
   }
   double ob = 0.0;
   std::vector<double> wb(dJ_dnu ? N : 0);
-  HIPCHK(fl->ctx, hipMemcpyAsync(&ob, obar, sizeof ob, hipMemcpyDeviceToHost, s));
-  if (dJ_dnu) HIPCHK(fl->ctx, hipMemcpyAsync(wb.data(), wbar, N * sizeof(double), hipMemcpyDeviceToHost, s));
+  ADJ_HIP(hipMemcpyAsync(&ob, obar, sizeof ob, hipMemcpyDeviceToHost, s));
+  if (dJ_dnu) ADJ_HIP(hipMemcpyAsync(wb.data(), wbar, N * sizeof(double), hipMemcpyDeviceToHost, s));
   std::vector<float> om(fl->omega && dJ_dnu ? N : 0);
-  if (!om.empty()) HIPCHK(fl->ctx, hipMemcpyAsync(om.data(), fl->omega, N * 4, hipMemcpyDeviceToHost, s));
-  HIPCHK(fl->ctx, hipStreamSynchronize(s));
+  if (!om.empty()) ADJ_HIP(hipMemcpyAsync(om.data(), fl->omega, N * 4, hipMemcpyDeviceToHost, s));
+  ADJ_HIP(hipStreamSynchronize(s));
   if (dJ_domega) *dJ_domega = ob;
   if (dJ_dnu) {
     // omega_plus = 1 / (3 nu + 1/2): d omega_plus / d nu = -3 omega_plus^2
@@ -2323,8 +2342,10 @@ int lbm_adjoint_torque(lbm_fields* fl, int steps, int window, int snapshots, dou
   }
   if (forward_steps) *forward_steps = advanced;
   ADJ_TRY(snap_restore(fl, fin));
-  HIPCHK(fl->ctx, hipStreamSynchronize(s));
+  ADJ_HIP(hipStreamSynchronize(s));
+  reversing = false;
   cleanup();
+#undef ADJ_HIP
 #undef ADJ_TRY
   return LBM_OK;
 }
@@ -2544,6 +2565,25 @@ int lbm_checkpoint_read_part(lbm_fields* fl, const char* path, int64_t step, lbm
     return fail(fl->ctx, LBM_ERR_UNSUPPORTED, "CKPT_UNSUPPORTED: scalar distributions are not checkpointed");
   if (open_bits(fl))
     return fail(fl->ctx, LBM_ERR_UNSUPPORTED, "CKPT_UNSUPPORTED: open-face stale slots are not checkpointed");
+  const PadGeom g(fl->d);
+  const int per[3] = {(int)fl->d.per[0], (int)fl->d.per[1], (int)fl->d.per[2]};
+  // the manifest's mask hash against the current masks, before the rank file
+  // is read (solver.rs:4954-4962)
+  std::vector<uint8_t> solid(g.nc);
+  int rc = lbm_read_solid(fl, solid.data());
+  if (rc) return rc;
+  {
+    std::vector<float> wall;
+    if (fl->wall_u) {
+      wall.resize(3 * g.nc);
+      HIPCHK(fl->ctx, hipMemcpyAsync(wall.data(), fl->wall_u, 3 * g.nc * 4, hipMemcpyDeviceToHost, fl->ctx->stream));
+      HIPCHK(fl->ctx, hipStreamSynchronize(fl->ctx->stream));
+    }
+    const uint64_t cur = mask_hash(g, per, solid, wall);
+    if (cur != info->mask_hash)
+      return fail(fl->ctx, LBM_ERR_INVALID, "CKPT_MASK_MISMATCH: mask_hash differed: checkpoint=fnv1a64:%016llx "
+                  "current=fnv1a64:%016llx", (unsigned long long)info->mask_hash, (unsigned long long)cur);
+  }
   FILE* fp = fopen(path, "rb");
   if (!fp) return fail(fl->ctx, LBM_ERR_INVALID, "CKPT_IO: cannot open %s", path);
   std::vector<uint8_t> b;
@@ -2576,7 +2616,6 @@ int lbm_checkpoint_read_part(lbm_fields* fl, const char* path, int64_t step, lbm
       !take(&nsec, 4))
     return fail(fl->ctx, LBM_ERR_INVALID, "CKPT_TRUNCATED: rank file ended inside the header");
   if (dtype != 0) return fail(fl->ctx, LBM_ERR_INVALID, "CKPT_DTYPE_MISMATCH: checkpoint dtype is not f32");
-  const PadGeom g(fl->d);
   if (q16 != fl->Q || d16 != 3 || lat != (fl->Q == 19 ? 319 : 327))
     return fail(fl->ctx, LBM_ERR_INVALID, "CKPT_LATTICE_MISMATCH: checkpoint lattice %u (Q %u)", lat, q16);
   if (np != g.np || nc != g.nc)
@@ -2596,12 +2635,15 @@ int lbm_checkpoint_read_part(lbm_fields* fl, const char* path, int64_t step, lbm
   memcpy(&ph, b.data() + pend, 8);
   const uint8_t* payload = b.data() + pos;
   const size_t plen = pend - pos;
+  if (ph != info->payload_hash)
+    return fail(fl->ctx, LBM_ERR_INVALID, "CKPT_PAYLOAD_CORRUPT: rank payload hash fnv1a64:%016llx differs from "
+                "manifest fnv1a64:%016llx", (unsigned long long)ph, (unsigned long long)info->payload_hash);
   if (fnv1a(payload, plen) != ph)
     return fail(fl->ctx, LBM_ERR_INVALID, "CKPT_PAYLOAD_CORRUPT: payload hash does not match its contents");
   auto section = [&](uint32_t id, uint64_t want, const uint8_t** out) -> int {
     for (auto& e : tab)
       if (e.id == id) {
-        if (e.off + e.len > plen)
+        if (e.off > plen || e.len > plen - e.off)
           return fail(fl->ctx, LBM_ERR_INVALID, "CKPT_TRUNCATED: section %u exceeds the payload", id);
         if (e.len != want)
           return fail(fl->ctx, LBM_ERR_INVALID, "CKPT_GEOM_MISMATCH: section %u has %llu bytes, expected %llu",
@@ -2614,23 +2656,19 @@ int lbm_checkpoint_read_part(lbm_fields* fl, const char* path, int64_t step, lbm
   };
   const size_t Q = (size_t)fl->Q;
   const uint8_t *sf = nullptr, *sm = nullptr, *ss = nullptr, *sff = nullptr;
-  int rc = section(kSecF, Q * g.np * 4, &sf);
+  rc = section(kSecF, Q * g.np * 4, &sf);
   if (!rc) rc = section(kSecMoments, 4 * g.nc * 4, &sm);
   if (!rc) rc = section(kSecSolid, g.np, &ss);
   if (!rc) rc = section(kSecForce, 3 * g.nc * 4, &sff);
   if (rc) return rc;
   if (!sf) return fail(fl->ctx, LBM_ERR_INVALID, "CKPT_TRUNCATED: checkpoint is missing required section F_PRIMARY");
   if (!sm) return fail(fl->ctx, LBM_ERR_INVALID, "CKPT_TRUNCATED: checkpoint is missing required section MOMENTS");
-  const int per[3] = {(int)fl->d.per[0], (int)fl->d.per[1], (int)fl->d.per[2]};
-  std::vector<uint8_t> solid(g.nc);
-  rc = lbm_read_solid(fl, solid.data());
-  if (rc) return rc;
   if (ss) {
     std::vector<uint8_t> mine(g.np);
     pad_plane(g, per, solid.data(), mine.data());
     for (size_t k = 0; k < g.np; ++k)
       if ((ss[k] != 0) != (mine[k] != 0))
-        return fail(fl->ctx, LBM_ERR_INVALID, "CKPT_GEOM_MISMATCH: serialized solid mask differs from rebuilt mask");
+        return fail(fl->ctx, LBM_ERR_INVALID, "CKPT_MASK_MISMATCH: serialized solid mask differs from rebuilt mask");
   }
   // install: populations (core cells), force field, cached-moment force
   std::vector<float> f(Q * g.nc);

@@ -152,6 +152,7 @@ class Solver:
             self.set_masks(solid, wall_u)
         self._ibm_key = None
         self.scalars = []
+        self.sources = []
 
     # ---- lifetime ---------------------------------------------------------
     def close(self):
@@ -244,6 +245,7 @@ class Solver:
                 for c in range(3):
                     e.u[c] = float(src[3][c])
         self._chk(self.lib.lbm_set_volume_sources(self.h, len(sources), arr))
+        self.sources = [tuple(src) for src in sources]
 
     def set_force_probe(self, mask):
         """``Solver::set_force_probe`` (solver.rs:2741-2760): solids whose

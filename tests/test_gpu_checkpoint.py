@@ -144,7 +144,17 @@ def test_corrupt_and_mismatched_checkpoints_are_refused(tmp_path):
     other = L.Solver(19, (n, n, n), nu=0.03, collision="trt", solid=s2, wall_u=wall.astype(np.float32))
     with pytest.raises(K.CheckpointError) as e:
         other.restore_checkpoint(str(tmp_path))
-    assert e.value.code == "CKPT_GEOM_MISMATCH"
+    assert e.value.code == "CKPT_MASK_MISMATCH"
+    # other wall velocity: mask hash, refused before any state is installed
+    w2 = wall.astype(np.float32).copy()
+    w2[w2[:, 0] != 0, 0] *= 2.0
+    other = L.Solver(19, (n, n, n), nu=0.03, collision="trt", solid=solid, wall_u=w2)
+    other.run(2)
+    before = other.populations()
+    with pytest.raises(K.CheckpointError) as e:
+        other.restore_checkpoint(str(tmp_path))
+    assert e.value.code == "CKPT_MASK_MISMATCH"
+    assert np.array_equal(other.populations(), before) and other.time == 2
     # one flipped payload byte
     bad = bytearray(good)
     bad[len(bad) // 2] ^= 0x40
@@ -154,6 +164,45 @@ def test_corrupt_and_mismatched_checkpoints_are_refused(tmp_path):
     assert e.value.code == "CKPT_PAYLOAD_CORRUPT"
     # truncated file
     open(path, "wb").write(good[:100])
+    with pytest.raises(K.CheckpointError) as e:
+        _cavity().restore_checkpoint(str(tmp_path))
+    assert e.value.code == "CKPT_TRUNCATED"
+
+
+def test_volume_sources_enter_the_scenario_hash(tmp_path):
+    n = 16
+    a = _cavity()
+    a.set_volume_sources([((4, 4, 4), (6, 6, 6), 1e-4)])
+    a.run(2)
+    a.save_checkpoint(str(tmp_path))
+    b = _cavity()
+    b.set_volume_sources([((4, 4, 4), (6, 6, 6), 1e-4, (0.01, 0.0, 0.0))])
+    with pytest.raises(K.CheckpointError) as e:
+        b.restore_checkpoint(str(tmp_path))
+    assert e.value.code == "CKPT_SCENARIO_MISMATCH"
+    c = _cavity()
+    c.set_volume_sources([((4, 4, 4), (6, 6, 6), 1e-4)])
+    assert c.restore_checkpoint(str(tmp_path)) == 2
+    assert np.array_equal(c.populations(), a.populations())
+    assert n == a.dims[0]
+
+
+def test_section_offset_past_the_payload_is_truncated(tmp_path):
+    import struct
+
+    a = _cavity()
+    a.run(1)
+    man = a.save_checkpoint(str(tmp_path))
+    path = os.path.join(str(tmp_path), man["ranks"][0]["file"])
+    raw = bytearray(open(path, "rb").read())
+    # header: magic 8, version 4, endian 1, dtype 1, lattice 2, q 2, d 2, np 8, n_core 8, nsec 4;
+    # then (id u32, offset u64, len u64) per section: make the first offset wrap
+    # offset + len past 2^64
+    tab = 8 + 4 + 1 + 1 + 2 + 2 + 2 + 8 + 8 + 4
+    sec_id, off, ln = struct.unpack_from("<IQQ", raw, tab)
+    assert sec_id == 1 and off == 0
+    struct.pack_into("<Q", raw, tab + 4, (1 << 64) - ln + 8)
+    open(path, "wb").write(bytes(raw))
     with pytest.raises(K.CheckpointError) as e:
         _cavity().restore_checkpoint(str(tmp_path))
     assert e.value.code == "CKPT_TRUNCATED"
