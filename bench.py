@@ -478,7 +478,7 @@ def main():
                          f"{args.cpu_steps} steps incl. IBM, {secs:.1f}s"}
 
     out = {
-        "metric": "MLUPS & % HBM roofline, D3Q27 tank 512^3",
+        "metric": f"MLUPS & % HBM roofline, D3Q{Q} tank {args.size}^3",
         "value": round(mlups_fluid, 2),
         "unit": "MLUPS",
         "n_gpus": 1,

@@ -481,7 +481,7 @@ extern "C" {
 
 const char* lbm_build_info(void) {
   return "lbmflow_amd liblbm_hip: gfx950, AA-pattern fused collide-stream, TRT/BGK (D3Q19/D3Q27), "
-         "central moment (D3Q27), storage f32|f16|bf16 (shifted, 16-bit scaled by 1/w)";
+         "central moment (D3Q19/D3Q27), storage f32|f16|bf16 (shifted, 16-bit scaled by 1/w)";
 }
 
 int lbm_ctx_create(int device, lbm_ctx** out) {
@@ -710,8 +710,6 @@ int lbm_boundary_links(lbm_fields* fl, int64_t* count, int64_t* cells, uint8_t* 
 int lbm_set_params(lbm_fields* fl, const lbm_step_params* p) {
   if (!fl || !p) return LBM_ERR_INVALID;
   if (p->collision < 0 || p->collision > 2) return fail(fl->ctx, LBM_ERR_INVALID, "collision %d", p->collision);
-  if (p->collision == LBM_COLLISION_CM && fl->Q != 27)
-    return fail(fl->ctx, LBM_ERR_UNSUPPORTED, "central-moment collision is implemented for D3Q27 only");
   if (!(p->omega_p > 0.0 && p->omega_p < 2.0) || !(p->omega_m > 0.0 && p->omega_m < 2.0))
     return fail(fl->ctx, LBM_ERR_INVALID, "relaxation rates must lie in (0, 2)");
   fl->params = *p;

@@ -254,6 +254,178 @@ __device__ __forceinline__ void collide_cm27_general(float* fv, float r, float u
   });
 }
 
+// D3Q19 central moments.  The basis is the reference's (central_basis,
+// kernels.rs:162-183): exponents (ex, ey, ez) in {0,1,2}^3, x outermost,
+// without the eight with all three positive.  It is not a tensor product, so
+// the transforms are the raw-moment matrix R (entries c^e in {-1, 0, 1}),
+// the binomial shift by -u / +u (kernels.rs:332-370) and the inverse
+// P = R^-1, whose entries (0, +-1/4, +-1/2, +-1) are exact in fp32.  All three
+// are compile-time tables; zero terms vanish when the loops unroll.
+struct Cm19Tables {
+  int e[19][3];
+  int idx[3][3][3];  // exponent -> moment, -1 when not in the basis
+  double P[19][19];
+};
+
+constexpr int cm19_phi(int q, int m, const Cm19Tables& t) {
+  int v = 1;
+  for (int a = 0; a < 3; ++a)
+    for (int k = 0; k < t.e[m][a]; ++k) v *= lat_c(q, a);
+  return v;
+}
+
+constexpr Cm19Tables make_cm19() {
+  Cm19Tables t{};
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b)
+      for (int c = 0; c < 3; ++c) t.idx[a][b][c] = -1;
+  int n = 0;
+  for (int ax = 0; ax <= 2; ++ax)
+    for (int ay = 0; ay <= 2; ++ay)
+      for (int az = 0; az <= 2; ++az) {
+        if (ax > 0 && ay > 0 && az > 0) continue;
+        t.e[n][0] = ax;
+        t.e[n][1] = ay;
+        t.e[n][2] = az;
+        t.idx[ax][ay][az] = n;
+        ++n;
+      }
+  // Gauss-Jordan inverse of R (partial pivoting), exact here: R is integer
+  // with a power-of-two-denominator inverse
+  double A[19][38] = {};
+  for (int m = 0; m < 19; ++m) {
+    for (int q = 0; q < 19; ++q) A[m][q] = cm19_phi(q, m, t);
+    A[m][19 + m] = 1.0;
+  }
+  for (int col = 0; col < 19; ++col) {
+    int piv = col;
+    for (int r = col + 1; r < 19; ++r)
+      if ((A[r][col] < 0 ? -A[r][col] : A[r][col]) > (A[piv][col] < 0 ? -A[piv][col] : A[piv][col])) piv = r;
+    for (int j = 0; j < 38; ++j) {
+      const double tmp = A[col][j];
+      A[col][j] = A[piv][j];
+      A[piv][j] = tmp;
+    }
+    const double inv = 1.0 / A[col][col];
+    for (int j = 0; j < 38; ++j) A[col][j] *= inv;
+    for (int r = 0; r < 19; ++r) {
+      if (r == col || A[r][col] == 0.0) continue;
+      const double f = A[r][col];
+      for (int j = 0; j < 38; ++j) A[r][j] -= f * A[col][j];
+    }
+  }
+  for (int q = 0; q < 19; ++q)
+    for (int m = 0; m < 19; ++m) t.P[q][m] = A[q][19 + m];
+  return t;
+}
+inline constexpr Cm19Tables kCm19 = make_cm19();
+
+constexpr int binom2(int n, int k) { return (k == 0 || k == n) ? 1 : 2; }  // n <= 2
+
+// u^k for k in 0..2
+__device__ __forceinline__ float pow2u(float u, int k) { return k == 0 ? 1.0f : (k == 1 ? u : u * u); }
+
+// central moments (shift by -u) of the raw moments R, or raw moments (shift
+// by +u) of central moments: out[m] = sum_{k <= e_m} C(e,k) s^(e-k) in[idx(k)]
+template <bool TO_CENTRAL>
+__device__ __forceinline__ void cm19_shift(const float* in, float* out, float ux, float uy, float uz) {
+#pragma clang fp contract(on)
+  const float sx = TO_CENTRAL ? -ux : ux, sy = TO_CENTRAL ? -uy : uy, sz = TO_CENTRAL ? -uz : uz;
+  sfor<0, 19>([&](auto mc) {
+    constexpr int m = decltype(mc)::value;
+    constexpr int ex = kCm19.e[m][0], ey = kCm19.e[m][1], ez = kCm19.e[m][2];
+    float acc = in[m];  // k = e: coefficient 1
+    sfor<0, (ex + 1) * (ey + 1) * (ez + 1)>([&](auto kc) {
+      constexpr int kk = decltype(kc)::value;
+      constexpr int kx = kk / ((ey + 1) * (ez + 1)), ky = (kk / (ez + 1)) % (ey + 1), kz = kk % (ez + 1);
+      if constexpr (!(kx == ex && ky == ey && kz == ez)) {
+        constexpr int j = kCm19.idx[kx][ky][kz];
+        constexpr float coef = (float)(binom2(ex, kx) * binom2(ey, ky) * binom2(ez, kz));
+        acc = acc + coef * (pow2u(sx, ex - kx) * pow2u(sy, ey - ky) * pow2u(sz, ez - kz)) * in[j];
+      }
+    });
+    out[m] = acc;
+  });
+}
+
+// raw moments of a population vector: R[m] = sum_q c_q^e_m v_q
+__device__ __forceinline__ void cm19_raw(const float* v, float* R) {
+  sfor<0, 19>([&](auto mc) {
+    constexpr int m = decltype(mc)::value;
+    float acc = 0.0f;
+    sfor<0, 19>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      constexpr int c = cm19_phi(q, m, kCm19);
+      if constexpr (c == 1) acc = acc + v[q];
+      if constexpr (c == -1) acc = acc - v[q];
+    });
+    R[m] = acc;
+  });
+}
+
+// D3Q19 central-moment collision, general form (any force): feq + B(d) as
+// for D3Q27, with the matrix transforms above.
+__device__ __forceinline__ void collide_cm19_general(float* fv, float r, float ux, float uy, float uz,
+                                                     const float F[3], bool fon, float os_base) {
+#pragma clang fp contract(on)
+  const float usq = ux * ux + uy * uy + uz * uz;
+  const float b = (r - 1.0f) - r * 1.5f * usq;
+  float feq[19], G[19], K[19], Sm[19];
+  sfor<0, 19>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    const float cu = cdot<q>(ux, uy, uz);
+    feq[q] = lat_wf(19, q) * (b + r * cu * (3.0f + 4.5f * cu));
+    G[q] = fv[q] - feq[q];
+  });
+  cm19_raw(G, Sm);
+  cm19_shift<true>(Sm, K, ux, uy, uz);
+  if (fon) {
+    const float uf = ux * F[0] + uy * F[1] + uz * F[2];
+    sfor<0, 19>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      const float cu = cdot<q>(ux, uy, uz);
+      const float cf = cdot<q>(F[0], F[1], F[2]);
+      G[q] = lat_wf(19, q) * (3.0f * (cf - uf) + 9.0f * cu * cf);
+    });
+    float Rs[19];
+    cm19_raw(G, Rs);
+    cm19_shift<true>(Rs, Sm, ux, uy, uz);
+  } else {
+#pragma unroll
+    for (int m = 0; m < 19; ++m) Sm[m] = 0.0f;
+  }
+  const float os = cm_rate(os_base, ux, uy, uz);
+  const float a1 = 1.0f - os, a2 = 1.0f - 0.5f * os;
+  constexpr int ixx = kCm19.idx[2][0][0], iyy = kCm19.idx[0][2][0], izz = kCm19.idx[0][0][2];
+  const float kb = (K[ixx] + K[iyy] + K[izz]) * (1.0f / 3.0f);
+  const float sb = (Sm[ixx] + Sm[iyy] + Sm[izz]) * (1.0f / 3.0f);
+  float D[19];
+  sfor<0, 19>([&](auto mc) {
+    constexpr int m = decltype(mc)::value;
+    constexpr int ex = kCm19.e[m][0], ey = kCm19.e[m][1], ez = kCm19.e[m][2];
+    constexpr int order = ex + ey + ez;
+    if constexpr (order <= 1)
+      D[m] = K[m] + Sm[m];
+    else if constexpr (order == 2 && (ex == 2 || ey == 2 || ez == 2))
+      D[m] = a1 * (K[m] - kb) + 0.5f * sb + a2 * (Sm[m] - sb);
+    else if constexpr (order == 2)
+      D[m] = a1 * K[m] + a2 * Sm[m];
+    else
+      D[m] = 0.5f * Sm[m];
+  });
+  cm19_shift<false>(D, K, ux, uy, uz);
+  sfor<0, 19>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    float acc = feq[q];
+    sfor<0, 19>([&](auto mc) {
+      constexpr int m = decltype(mc)::value;
+      constexpr float pc = (float)kCm19.P[q][m];
+      if constexpr (pc != 0.0f) acc = acc + pc * K[m];
+    });
+    fv[q] = acc;
+  });
+}
+
 // Force-free D3Q27 central-moment collision straight from the populations.
 // The ten raw moments of order <= 2 of f - w (partial sums z, y, x) give rho
 // and u, and -- because the second-order equilibrium reproduces those moments
@@ -545,8 +717,12 @@ __device__ __forceinline__ void cell_step(const HydroArgs& a, uint32_t i, uint32
     float ux, uy, uz;
     moments<Q>(fv, Fm, fon, r, ux, uy, uz);
     collide_trt<Q>(fv, r, ux, uy, uz, Fc, fon, op, cp, a.p);
+  } else if constexpr (Q == 19) {
+    float ux, uy, uz;
+    moments<Q>(fv, Fm, fon, r, ux, uy, uz);
+    collide_cm19_general(fv, r, ux, uy, uz, Fc, fon, os);
   } else {
-    static_assert(Q == 27, "central-moment kernels are D3Q27");
+    static_assert(Q == 27, "central-moment kernels are D3Q19 or D3Q27");
     if constexpr (FON) {
       float ux, uy, uz;
       moments<Q>(fv, Fm, fon, r, ux, uy, uz);
@@ -1104,6 +1280,7 @@ static hipError_t hydro_s(int Q, int collision, int odd, const HydroArgs& a, hip
   if (Q == 19 && kind == 0) return hydro_q<19, S, 0>(odd, a, s, list);
   if (Q == 27 && kind == 0) return hydro_q<27, S, 0>(odd, a, s, list);
   if (Q == 27 && kind == 1) return hydro_q<27, S, 1>(odd, a, s, list);
+  if (Q == 19 && kind == 1) return hydro_q<19, S, 1>(odd, a, s, list);
   return hipErrorInvalidValue;
 }
 
@@ -1147,6 +1324,7 @@ static hipError_t batch_s(int Q, int collision, int odd, const HydroArgs* tab, c
   if (Q == 19 && kind == 0) return batch_q<19, S, 0>(odd, tab, ent, nent, grid, cells, ncells, s);
   if (Q == 27 && kind == 0) return batch_q<27, S, 0>(odd, tab, ent, nent, grid, cells, ncells, s);
   if (Q == 27 && kind == 1) return batch_q<27, S, 1>(odd, tab, ent, nent, grid, cells, ncells, s);
+  if (Q == 19 && kind == 1) return batch_q<19, S, 1>(odd, tab, ent, nent, grid, cells, ncells, s);
   return hipErrorInvalidValue;
 }
 
