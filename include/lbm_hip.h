@@ -434,6 +434,15 @@ int lbm_scalar_gather(lbm_fields* fl, int index, float* conc);
 /* qoi::scalar_cv over fluid cells (qoi.rs:507-532), f64 on the device.
  * *defined = 0 when the mean is zero or there are no fluid cells. */
 int lbm_scalar_cv(lbm_fields* fl, int index, double* cv, int* defined);
+/* qoi::compartment_cv (qoi.rs:699-739; the runner's final mixing QOIs,
+ * runner.rs:1818): the CV of scalar `index` over the fluid cells of three
+ * horizontal compartments, in the order top, bulk, near_impeller.  A cell of
+ * plane z is top when (z + 0.5)/nz >= 2/3, else near_impeller when
+ * has_impeller_z and |(z + 0.5) - impeller_z| <= nz/6, else bulk.
+ * defined[k] = 0 where the compartment is empty or its mean is zero (the
+ * reference's None); cell_count[k] counts its fluid cells. */
+int lbm_scalar_compartment_cv(lbm_fields* fl, int index, int has_impeller_z, double impeller_z, double cv[3],
+                              int defined[3], int64_t cell_count[3]);
 
 /* ---- QOI reducers (qoi.rs:477-556) --------------------------------------- */
 /* power_qois: out = {power, n_hz, np, p_over_v}. */

@@ -137,6 +137,7 @@ SIGNATURES = {
     "lbm_scalar_set": (_I, [_P, _I, _pF]),
     "lbm_scalar_gather": (_I, [_P, _I, _pF]),
     "lbm_scalar_cv": (_I, [_P, _I, _pD, _pI]),
+    "lbm_scalar_compartment_cv": (_I, [_P, _I, _I, _D, _pD, _pI, _pI64]),
     "lbm_qoi_power": (_I, [_D, _D, _D, _D, _D, _pD]),
     "lbm_qoi_mixing_time": (_I, [C.c_int64, _pD, _pD, _pD]),
 }

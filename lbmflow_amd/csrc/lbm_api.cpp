@@ -1711,6 +1711,16 @@ int lbm_scalar_cv(lbm_fields* fl, int index, double* cv, int* defined) {
   return scalar_cv(view_of(fl), fl->scalars[index], cv, defined);
 }
 
+int lbm_scalar_compartment_cv(lbm_fields* fl, int index, int has_impeller_z, double impeller_z, double cv[3],
+                              int defined[3], int64_t cell_count[3]) {
+  if (!fl || !cv || !defined || !cell_count || index < 0 || index >= (int)fl->scalars.size())
+    return LBM_ERR_INVALID;
+  if (has_impeller_z && !std::isfinite(impeller_z))
+    return fail(fl->ctx, LBM_ERR_INVALID, "impeller height must be finite");
+  return scalar_compartment_cv(view_of(fl), fl->scalars[index], has_impeller_z ? 1 : 0, impeller_z, cv, defined,
+                               cell_count);
+}
+
 // ---- QOIs -------------------------------------------------------------------
 int lbm_qoi_power(double torque, double omega, double rho, double diameter, double volume, double out[4]) {
   // power_qois, qoi.rs:477-505

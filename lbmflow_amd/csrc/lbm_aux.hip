@@ -996,36 +996,45 @@ __global__ void k_scalar_stream(Dom d, const uint32_t* mask, const float* h, flo
   C[i] = sum;
 }
 
-// two-pass CV over fluid cells (qoi::scalar_cv, qoi.rs:507-532)
-__global__ __launch_bounds__(256) void k_cv_pass(Dom d, const uint32_t* mask, const float* C, int pass, double mean,
+// Two-pass CV over fluid cells (qoi::scalar_cv, qoi.rs:507-532), for the
+// whole fluid (parts = 1) or the three compartments of compartment_cv
+// (qoi.rs:699-739, parts = 3): a fluid cell of plane z is "top" when
+// (z + 0.5)/nz >= 2/3, else "near_impeller" when an impeller height is given
+// and |(z + 0.5) - z_imp| <= nz/6, else "bulk" (f64, the reference's
+// expressions).  Pass 0 sums counts and values per compartment, pass 1 the
+// squared deviations from the compartment means.  Per block: 6 doubles
+// (count, sum per compartment; pass 1 fills the first 3 with the variances).
+__global__ __launch_bounds__(256) void k_cv_pass(Dom d, const uint32_t* mask, const float* C, int pass, int parts,
+                                                 int has_imp, double imp_z, double m0, double m1, double m2,
                                                  double* partials) {
-  __shared__ double s0[256], s1[256];
-  double a = 0.0, b = 0.0;
+  __shared__ double lds[4 * 6];
+  double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  const double nz = (double)d.nz;
   for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < d.N; i += gridDim.x * 256) {
     if (mask[i] & kMaskSolid) continue;
+    int k = 0;
+    if (parts == 3) {
+      const double zc = (double)(i / (d.nx * d.ny)) + 0.5;
+      if (zc / nz >= 2.0 / 3.0)
+        k = 0;
+      else if (has_imp && fabs(zc - imp_z) <= nz / 6.0)
+        k = 2;
+      else
+        k = 1;
+    }
     const double v = (double)C[i];
     if (pass == 0) {
-      a += 1.0;
-      b += v;
+      acc[2 * k] += 1.0;
+      acc[2 * k + 1] += v;
     } else {
-      const double dv = v - mean;
-      a += dv * dv;
+      const double dv = v - (k == 0 ? m0 : (k == 1 ? m1 : m2));
+      acc[k] += dv * dv;
     }
   }
-  s0[threadIdx.x] = a;
-  s1[threadIdx.x] = b;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) {
-      s0[threadIdx.x] += s0[threadIdx.x + w];
-      s1[threadIdx.x] += s1[threadIdx.x + w];
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    partials[2 * blockIdx.x] = s0[0];
-    partials[2 * blockIdx.x + 1] = s1[0];
-  }
+  block_sum256<6>(acc, lds);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int k = 0; k < 6; ++k) partials[6 * blockIdx.x + k] = acc[k];
 }
 
 ScalarState* scalar_create(const FieldsView& v, double diffusivity, int Qs, std::string* err) {
@@ -1040,7 +1049,7 @@ ScalarState* scalar_create(const FieldsView& v, double diffusivity, int Qs, std:
   if (e == hipSuccess) e = hipMalloc((void**)&s->h2, (size_t)Qs * N * 4);
   if (e == hipSuccess) e = hipMalloc((void**)&s->C, N * 4);
   if (e == hipSuccess) e = hipMalloc((void**)&s->mom, 4 * N * 4);
-  if (e == hipSuccess) e = hipMalloc((void**)&s->partials, 2 * s->npartials * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc((void**)&s->partials, 6 * s->npartials * sizeof(double));
   if (e == hipSuccess) e = hipMemsetAsync(s->h, 0, (size_t)Qs * N * 4, v.stream);
   if (e == hipSuccess) e = hipMemsetAsync(s->h2, 0, (size_t)Qs * N * 4, v.stream);
   if (e == hipSuccess) e = hipMemsetAsync(s->C, 0, N * 4, v.stream);
@@ -1113,32 +1122,58 @@ int scalar_step(const FieldsView& v, ScalarState* s) {
   return LBM_OK;
 }
 
-int scalar_cv(const FieldsView& v, ScalarState* s, double* cv, int* defined) {
+// Both passes of k_cv_pass; results per compartment (parts = 1 or 3).
+static int cv_parts(const FieldsView& v, ScalarState* s, int parts, int has_imp, double imp_z, double* cv,
+                    int* defined, int64_t* counts) {
   const uint32_t np = std::min<uint32_t>(s->npartials, nblk(v.d.N));
-  std::vector<double> h(2 * np);
-  hipLaunchKernelGGL(k_cv_pass, dim3(np), dim3(256), 0, v.stream, v.d, v.mask, s->C, 0, 0.0, s->partials);
-  AUXCHK(v, hipGetLastError());
-  AUXCHK(v, hipMemcpyAsync(h.data(), s->partials, 2 * np * sizeof(double), hipMemcpyDeviceToHost, v.stream));
-  AUXCHK(v, hipStreamSynchronize(v.stream));
-  double n = 0.0, sum = 0.0;
-  for (uint32_t k = 0; k < np; ++k) {
-    n += h[2 * k];
-    sum += h[2 * k + 1];
+  std::vector<double> h(6 * (size_t)np);
+  auto run = [&](int pass, const double* mean) -> int {
+    hipLaunchKernelGGL(k_cv_pass, dim3(np), dim3(256), 0, v.stream, v.d, v.mask, s->C, pass, parts, has_imp, imp_z,
+                       mean[0], mean[1], mean[2], s->partials);
+    AUXCHK(v, hipGetLastError());
+    AUXCHK(v, hipMemcpyAsync(h.data(), s->partials, h.size() * sizeof(double), hipMemcpyDeviceToHost, v.stream));
+    AUXCHK(v, hipStreamSynchronize(v.stream));
+    return LBM_OK;
+  };
+  const double zero[3] = {0.0, 0.0, 0.0};
+  int rc = run(0, zero);
+  if (rc) return rc;
+  double n[3] = {0, 0, 0}, sum[3] = {0, 0, 0}, mean[3] = {0, 0, 0};
+  for (uint32_t b = 0; b < np; ++b)
+    for (int k = 0; k < parts; ++k) {
+      n[k] += h[6 * (size_t)b + 2 * k];
+      sum[k] += h[6 * (size_t)b + 2 * k + 1];
+    }
+  bool any = false;
+  for (int k = 0; k < parts; ++k) {
+    counts[k] = (int64_t)n[k];
+    defined[k] = 0;
+    cv[k] = 0.0;
+    if (n[k] == 0.0) continue;
+    mean[k] = sum[k] / n[k];
+    if (mean[k] == 0.0) continue;
+    defined[k] = 1;
+    any = true;
   }
-  *defined = 0;
-  *cv = 0.0;
-  if (n == 0.0) return LBM_OK;
-  const double mean = sum / n;
-  if (mean == 0.0) return LBM_OK;
-  hipLaunchKernelGGL(k_cv_pass, dim3(np), dim3(256), 0, v.stream, v.d, v.mask, s->C, 1, mean, s->partials);
-  AUXCHK(v, hipGetLastError());
-  AUXCHK(v, hipMemcpyAsync(h.data(), s->partials, 2 * np * sizeof(double), hipMemcpyDeviceToHost, v.stream));
-  AUXCHK(v, hipStreamSynchronize(v.stream));
-  double var = 0.0;
-  for (uint32_t k = 0; k < np; ++k) var += h[2 * k];
-  *cv = std::sqrt(var / n) / std::fabs(mean);
-  *defined = 1;
+  if (!any) return LBM_OK;
+  rc = run(1, mean);
+  if (rc) return rc;
+  double var[3] = {0, 0, 0};
+  for (uint32_t b = 0; b < np; ++b)
+    for (int k = 0; k < parts; ++k) var[k] += h[6 * (size_t)b + k];
+  for (int k = 0; k < parts; ++k)
+    if (defined[k]) cv[k] = std::sqrt(var[k] / n[k]) / std::fabs(mean[k]);
   return LBM_OK;
+}
+
+int scalar_cv(const FieldsView& v, ScalarState* s, double* cv, int* defined) {
+  int64_t count = 0;
+  return cv_parts(v, s, 1, 0, 0.0, cv, defined, &count);
+}
+
+int scalar_compartment_cv(const FieldsView& v, ScalarState* s, int has_imp, double imp_z, double cv[3],
+                          int defined[3], int64_t counts[3]) {
+  return cv_parts(v, s, 3, has_imp, imp_z, cv, defined, counts);
 }
 
 }  // namespace lbm

@@ -80,5 +80,7 @@ int scalar_set(const FieldsView& v, ScalarState* s, const float* conc);
 int scalar_gather(const FieldsView& v, ScalarState* s, float* conc);
 int scalar_step(const FieldsView& v, ScalarState* s);
 int scalar_cv(const FieldsView& v, ScalarState* s, double* cv, int* defined);
+int scalar_compartment_cv(const FieldsView& v, ScalarState* s, int has_imp, double imp_z, double cv[3],
+                          int defined[3], int64_t counts[3]);
 
 }  // namespace lbm

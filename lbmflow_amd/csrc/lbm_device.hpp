@@ -21,6 +21,31 @@ __device__ __forceinline__ void sfor(F&& f) {
   }
 }
 
+// Sums N doubles over a 256-thread block (four 64-lane waves).  Each wave
+// reduces its lanes with a fixed xor butterfly through cross-lane shuffles
+// (no LDS, no atomics), then thread 0 adds the four wave sums in wave order
+// through 4N doubles of LDS.  The order of every addition is fixed by the
+// launch shape, so results repeat bit for bit.  On return thread 0 holds the
+// block sums in v; other threads hold their wave's sums.
+template <int N>
+__device__ __forceinline__ void block_sum256(double* v, double* lds) {
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    double x = v[k];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    v[k] = x;
+  }
+  const int w = (int)(threadIdx.x >> 6);
+  if ((threadIdx.x & 63) == 0)
+#pragma unroll
+    for (int k = 0; k < N; ++k) lds[w * N + k] = v[k];
+  __syncthreads();
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int k = 0; k < N; ++k) v[k] = ((lds[k] + lds[N + k]) + lds[2 * N + k]) + lds[3 * N + k];
+}
+
 struct bf16s {
   uint16_t v;
 };

@@ -1049,7 +1049,7 @@ __global__ void k_init_with(Dom d, S* f, const uint32_t* mask, const float* rho,
 template <int Q, typename S>
 __global__ __launch_bounds__(256) void k_reduce(Dom d, const S* f, const uint32_t* mask, const float* F_cur,
                                                 HydroParams p, int odd, int kind, int axis, double* partials) {
-  __shared__ double sm[256];
+  __shared__ double sm[4];
   double acc = 0.0;
   for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < d.N; i += gridDim.x * 256) {
     const uint32_t m = mask[i];
@@ -1079,13 +1079,8 @@ __global__ __launch_bounds__(256) void k_reduce(Dom d, const S* f, const uint32_
       acc += mom + 0.5 * fa;
     }
   }
-  sm[threadIdx.x] = acc;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) sm[threadIdx.x] += sm[threadIdx.x + w];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) partials[blockIdx.x] = sm[0];
+  block_sum256<1>(&acc, sm);
+  if (threadIdx.x == 0) partials[blockIdx.x] = acc;
 }
 
 // masks: links in the pull convention plus cell flags

@@ -503,6 +503,18 @@ class Solver:
         self._chk(self.lib.lbm_scalar_cv(self.h, index, C.byref(cv), C.byref(ok)))
         return cv.value if ok.value else None
 
+    def compartment_cv(self, index: int, impeller_center_z: float | None = None):
+        """``qoi::compartment_cv`` (qoi.rs:699-739) of scalar `index`: a list
+        of (name, cv or None, cell_count) for top, bulk and near_impeller."""
+        cv = np.zeros(3)
+        ok = np.zeros(3, np.int32)
+        cnt = np.zeros(3, np.int64)
+        has = impeller_center_z is not None
+        self._chk(self.lib.lbm_scalar_compartment_cv(self.h, index, int(has), float(impeller_center_z or 0.0),
+                                                     _dp(cv), _ip(ok), cnt.ctypes.data_as(C.POINTER(C.c_int64))))
+        return [(name, float(cv[k]) if ok[k] else None, int(cnt[k]))
+                for k, name in enumerate(("top", "bulk", "near_impeller"))]
+
 
 # ---- geometry and QOI helpers ---------------------------------------------------
 

@@ -151,3 +151,18 @@ def np_from_torque(torque, tc, steps, every=1):
     tq = win.mean() if len(win) else torque.mean()
     q = O.power_qois(tq, tc["omega"], 1.0, tc["D"], 1.0)
     return q["np"], tq
+
+
+def links_from_solid(solid, dims, Q, periodic=(0, 0, 0)):
+    c, _, _, _ = O.lattice(Q)
+    nx, ny, nz = dims
+    s3 = solid.reshape(nz, ny, nx).astype(bool)
+    out = []
+    for q in range(1, Q):
+        # source cell of direction q at (x, y, z) is (x, y, z) - c_q
+        src = np.roll(s3, shift=(int(c[q, 2]), int(c[q, 1]), int(c[q, 0])), axis=(0, 1, 2))
+        hit = (~s3) & src
+        cells = np.flatnonzero(hit.reshape(-1))
+        out.extend((int(i), q) for i in cells)
+    out.sort()
+    return np.array([o[0] for o in out], np.int64), np.array([o[1] for o in out], np.uint8)

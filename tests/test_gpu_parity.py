@@ -37,21 +37,6 @@ def assert_moments_exact(orc, hip, solid, what=""):
         assert np.array_equal(a, b), f"{what}: moment {name} max |d| {np.abs(a - b).max()}"
 
 
-def links_from_solid(solid, dims, Q, periodic=(0, 0, 0)):
-    c, _, _, _ = O.lattice(Q)
-    nx, ny, nz = dims
-    s3 = solid.reshape(nz, ny, nx).astype(bool)
-    out = []
-    for q in range(1, Q):
-        # source cell of direction q at (x, y, z) is (x, y, z) - c_q
-        src = np.roll(s3, shift=(int(c[q, 2]), int(c[q, 1]), int(c[q, 0])), axis=(0, 1, 2))
-        hit = (~s3) & src
-        cells = np.flatnonzero(hit.reshape(-1))
-        out.extend((int(i), q) for i in cells)
-    out.sort()
-    return np.array([o[0] for o in out], np.int64), np.array([o[1] for o in out], np.uint8)
-
-
 def assert_cm_gate(a, b, Q, what=""):
     """Central-moment bar: the reference's own CpuSimd-vs-CpuScalar gate,
     1e-6 absolute on the stored deviations (backend_simd_equiv.rs:1-25), and
@@ -77,7 +62,7 @@ def test_cavity_trt_bitexact(Q):
     orc, hip = cases.pair(Q, (n, n, n), collision="trt", nu=0.05 * (n - 2) / 100.0, solid=solid, wall_u=wall)
     assert np.array_equal(hip.solid(), solid)
     cells, qs = hip.boundary_links()
-    ec, eq = links_from_solid(solid, (n, n, n), Q)
+    ec, eq = cases.links_from_solid(solid, (n, n, n), Q)
     assert np.array_equal(cells, ec) and np.array_equal(qs, eq)
     done = 0
     for k in (1, 1, 5, 33):
@@ -303,7 +288,7 @@ def test_tank_flags_and_links_bitexact():
     hip2.run(1)
     assert np.array_equal(hip2.solid(), solid)
     cells, qs = hip2.boundary_links()
-    ec, eq = links_from_solid(solid, (n, n, n), 27)
+    ec, eq = cases.links_from_solid(solid, (n, n, n), 27)
     assert np.array_equal(cells, ec) and np.array_equal(qs, eq)
 
 

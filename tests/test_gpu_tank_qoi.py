@@ -31,3 +31,27 @@ def test_tank_torque_np_and_cv_match_oracle(collision, Q):
     assert (mg["t95"] is None) == (mo["t95"] is None)
     if mo["t95"] is not None:
         assert abs(mg["t95"] - mo["t95"]) <= 0.005 * mo["t95"] + 1.0
+
+
+@pytest.mark.parametrize("lattice", ["hydro", "d3q7"])
+def test_compartment_cv_matches_oracle(lattice):
+    """compartment_cv on the device against the oracle's restatement
+    (qoi.rs:699-739) over the concentration the device holds, as the runner
+    evaluates it (runner.rs:1818): counts equal, CV within 1e-12."""
+    import lbmflow_amd as L
+
+    tc = cases.tank_case(24, nu=0.02, collision="trt", Q=19, sigma=2.5)
+    t_g, cv_g, sg = cases.run_tank_gpu(tc, 30, scalar_D=0.02, d3q7=lattice == "d3q7")
+    conc = sg.gather_scalar_concentration(0).astype(np.float64)
+    solid = sg.solid()
+    for imp_z in (None, tc["T"] / 3.0):  # impeller plane at the clearance C = T/3
+        got = sg.compartment_cv(0, imp_z)
+        want = O.compartment_cv(sg.dims, conc, solid, imp_z=imp_z)
+        for name, cv, cnt in got:
+            cv_o, cnt_o = want[name]
+            assert cnt == cnt_o, name
+            assert (cv is None) == (cv_o is None), name
+            if cv is not None:
+                assert cv == pytest.approx(cv_o, rel=1e-12), name
+    whole = sg.scalar_cv(0)
+    assert whole == pytest.approx(O.scalar_cv(conc, (solid == 0).astype(np.uint8)), rel=1e-12)

@@ -550,6 +550,31 @@ def test_compartment_cv_partitions_fluid():
     assert sum(cnt for _, cnt in out.values()) == (solid == 0).sum()
 
 
+@pytest.mark.parametrize("imp_z", [None, 4.0, 9.5])
+def test_compartment_cv_matches_brute_force(imp_z):
+    """compartment_cv (qoi.rs:699-739) against numpy on a non-cubic grid:
+    top where (z + 0.5)/nz >= 2/3, near_impeller within nz/6 of the impeller
+    height, bulk otherwise; CV = population std / |mean| per compartment."""
+    dims = (11, 9, 14)
+    nx, ny, nz = dims
+    rng = np.random.default_rng(5)
+    solid = (rng.random(nx * ny * nz) < 0.2).astype(np.uint8)
+    v = 1.0 + rng.random(nx * ny * nz)
+    out = O.compartment_cv(dims, v, solid, imp_z=imp_z)
+    z = np.repeat(np.arange(nz), nx * ny) + 0.5
+    fl = solid == 0
+    top = fl & (z / nz >= 2.0 / 3.0)
+    near = fl & ~top & (np.abs(z - imp_z) <= nz / 6.0) if imp_z is not None else np.zeros_like(fl)
+    bulk = fl & ~top & ~near
+    for name, m in (("top", top), ("bulk", bulk), ("near_impeller", near)):
+        cv, cnt = out[name]
+        assert cnt == m.sum()
+        if cnt == 0:
+            assert cv is None
+        else:
+            assert cv == pytest.approx(np.std(v[m]) / abs(np.mean(v[m])), rel=1e-12)
+
+
 # ---- restated reference tests: moving walls, advection, IBM sweeps ------------------
 
 @pytest.mark.parametrize("Q,coll", [(19, "trt"), (27, "trt"), (27, "cm")])
