@@ -1353,12 +1353,11 @@ int lbm_run_span(lbm_fields* fl, int steps) {
 // IBM, collide-stream, scalars.  Launches only; the IBM diagnostics go to the
 // device ring.
 static int ibm_step(lbm_fields* fl) {
-  if (!fl->flist_host.empty())
-    HIPCHK(fl->ctx, launch_copy_force_list(fl->flist, (uint32_t)fl->flist_host.size(), fl->F_cur, nullptr,
-                                           fl->ctx->stream));
+  // clear the field at its listed cells (in the gather's launch), then the IBM
   fl->field_present = false;
   fl->field_uniform = false;
-  int rc = ibm_enqueue(view_of(fl), fl->ibm, fl->ring, fl->ring_cap, fl->ring_pos);
+  int rc = ibm_enqueue(view_of(fl), fl->ibm, fl->ring, fl->ring_cap, fl->ring_pos,
+                       fl->flist_host.empty() ? nullptr : fl->flist, (uint32_t)fl->flist_host.size());
   if (rc) return rc;
   fl->field_present = true;
   return hydro_step(fl);
@@ -1394,7 +1393,7 @@ int lbm_run_ibm_span(lbm_fields* fl, int steps, int use_graph, double* torque, l
     if (rc) return rc;
   }
   // a fill kernel, not a memset, on the stream the graph runs on
-  HIPCHK(fl->ctx, launch_fill_f32((float*)fl->ring_pos, 1, 0.0f, s));
+  HIPCHK(fl->ctx, launch_fill_f32((float*)fl->ring_pos, 2, 0.0f, s));  // slot, blocks done
   int k = 0;
   if (use_graph && !fl->timing && steps >= 2) {
     const HydroArgs a = hydro_args(fl);
@@ -1987,7 +1986,7 @@ int lbm_batch_run_span(lbm_batch* b, int steps, int use_graph, double* torque) {
     b->ring_cap = (uint32_t)steps;
   }
   if (b->ibm && !b->ring_pos) HIPCHK(b->ctx, hipMalloc((void**)&b->ring_pos, 256));
-  if (b->ibm) HIPCHK(b->ctx, launch_fill_f32((float*)b->ring_pos, 1, 0.0f, s));
+  if (b->ibm) HIPCHK(b->ctx, launch_fill_f32((float*)b->ring_pos, 2, 0.0f, s));  // slot, blocks done
   int k = 0;
   if (use_graph && steps >= 2) {
     const int parity = b->mem[0]->parity;

@@ -7,11 +7,12 @@
 // quantity the reference accumulates over markers in marker order
 // (cell_kernel_sum, spread, du) is gathered here cell-centrically over a
 // marker-ordered CSR list, so the f64 sums follow the reference's order and
-// the force field is reproduced bit for bit.  Per-marker quantities
-// (interpolated velocity, mobility, impulse) are marker-parallel.  The
-// diagnostics (torque, net forces, slip statistics) are reduced over markers
-// with a fixed tree.  The slip-tolerance early exit is a device flag, so the
-// host never waits inside a sweep.
+// the force field is reproduced bit for bit.  A step is two launches: the
+// gather of u_now and rho at the stencil cells, then one block per body
+// that runs every sweep (per-marker interpolation, mobility and impulse;
+// per-cell spread; fixed-tree marker reductions), commits the force field
+// and records the diagnostics.  The slip-tolerance early exit is taken
+// inside that block, so the host never waits inside a sweep.
 //
 // Scalar ADE: BGK collide toward h_eq = w C (1 + c.u/cs^2) with the cached
 // concentration and the post-step hydro velocity, pull streaming with
@@ -74,6 +75,12 @@ struct IbmState {
   IbmHost host;  // K == 1: the body this state was built from
   // per instance
   uint32_t* moff = nullptr;  // K + 1 marker offsets
+  uint32_t* uoff = nullptr;  // K + 1 stencil-cell offsets
+  // fused path: dynamic LDS of k_ibm_sweeps (0: the wide path), sized for
+  // the largest body (mmax markers, umax stencil cells)
+  size_t lds_bytes = 0;
+  int level = 0;
+  uint32_t mmax = 0, umax = 0, pmax = 0;
   double* cen = nullptr;     // K x 3
   double* scale = nullptr;   // K
   IbmDomain* dom = nullptr;  // K
@@ -93,6 +100,8 @@ struct IbmState {
   uint32_t* uc_inst = nullptr;  // U
   uint32_t* uc_off = nullptr;   // U + 1
   uint32_t* uc_ent = nullptr;   // P, marker order within a cell
+  uint32_t* uc_m = nullptr;     // P: marker of entry e (pt_m[uc_ent[e]])
+  double* uc_w = nullptr;       // P: weight of entry e (pt_w[uc_ent[e]])
   double* ksum = nullptr;       // U
   // per-apply work
   double* unow = nullptr;    // U x 3
@@ -122,13 +131,23 @@ struct IbmState {
 // (solver.rs:3040-3084: f64 momentum of the populations, force without the
 // just-cleared field).
 // With reset_K > 0 the first reset_K threads also clear the instances'
-// diagnostics (no other kernel of the gather reads them).
+// diagnostics (no other kernel of the gather reads them).  With nclr > 0
+// the first nclr threads also clear the force field at the listed cells
+// (the runner's per-step clear, runner.rs:1079-1129): the gather does not
+// read the field then (field_present is 0), and the commit that adds the new
+// spread runs in a later launch.
 template <int Q, typename S>
 __global__ void k_ibm_gather(const IbmDomain* dom, int odd, const uint32_t* uc_cell, const uint32_t* uc_inst,
                              uint32_t U, int field_present, double* unow, double* rho, double* du, double* spread,
-                             IbmDevDiag* diag, uint32_t reset_K) {
+                             IbmDevDiag* diag, uint32_t reset_K, const uint32_t* clr, uint32_t nclr, float* clrF) {
   const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
   if (u < reset_K) memset(diag + u, 0, sizeof(IbmDevDiag));
+  if (u < nclr) {
+    const size_t i = clr[u];
+    clrF[3 * i + 0] = 0.0f;
+    clrF[3 * i + 1] = 0.0f;
+    clrF[3 * i + 2] = 0.0f;
+  }
   if (u >= U) return;
   const IbmDomain& D = dom[uc_inst[u]];
   const Dom d = D.d;
@@ -161,6 +180,7 @@ __global__ void k_ibm_gather(const IbmDomain* dom, int odd, const uint32_t* uc_c
   rho[u] = (double)r;
 }
 
+// ---- wide path (bodies whose stencil data do not fit in LDS) -----------------
 __global__ void k_ibm_reset(IbmDevDiag* diag, int K) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k < K) memset(diag + k, 0, sizeof(IbmDevDiag));
@@ -337,6 +357,307 @@ __global__ void k_ibm_record(const IbmDevDiag* diag, int K, IbmDevDiag* ring, ui
   if (threadIdx.x == 0) *pos = slot + 1;
 }
 
+// ---------------------------------------------------------------------------
+// Fused path: all direct-forcing sweeps of one body in one block (1024
+// threads), after the gather: the impulses, the spread and du update, the
+// per-marker diagnostics and their reductions for every sweep (solver.rs:
+// 3107-3208), then the force-field commit and the diagnostics record.  One
+// launch per step, one block per body, every body of a batch at once; the
+// block leaves the sweep loop as soon as its body meets the slip tolerance
+// (device-side test, no host wait).
+//
+// LDS staging, copied in once per step: always the body's per-cell state
+// (u_now, du, spread, rho, kernel sum: 11 doubles per stencil cell) and its
+// marker impulses (3 per marker); when they fit as well (level 2), the
+// markers' stencils (point offsets, cell and weight per point) with their
+// targets, positions and weights, so the marker phases never leave the CU.
+// Bodies whose state does not fit take the wide path.
+// Order of every f64 operation is the reference's and the wide path's: per
+// cell over its markers in marker order, per marker over its stencil points
+// in point order, and the marker reductions on a fixed 256-lane tree (lane t
+// takes markers t, t + 256, ... in order, then a halving tree), so the force
+// field is the reference's bit for bit and both paths agree to the bit.
+// ---------------------------------------------------------------------------
+struct IbmSweepArgs {
+  const uint32_t *moff, *uoff, *mk_off, *pt_u, *uc_off, *uc_m, *uc_cell;
+  const double *pt_w, *uc_w, *target, *pos, *cen, *weight, *scale, *ksum, *unow, *rho;
+  IbmDevDiag* diag;
+  const IbmDomain* dom;
+  double relax, tol;
+  int max_iter;
+  int level;                  // 1: cell state + impulses in LDS; 2: also the marker stencils
+  uint32_t mmax, umax, pmax;  // LDS capacity in markers, cells and stencil points
+  // adjoint recording (single body): du before and imp of every sweep
+  double *du_hist, *imp_hist;
+  int64_t M;
+  uint32_t U;
+  // diagnostics ring (null: none); ring_pos[0] = slot, ring_pos[1] = blocks done
+  IbmDevDiag* ring;
+  uint32_t cap;
+  uint32_t* ring_pos;
+  int K;
+};
+
+// Dynamic LDS bytes of k_ibm_sweeps at a staging level.
+static inline size_t ibm_lds_bytes(int level, uint32_t mmax, uint32_t umax, uint32_t pmax) {
+  size_t b = ((size_t)mmax * 3 + (size_t)umax * 11) * sizeof(double);
+  if (level >= 2) b += ((size_t)mmax * 7 + pmax) * sizeof(double) + ((size_t)mmax + 1 + pmax) * sizeof(uint32_t);
+  return b;
+}
+
+// One body's view: LDS arrays (cell u at u - u0, marker m at m - m0, point p
+// at p - p0) or, for what is not staged, the global arrays.
+struct IbmView {
+  double *imp, *unow, *du, *spread, *rho, *ksum;
+  const uint32_t *mk_off, *pt_u;
+  const double *pt_w, *target, *pos, *weight;
+  int64_t m0;
+  uint32_t u0, p0;  // p0: subtracted from point indices (0 when unstaged)
+  int64_t mb;       // subtracted from marker indices of the marker arrays
+};
+
+// Per-marker sums over the stencil points in point order (solver.rs:
+// 3118-3140): the interpolated velocity sum w (u_now + du), with MOB the
+// mobility row sum w ksum / rho, and the represented force sum imp w.
+template <bool MOB>
+__device__ __forceinline__ void marker_sums(const IbmView& V, int64_t m, const double im[3], double um[3],
+                                            double& mob, double rep[3]) {
+  const uint32_t p0 = V.mk_off[m - V.mb] - V.p0, p1 = V.mk_off[m - V.mb + 1] - V.p0;
+  for (uint32_t p = p0; p < p1; ++p) {
+    const size_t u = V.pt_u[p] - V.u0;
+    const double w = V.pt_w[p];
+    for (int c = 0; c < 3; ++c) {
+      rep[c] += im[c] * w;
+      um[c] += w * (V.unow[3 * u + c] + V.du[3 * u + c]);
+    }
+    if (MOB) mob += w * V.ksum[u] / fmax(V.rho[u], 1.0e-30);
+  }
+}
+
+__global__ __launch_bounds__(1024) void k_ibm_sweeps(IbmSweepArgs a) {
+  extern __shared__ double dyn[];
+  __shared__ double sh[9][256];
+  __shared__ int conv;
+  const int k = blockIdx.x;
+  const int tid = threadIdx.x, nth = blockDim.x;
+  const int64_t m0 = a.moff[k], m1 = a.moff[k + 1];
+  const uint32_t u0 = a.uoff[k], u1 = a.uoff[k + 1];
+  const uint32_t nm = (uint32_t)(m1 - m0), nu = u1 - u0;
+  IbmDevDiag* diag = a.diag + k;
+  IbmView V;
+  V.m0 = m0;
+  V.u0 = u0;
+  V.imp = dyn;
+  V.unow = V.imp + 3 * (size_t)a.mmax;
+  V.du = V.unow + 3 * (size_t)a.umax;
+  V.spread = V.du + 3 * (size_t)a.umax;
+  V.rho = V.spread + 3 * (size_t)a.umax;
+  V.ksum = V.rho + a.umax;
+  if (tid == 0) {
+    memset(diag, 0, sizeof(IbmDevDiag));
+    conv = 0;
+  }
+  // stage: cell state (du, spread start at zero); at level 2 the markers too
+  for (uint32_t l = tid; l < nu; l += nth) {
+    const size_t u = u0 + l;
+    for (int c = 0; c < 3; ++c) {
+      V.unow[3 * l + c] = a.unow[3 * u + c];
+      V.du[3 * l + c] = 0.0;
+      V.spread[3 * l + c] = 0.0;
+    }
+    V.rho[l] = a.rho[u];
+    V.ksum[l] = a.ksum[u];
+  }
+  if (a.level >= 2) {
+    double* tg = V.ksum + a.umax;
+    double* ps = tg + 3 * (size_t)a.mmax;
+    double* wt = ps + 3 * (size_t)a.mmax;
+    double* pw = wt + a.mmax;
+    uint32_t* mo = (uint32_t*)(pw + a.pmax);
+    uint32_t* pu = mo + a.mmax + 1;
+    const uint32_t pbeg = a.mk_off[m0], np = a.mk_off[m1] - pbeg;
+    for (uint32_t l = tid; l < nm; l += nth) {
+      const int64_t m = m0 + l;
+      for (int c = 0; c < 3; ++c) {
+        tg[3 * l + c] = a.target[3 * m + c];
+        ps[3 * l + c] = a.pos[3 * m + c];
+      }
+      wt[l] = a.weight[m];
+    }
+    for (uint32_t l = tid; l <= nm; l += nth) mo[l] = a.mk_off[m0 + l];
+    for (uint32_t l = tid; l < np; l += nth) {
+      pu[l] = a.pt_u[pbeg + l];
+      pw[l] = a.pt_w[pbeg + l];
+    }
+    V.mk_off = mo;
+    V.pt_u = pu;
+    V.pt_w = pw;
+    V.target = tg;
+    V.pos = ps;
+    V.weight = wt;
+    V.mb = m0;
+    V.p0 = pbeg;
+  } else {
+    V.mk_off = a.mk_off;
+    V.pt_u = a.pt_u;
+    V.pt_w = a.pt_w;
+    V.target = a.target;
+    V.pos = a.pos;
+    V.weight = a.weight;
+    V.mb = 0;
+    V.p0 = 0;
+  }
+  __syncthreads();
+  for (int it = 0; it < a.max_iter; ++it) {
+    if (a.du_hist)
+      for (uint32_t l = tid; l < nu; l += nth)
+        for (int c = 0; c < 3; ++c) a.du_hist[3 * ((size_t)a.U * it + u0 + l) + c] = V.du[3 * (size_t)l + c];
+    if (it == 0) {  // later sweeps take the impulses the previous sweep's diagnostics computed
+      for (int64_t m = m0 + tid; m < m1; m += nth) {
+        double um[3] = {0.0, 0.0, 0.0}, mob = 0.0, rep[3] = {0.0, 0.0, 0.0};
+        const double none[3] = {0.0, 0.0, 0.0};
+        marker_sums<true>(V, m, none, um, mob, rep);
+        const double* t = V.target + 3 * (m - V.mb);
+        for (int c = 0; c < 3; ++c) {
+          const double slip = t[c] - um[c];
+          V.imp[3 * (m - m0) + c] = mob == 0.0 ? 0.0 : a.relax * slip / mob;
+        }
+      }
+      __syncthreads();
+    }
+    if (a.imp_hist)
+      for (uint32_t l = tid; l < nm; l += nth)
+        for (int c = 0; c < 3; ++c) a.imp_hist[3 * ((size_t)a.M * it + m0 + l) + c] = V.imp[3 * (size_t)l + c];
+    // spread and du per cell, over the cell's markers in marker order
+    for (uint32_t u = u0 + tid; u < u1; u += nth) {
+      const uint32_t l = u - u0;
+      const double inv_rho = 1.0 / fmax(V.rho[l], 1.0e-30);
+      double sp[3], dd[3];
+      for (int c = 0; c < 3; ++c) {
+        sp[c] = V.spread[3 * l + c];
+        dd[c] = V.du[3 * l + c];
+      }
+      const uint32_t e0 = a.uc_off[u], e1 = a.uc_off[u + 1];
+      for (uint32_t eb = e0; eb < e1; eb += 4) {
+        int64_t mm[4];
+        double ww[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const uint32_t e = min(eb + (uint32_t)j, e1 - 1);
+          mm[j] = (int64_t)a.uc_m[e] - m0;
+          ww[j] = a.uc_w[e];
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          if (eb + (uint32_t)j >= e1) break;
+          for (int c = 0; c < 3; ++c) {
+            const double cf = V.imp[3 * mm[j] + c] * ww[j];
+            sp[c] += cf;
+            dd[c] += cf * inv_rho;
+          }
+        }
+      }
+      for (int c = 0; c < 3; ++c) {
+        V.spread[3 * l + c] = sp[c];
+        V.du[3 * l + c] = dd[c];
+      }
+    }
+    __syncthreads();
+    // per marker, on the reduction's 256 lanes (lane t: markers t, t + 256,
+    // ... in order): represented force, torque, post-sweep slip, and (when a
+    // sweep follows) the next sweep's impulse from the same interpolated
+    // velocity, written over this sweep's impulse of the same marker; the
+    // lane accumulates its reduction terms as it goes
+    const bool next = it + 1 < a.max_iter;
+    if (tid < 256) {
+      double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // tq[3], rep[3], slipmax, ssq, wsum
+      for (int64_t m = m0 + tid; m < m1; m += 256) {
+        double rep[3] = {0.0, 0.0, 0.0}, um[3] = {0.0, 0.0, 0.0};
+        double im[3];
+        for (int c = 0; c < 3; ++c) im[c] = V.imp[3 * (m - m0) + c];
+        double mob = 0.0;
+        if (next)
+          marker_sums<true>(V, m, im, um, mob, rep);
+        else
+          marker_sums<false>(V, m, im, um, mob, rep);
+        const double* t = V.target + 3 * (m - V.mb);
+        if (next)
+          for (int c = 0; c < 3; ++c) {
+            const double slip = t[c] - um[c];
+            V.imp[3 * (m - m0) + c] = mob == 0.0 ? 0.0 : a.relax * slip / mob;
+          }
+        const double* pm = V.pos + 3 * (m - V.mb);
+        const double r[3] = {pm[0] - a.cen[3 * k], pm[1] - a.cen[3 * k + 1], pm[2] - a.cen[3 * k + 2]};
+        const double tq[3] = {r[1] * rep[2] - r[2] * rep[1], r[2] * rep[0] - r[0] * rep[2],
+                              r[0] * rep[1] - r[1] * rep[0]};
+        for (int c = 0; c < 3; ++c) {
+          acc[c] += tq[c];
+          acc[3 + c] += rep[c];
+        }
+        const double sl[3] = {t[0] - um[0], t[1] - um[1], t[2] - um[2]};
+        const double slip = sqrt(0.0 + sl[0] * sl[0] + sl[1] * sl[1] + sl[2] * sl[2]);
+        if (V.mk_off[m - V.mb + 1] > V.mk_off[m - V.mb]) {
+          const double w = V.weight[m - V.mb];
+          acc[6] = fmax(acc[6], slip);
+          acc[7] += w * slip * slip;
+          acc[8] += w;
+        }
+      }
+      for (int j = 0; j < 9; ++j) sh[j][tid] = acc[j];
+    }
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+      if (tid < w)
+        for (int j = 0; j < 9; ++j)
+          sh[j][tid] = j == 6 ? fmax(sh[j][tid], sh[j][tid + w]) : sh[j][tid] + sh[j][tid + w];
+      __syncthreads();
+    }
+    if (tid == 0) {
+      for (int c = 0; c < 3; ++c) {
+        diag->torque[c] -= sh[c][0];
+        diag->marker_force[c] += sh[3 + c][0];
+        diag->fluid_force[c] += sh[3 + c][0];
+      }
+      diag->iterations = it + 1;
+      diag->slip_max = sh[6][0];
+      diag->slip_rms = sh[8][0] > 0.0 ? sqrt(sh[7][0] / sh[8][0]) : 0.0;
+      diag->slip_max_rel = diag->slip_max / a.scale[k];
+      diag->slip_rms_rel = diag->slip_rms / a.scale[k];
+      const double* ff = diag->fluid_force;
+      const double* mf = diag->marker_force;
+      const double e[3] = {ff[0] - mf[0], ff[1] - mf[1], ff[2] - mf[2]};
+      const double ne = sqrt(0.0 + e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
+      const double nmf = sqrt(0.0 + mf[0] * mf[0] + mf[1] * mf[1] + mf[2] * mf[2]);
+      diag->mom_err = ne / fmax(nmf, 1.0e-12);
+      if (diag->slip_max_rel <= a.tol) diag->converged = 1;
+      conv = diag->converged;
+    }
+    __syncthreads();
+    if (conv) break;
+  }
+  // force_field += T(spread)
+  if (u1 > u0) {
+    float* F = a.dom[k].F_cur;
+    for (uint32_t u = u0 + tid; u < u1; u += nth) {
+      const size_t i = a.uc_cell[u];
+      const uint32_t l = u - u0;
+      for (int c = 0; c < 3; ++c) F[3 * i + c] = F[3 * i + c] + (float)V.spread[3 * l + c];
+    }
+  }
+  // append this body's diagnostics to the ring at the current slot; the last
+  // block to finish advances the slot (every block has read it by then)
+  if (a.ring && tid == 0) {
+    // (the slot's value is in hand before the count is bumped; the ring and
+    // both words are read only by later launches)
+    const uint32_t slot = a.ring_pos[0];
+    if (slot < a.cap) a.ring[(size_t)slot * a.K + k] = *diag;
+    if (atomicAdd(&a.ring_pos[1], 1u) == (uint32_t)a.K - 1) {
+      a.ring_pos[0] = slot + 1;
+      a.ring_pos[1] = 0u;
+    }
+  }
+}
+
 // marker_stencil, rotating_ibm.rs:163-224 (host, f64)
 static void marker_stencil(const double p[3], const uint32_t dims[3], int radius, std::vector<int64_t>& cells,
                            std::vector<double>& w) {
@@ -386,7 +707,7 @@ static hipError_t scratch(T** dst, size_t n) {
 
 void ibm_destroy(IbmState* s) {
   if (!s) return;
-  void* ptrs[] = {s->moff,    s->cen,    s->scale,   s->dom,    s->pos,    s->weight, s->target, s->mk_off,
+  void* ptrs[] = {s->uc_m,    s->uc_w,   s->moff,    s->uoff,   s->cen,    s->scale,   s->dom,    s->pos,    s->weight, s->target, s->mk_off,
                   s->mk_inst, s->pt_u,   s->pt_m,    s->pt_w,   s->uc_cell, s->uc_inst, s->uc_off, s->uc_ent,
                   s->ksum,    s->unow,   s->rho,     s->du,     s->spread, s->imp,    s->imp_next, s->mrep, s->mtq,
                   s->mslip,   s->mact,   s->diag,    s->imp_hist, s->du_hist, s->a_T, s->a_um, s->a_mob,
@@ -402,7 +723,7 @@ static IbmState* ibm_build(const std::vector<const IbmHost*>& parts, const lbm_i
   s->K = (int)parts.size();
   s->cfg = cfg;
   std::vector<double> P, W, T, cen, scale;
-  std::vector<uint32_t> moff{0}, mk_off{0}, mk_inst, pt_u, pt_m, uc_cell, uc_inst, uc_off, uc_ent;
+  std::vector<uint32_t> moff{0}, uoff{0}, mk_off{0}, mk_inst, pt_u, pt_m, uc_cell, uc_inst, uc_off, uc_ent;
   std::vector<double> pt_w, ksum;
   for (int k = 0; k < s->K; ++k) {
     const IbmHost& h = *parts[k];
@@ -429,8 +750,15 @@ static IbmState* ibm_build(const std::vector<const IbmHost*>& parts, const lbm_i
       uc_off.push_back(p0 + h.uc_off[u]);
       ksum.push_back(h.ksum[u]);
     }
+    uoff.push_back((uint32_t)uc_cell.size());
   }
   uc_off.push_back((uint32_t)pt_u.size());
+  std::vector<uint32_t> uc_m(uc_ent.size());
+  std::vector<double> uc_w(uc_ent.size());
+  for (size_t e = 0; e < uc_ent.size(); ++e) {
+    uc_m[e] = pt_m[uc_ent[e]];
+    uc_w[e] = pt_w[uc_ent[e]];
+  }
   s->M = (int64_t)mk_inst.size();
   s->P = (uint32_t)pt_u.size();
   s->U = (uint32_t)uc_cell.size();
@@ -441,6 +769,33 @@ static IbmState* ibm_build(const std::vector<const IbmHost*>& parts, const lbm_i
     if (e == hipSuccess) e = r;
   };
   chain(upload(&s->moff, moff));
+  chain(upload(&s->uoff, uoff));
+  // the fused path when the largest body's state fits in LDS beside the
+  // kernel's 18 KB of reduction arrays: level 2 (state and marker stencils)
+  // if possible, else level 1 (state only); LBM_IBM_PATH=wide forces the
+  // wide path (tuning and A/B tests)
+  {
+    for (size_t k = 0; k + 1 < moff.size(); ++k) {
+      s->mmax = std::max(s->mmax, moff[k + 1] - moff[k]);
+      s->pmax = std::max(s->pmax, mk_off[moff[k + 1]] - mk_off[moff[k]]);
+    }
+    for (size_t k = 0; k + 1 < uoff.size(); ++k) s->umax = std::max(s->umax, uoff[k + 1] - uoff[k]);
+    int dev = 0, lim = 0;
+    const char* path = getenv("LBM_IBM_PATH");
+    const bool wide = path && strcmp(path, "wide") == 0;
+    const int force_level = path && strcmp(path, "fused1") == 0 ? 1 : 0;
+    if (!wide && s->mmax > 0 && hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess) {
+      const size_t fixed = 9 * 256 * sizeof(double) + 64;
+      for (int level = force_level ? force_level : 2; level >= 1 && !s->lds_bytes; --level) {
+        const size_t need = ibm_lds_bytes(level, s->mmax, s->umax, s->pmax);
+        if (need + fixed <= (size_t)lim) {
+          s->lds_bytes = need;
+          s->level = level;
+        }
+      }
+    }
+  }
   chain(upload(&s->cen, cen));
   chain(upload(&s->scale, scale));
   chain(scratch(&s->dom, s->K));
@@ -456,6 +811,8 @@ static IbmState* ibm_build(const std::vector<const IbmHost*>& parts, const lbm_i
   chain(upload(&s->uc_inst, uc_inst));
   chain(upload(&s->uc_off, uc_off));
   chain(upload(&s->uc_ent, uc_ent));
+  chain(upload(&s->uc_m, uc_m));
+  chain(upload(&s->uc_w, uc_w));
   chain(upload(&s->ksum, ksum));
   chain(scratch(&s->unow, 3 * (size_t)s->U));
   chain(scratch(&s->rho, s->U));
@@ -605,9 +962,10 @@ IbmDomain ibm_domain_of(const FieldsView& v) {
 }
 
 template <int Q, typename S>
-static void gather_q(const FieldsView& v, IbmState* s, uint32_t reset_K) {
-  hipLaunchKernelGGL((k_ibm_gather<Q, S>), dim3(nblk(s->U)), dim3(256), 0, v.stream, s->dom, v.parity, s->uc_cell,
-                     s->uc_inst, s->U, v.field_present ? 1 : 0, s->unow, s->rho, s->du, s->spread, s->diag, reset_K);
+static void gather_q(const FieldsView& v, IbmState* s, uint32_t reset_K, const uint32_t* clr, uint32_t nclr) {
+  hipLaunchKernelGGL((k_ibm_gather<Q, S>), dim3(nblk(std::max(s->U, nclr))), dim3(256), 0, v.stream, s->dom, v.parity,
+                     s->uc_cell, s->uc_inst, s->U, v.field_present ? 1 : 0, s->unow, s->rho, s->du, s->spread, s->diag,
+                     reset_K, clr, nclr, v.F_cur);
 }
 
 static void diag_to_abi(const IbmDevDiag& h, lbm_ibm_diag* out) {
@@ -624,20 +982,23 @@ static void diag_to_abi(const IbmDevDiag& h, lbm_ibm_diag* out) {
   out->momentum_error_rel = h.mom_err;
 }
 
+static int ibm_enqueue_impl(const FieldsView& v, IbmState* s, void* ring, uint32_t cap, uint32_t* pos,
+                            bool record, const uint32_t* clr, uint32_t nclr);
+
+int ibm_enqueue(const FieldsView& v, IbmState* s, void* ring, uint32_t cap, uint32_t* pos, const uint32_t* clear_list,
+                uint32_t clear_n) {
+  return ibm_enqueue_impl(v, s, ring, cap, pos, false, clear_list, clear_n);
+}
+
 __global__ void k_copy_d(double* dst, const double* src, size_t n) {
   const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k < n) dst[k] = src[k];
 }
 
-static int ibm_enqueue_impl(const FieldsView& v, IbmState* s, void* ring, uint32_t cap, uint32_t* pos,
-                            bool record);
-
-int ibm_enqueue(const FieldsView& v, IbmState* s, void* ring, uint32_t cap, uint32_t* pos) {
-  return ibm_enqueue_impl(v, s, ring, cap, pos, false);
-}
-
-static int ibm_enqueue_impl(const FieldsView& v, IbmState* s, void* ring, uint32_t cap, uint32_t* pos,
-                            bool record) {
+// wide path: one kernel per phase across the whole GPU (13 launches per
+// step at three sweeps), for bodies whose state does not fit in LDS
+static int ibm_enqueue_wide(const FieldsView& v, IbmState* s, void* ring, uint32_t cap, uint32_t* pos, bool record,
+                            const uint32_t* clr, uint32_t nclr) {
   if (!s->dom_valid) {
     if (v.ctx_err) *v.ctx_err = "IBM domain table not set";
     return LBM_ERR_STATE;
@@ -650,15 +1011,15 @@ static int ibm_enqueue_impl(const FieldsView& v, IbmState* s, void* ring, uint32
   hipStream_t st = v.stream;
   // the gather clears the diagnostics when it has a thread per instance
   const uint32_t reset_K = s->U >= (uint32_t)s->K ? (uint32_t)s->K : 0u;
-  if (s->U) {
+  if (s->U || nclr) {
     if (v.Q == 19) {
-      if (v.storage == kF32) gather_q<19, float>(v, s, reset_K);
-      else if (v.storage == kF16) gather_q<19, _Float16>(v, s, reset_K);
-      else gather_q<19, bf16s>(v, s, reset_K);
+      if (v.storage == kF32) gather_q<19, float>(v, s, reset_K, clr, nclr);
+      else if (v.storage == kF16) gather_q<19, _Float16>(v, s, reset_K, clr, nclr);
+      else gather_q<19, bf16s>(v, s, reset_K, clr, nclr);
     } else {
-      if (v.storage == kF32) gather_q<27, float>(v, s, reset_K);
-      else if (v.storage == kF16) gather_q<27, _Float16>(v, s, reset_K);
-      else gather_q<27, bf16s>(v, s, reset_K);
+      if (v.storage == kF32) gather_q<27, float>(v, s, reset_K, clr, nclr);
+      else if (v.storage == kF16) gather_q<27, _Float16>(v, s, reset_K, clr, nclr);
+      else gather_q<27, bf16s>(v, s, reset_K, clr, nclr);
     }
     AUXCHK(v, hipGetLastError());
   }
@@ -696,6 +1057,72 @@ static int ibm_enqueue_impl(const FieldsView& v, IbmState* s, void* ring, uint32
     hipLaunchKernelGGL(k_ibm_record, dim3(1), dim3(64), 0, st, s->diag, s->K, (IbmDevDiag*)ring, cap, pos);
     AUXCHK(v, hipGetLastError());
   }
+  return LBM_OK;
+}
+
+static int ibm_enqueue_impl(const FieldsView& v, IbmState* s, void* ring, uint32_t cap, uint32_t* pos,
+                            bool record, const uint32_t* clr, uint32_t nclr) {
+  if (!s->lds_bytes) return ibm_enqueue_wide(v, s, ring, cap, pos, record, clr, nclr);
+  if (!s->dom_valid) {
+    if (v.ctx_err) *v.ctx_err = "IBM domain table not set";
+    return LBM_ERR_STATE;
+  }
+  for (int k = 0; k < s->K; ++k)
+    if (!s->dom_h[k].F_cur) {
+      if (v.ctx_err) *v.ctx_err = "force arrays missing";
+      return LBM_ERR_STATE;
+    }
+  hipStream_t st = v.stream;
+  const uint32_t reset_K = 0;  // the sweep kernel clears the diagnostics
+  if (s->U || nclr) {
+    if (v.Q == 19) {
+      if (v.storage == kF32) gather_q<19, float>(v, s, reset_K, clr, nclr);
+      else if (v.storage == kF16) gather_q<19, _Float16>(v, s, reset_K, clr, nclr);
+      else gather_q<19, bf16s>(v, s, reset_K, clr, nclr);
+    } else {
+      if (v.storage == kF32) gather_q<27, float>(v, s, reset_K, clr, nclr);
+      else if (v.storage == kF16) gather_q<27, _Float16>(v, s, reset_K, clr, nclr);
+      else gather_q<27, bf16s>(v, s, reset_K, clr, nclr);
+    }
+    AUXCHK(v, hipGetLastError());
+  }
+  IbmSweepArgs a{};
+  a.moff = s->moff;
+  a.uoff = s->uoff;
+  a.mk_off = s->mk_off;
+  a.pt_u = s->pt_u;
+  a.uc_off = s->uc_off;
+  a.uc_m = s->uc_m;
+  a.uc_cell = s->uc_cell;
+  a.pt_w = s->pt_w;
+  a.uc_w = s->uc_w;
+  a.target = s->target;
+  a.pos = s->pos;
+  a.cen = s->cen;
+  a.weight = s->weight;
+  a.scale = s->scale;
+  a.ksum = s->ksum;
+  a.unow = s->unow;
+  a.rho = s->rho;
+  a.diag = s->diag;
+  a.dom = s->dom;
+  a.relax = s->cfg.relaxation;
+  a.tol = s->cfg.slip_tolerance;
+  a.max_iter = s->cfg.max_iterations;
+  a.level = s->level;
+  a.mmax = s->mmax;
+  a.umax = s->umax;
+  a.pmax = s->pmax;
+  a.du_hist = record ? s->du_hist : nullptr;
+  a.imp_hist = record ? s->imp_hist : nullptr;
+  a.M = s->M;
+  a.U = s->U;
+  a.ring = (IbmDevDiag*)ring;
+  a.cap = cap;
+  a.ring_pos = pos;
+  a.K = s->K;
+  hipLaunchKernelGGL(k_ibm_sweeps, dim3(s->K), dim3(1024), s->lds_bytes, st, a);
+  AUXCHK(v, hipGetLastError());
   return LBM_OK;
 }
 
@@ -860,7 +1287,7 @@ int ibm_adjoint_prepare(const FieldsView& v, IbmState* s) {
 int ibm_enqueue_recorded(const FieldsView& v, IbmState* s) {
   int rc = ibm_adjoint_prepare(v, s);
   if (rc) return rc;
-  return ibm_enqueue_impl(v, s, nullptr, 0, nullptr, true);
+  return ibm_enqueue_impl(v, s, nullptr, 0, nullptr, true, nullptr, 0);
 }
 
 int ibm_adjoint(const FieldsView& v, IbmState* s, const double* Fbar, const double* Gbar, double tq_bar, double* lam,

@@ -59,7 +59,10 @@ int ibm_apply(const FieldsView& v, IbmState* s, lbm_ibm_diag* diag);
 // The same update without the host read-back: the K diagnostics go to entry
 // *pos of a device ring of `cap` entries (K x ibm_ring_entry_bytes each) and
 // *pos advances.  Launches kernels only, so a stream capture can record it.
-int ibm_enqueue(const FieldsView& v, IbmState* s, void* ring, uint32_t cap, uint32_t* pos);
+// With clear_list, the force field is first cleared at those clear_n cells
+// (single body), in the same launch as the gather.
+int ibm_enqueue(const FieldsView& v, IbmState* s, void* ring, uint32_t cap, uint32_t* pos,
+                const uint32_t* clear_list = nullptr, uint32_t clear_n = 0);
 size_t ibm_ring_entry_bytes();
 // Adjoint support (single body).  ibm_enqueue_recorded is ibm_enqueue that
 // also keeps every sweep's impulses and du for ibm_adjoint, which runs the

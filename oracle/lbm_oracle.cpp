@@ -179,6 +179,49 @@ static void shifted_moments(const MomentTransform& t, const double u[3], const d
   }
 }
 
+// Factorised D3Q27 transforms (test infrastructure speed-up, not a
+// reference function).  The D3Q27 basis is the tensor product of the 1-D
+// exponents 0..2 (central_basis, kernels.rs:162-183: basis index m =
+// 9 ex + 3 ey + ez), so the central moments sum_q (c - u)^e f_q factor into
+// three passes of 1-D three-point lines, and the inverse (raw moments by
+// the binomial shift +u, then populations) into three inverse passes.  Same
+// mathematics as the generic matrix path in exact arithmetic; the two agree
+// to f64 round-off (pinned in tests).  On by default for D3Q27.
+static bool g_cm_factorised = true;
+
+// G[(cx+1)*9 + (cy+1)*3 + (cz+1)] = population of velocity c  ->
+// G[9 ex + 3 ey + ez] = central moment of exponent e
+static void chimera_fwd(double* G, const double u[3]) {
+  auto line = [](double& a0, double& a1, double& a2, double uu) {
+    const double gm = a0, g0 = a1, gp = a2;
+    const double k0 = (gm + g0) + gp;
+    const double k1 = (gp - gm) - uu * k0;
+    const double k2 = ((gp + gm) - 2.0 * uu * (gp - gm)) + uu * uu * k0;
+    a0 = k0;
+    a1 = k1;
+    a2 = k2;
+  };
+  for (int ab = 0; ab < 9; ++ab) line(G[ab * 3 + 0], G[ab * 3 + 1], G[ab * 3 + 2], u[2]);
+  for (int a = 0; a < 3; ++a)
+    for (int c = 0; c < 3; ++c) line(G[a * 9 + c], G[a * 9 + 3 + c], G[a * 9 + 6 + c], u[1]);
+  for (int bc = 0; bc < 9; ++bc) line(G[bc], G[9 + bc], G[18 + bc], u[0]);
+}
+// central moments -> populations (inverse of chimera_fwd)
+static void chimera_inv(double* G, const double u[3]) {
+  auto line = [](double& a0, double& a1, double& a2, double uu) {
+    const double k0 = a0, k1 = a1, k2 = a2;
+    const double m1 = k1 + uu * k0;
+    const double m2 = (k2 + 2.0 * uu * k1) + uu * uu * k0;
+    a0 = 0.5 * (m2 - m1);
+    a1 = k0 - m2;
+    a2 = 0.5 * (m1 + m2);
+  };
+  for (int bc = 0; bc < 9; ++bc) line(G[bc], G[9 + bc], G[18 + bc], u[0]);
+  for (int a = 0; a < 3; ++a)
+    for (int c = 0; c < 3; ++c) line(G[a * 9 + c], G[a * 9 + 3 + c], G[a * 9 + 6 + c], u[1]);
+  for (int ab = 0; ab < 9; ++ab) line(G[ab * 3 + 0], G[ab * 3 + 1], G[ab * 3 + 2], u[2]);
+}
+
 // populations_to_central_moments, kernels.rs:318-330
 static void pops_to_central(const MomentTransform& t, const double u[3], const double* pop, double* out) {
   double raw[27];
@@ -510,9 +553,22 @@ struct Solver {
       feq_phys[q] = L.w[q] * r * (1.0 + 3.0 * cu + 4.5 * cu * cu - 1.5 * usq);
     }
     double mom[27], eq[27], src_mom[27];
-    pops_to_central(tr, u, phys, mom);
-    pops_to_central(tr, u, feq_phys, eq);
-    pops_to_central(tr, u, src, src_mom);
+    const bool fact = Q == 27 && g_cm_factorised;
+    if (fact) {
+      for (int q = 0; q < 27; ++q) {
+        const int g = (L.c[q][0] + 1) * 9 + (L.c[q][1] + 1) * 3 + (L.c[q][2] + 1);
+        mom[g] = phys[q];
+        eq[g] = feq_phys[q];
+        src_mom[g] = src[q];
+      }
+      chimera_fwd(mom, u);
+      chimera_fwd(eq, u);
+      chimera_fwd(src_mom, u);
+    } else {
+      pops_to_central(tr, u, phys, mom);
+      pops_to_central(tr, u, feq_phys, eq);
+      pops_to_central(tr, u, src, src_mom);
+    }
     double os_base = has_omega ? (double)omega_field[c] : (double)kp_os;
     double usq2 = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
     double vc = -0.16 * usq2;
@@ -544,6 +600,14 @@ struct Solver {
       double dev_neq = mom[idx] - eq[idx] - bulk_neq;
       double dev_src = src_mom[idx] - bulk_src;
       post[idx] = eq[idx] + (1.0 - os) * dev_neq + 0.5 * bulk_src + (1.0 - 0.5 * os) * dev_src;
+    }
+    if (fact) {
+      chimera_inv(post, u);
+      for (int q = 0; q < 27; ++q) {
+        const int g = (L.c[q][0] + 1) * 9 + (L.c[q][1] + 1) * 3 + (L.c[q][2] + 1);
+        f[q * np + i] = (T)(post[g] - L.w[q]);
+      }
+      return;
     }
     double raw_post[27];
     shifted_moments(tr, u, post, raw_post, false);
@@ -1284,6 +1348,7 @@ struct Solver {
     long x, y, z;
     double w;
   };
+  std::vector<long> ibm_slot;  // IBM: compact index of each stencil cell, -1 elsewhere
   // marker_stencil, rotating_ibm.rs:163-224.  `d` is the lattice's
   // dimension: axes a >= d take the single point (0, 1.0) (rotating_ibm.rs:
   // 176-179), the reference's 2D case, used here by one-cell-deep boxes
@@ -1327,27 +1392,56 @@ struct Solver {
   int apply_ibm(int nm, const double* pos, const double* weight, const double center[3], const double omega[3],
                 int max_iter, double tol, int radius, double relax, double* diag_out, int stencil_dim = 3) {
     int dims[3] = {nx, ny, nz};
-    size_t n = nc;
-    std::vector<double> u_now(3 * n, 0.0);
-    std::vector<uint8_t> sol(n, 0);
-    for (long z = 0; z < nz; ++z)
-      for (long y = 0; y < ny; ++y)
-        for (long x = 0; x < nx; ++x) {
-          size_t gi = cidx(x, y, z), pi = pidx(x, y, z);
-          sol[gi] = solid[pi];
-          if (sol[gi]) continue;
-          double mom[3] = {0.0, 0.0, 0.0};
-          for (int q = 0; q < L.Q; ++q) {
-            double fq = (double)f[q * np + pi];
-            for (int a = 0; a < 3; ++a) mom[a] += (double)L.c[q][a] * fq;
-          }
-          double frc[3] = {(double)force[0], (double)force[1], (double)force[2]};
-          if (has_ff)
-            for (int a = 0; a < 3; ++a) frc[a] += (double)ff[3 * gi + a];
-          double inv_rho = 1.0 / std::max((double)rho[gi], 1.0e-30);
-          for (int a = 0; a < 3; ++a) u_now[3 * gi + a] = (mom[a] + 0.5 * frc[a]) * inv_rho;
+    // Only the stencil cells enter the update: u_now, du, spread and the
+    // kernel sums live in compact arrays over the distinct stencil cells
+    // (slot[gi] = compact index), with the reference's per-cell arithmetic
+    // and accumulation order (markers in order, stencil points in order).
+    std::vector<std::vector<SP>> st(nm);
+    for (int m = 0; m < nm; ++m) {
+      marker_stencil(&pos[3 * m], dims, radius, st[m], stencil_dim);
+      if (st[m].empty()) return -1;
+    }
+    if (ibm_slot.size() != nc) ibm_slot.assign(nc, -1);
+    std::vector<size_t> cells;
+    for (int m = 0; m < nm; ++m)
+      for (auto& sp : st[m]) {
+        const size_t gi = cidx(sp.x, sp.y, sp.z);
+        if (ibm_slot[gi] < 0) {
+          ibm_slot[gi] = (long)cells.size();
+          cells.push_back(gi);
         }
-    std::vector<double> du(3 * n, 0.0), spread(3 * n, 0.0);
+      }
+    const size_t nk = cells.size();
+    struct Clear {  // leaves slot all -1 on every return path
+      std::vector<long>& slot;
+      const std::vector<size_t>& cells;
+      ~Clear() {
+        for (size_t gi : cells) slot[gi] = -1;
+      }
+    } clear_slots{ibm_slot, cells};
+    std::vector<double> u_now(3 * nk, 0.0);
+    std::vector<uint8_t> sol(nk, 0);
+    std::vector<double> rho_k(nk, 1.0);
+    for (size_t k = 0; k < nk; ++k) {
+      const size_t gi = cells[k];
+      const long x = (long)(gi % (size_t)nx), y = (long)((gi / (size_t)nx) % (size_t)ny),
+                 z = (long)(gi / ((size_t)nx * ny));
+      const size_t pi = pidx(x, y, z);
+      sol[k] = solid[pi];
+      rho_k[k] = (double)rho[gi];
+      if (sol[k]) continue;
+      double mom[3] = {0.0, 0.0, 0.0};
+      for (int q = 0; q < L.Q; ++q) {
+        double fq = (double)f[q * np + pi];
+        for (int a = 0; a < 3; ++a) mom[a] += (double)L.c[q][a] * fq;
+      }
+      double frc[3] = {(double)force[0], (double)force[1], (double)force[2]};
+      if (has_ff)
+        for (int a = 0; a < 3; ++a) frc[a] += (double)ff[3 * gi + a];
+      double inv_rho = 1.0 / std::max(rho_k[k], 1.0e-30);
+      for (int a = 0; a < 3; ++a) u_now[3 * k + a] = (mom[a] + 0.5 * frc[a]) * inv_rho;
+    }
+    std::vector<double> du(3 * nk, 0.0), spread(3 * nk, 0.0);
     double iterations = 0, slip_max = 0, slip_rms = 0, slip_max_rel = 0, slip_rms_rel = 0;
     double torque[3] = {0, 0, 0}, fluid_force[3] = {0, 0, 0}, marker_force[3] = {0, 0, 0};
     double mom_err = 0;
@@ -1364,17 +1458,13 @@ struct Solver {
       max_target = std::max(max_target, norm3(t));
     }
     double scale = std::max(max_target, 1.0);
-    std::vector<std::vector<SP>> st(nm);
-    for (int m = 0; m < nm; ++m) {
-      marker_stencil(&pos[3 * m], dims, radius, st[m], stencil_dim);
-      if (st[m].empty()) return -1;
-    }
-    std::vector<double> ksum(n, 0.0);
+    auto K = [&](const SP& sp) { return (size_t)ibm_slot[cidx(sp.x, sp.y, sp.z)]; };
+    std::vector<double> ksum(nk, 0.0);
     for (int m = 0; m < nm; ++m)
       for (auto& sp : st[m]) {
-        size_t gi = cidx(sp.x, sp.y, sp.z);
-        if (sol[gi]) continue;
-        ksum[gi] += sp.w;
+        size_t k = K(sp);
+        if (sol[k]) continue;
+        ksum[k] += sp.w;
       }
     std::vector<double> imp(3 * (size_t)nm);
     for (int iter = 0; iter < max_iter; ++iter) {
@@ -1384,11 +1474,11 @@ struct Solver {
         target(&pos[3 * m], t);
         double um[3] = {0, 0, 0}, mob = 0.0;
         for (auto& sp : st[m]) {
-          size_t gi = cidx(sp.x, sp.y, sp.z);
-          if (sol[gi]) continue;
-          double rc = (double)rho[gi];
-          for (int a = 0; a < 3; ++a) um[a] += sp.w * (u_now[3 * gi + a] + du[3 * gi + a]);
-          mob += sp.w * ksum[gi] / std::max(rc, 1.0e-30);
+          size_t k = K(sp);
+          if (sol[k]) continue;
+          double rc = rho_k[k];
+          for (int a = 0; a < 3; ++a) um[a] += sp.w * (u_now[3 * k + a] + du[3 * k + a]);
+          mob += sp.w * ksum[k] / std::max(rc, 1.0e-30);
         }
         if (mob == 0.0) continue;
         for (int a = 0; a < 3; ++a) imp[3 * m + a] = relax * (t[a] - um[a]) / mob;
@@ -1396,16 +1486,16 @@ struct Solver {
       for (int m = 0; m < nm; ++m) {
         double rep[3] = {0, 0, 0};
         for (auto& sp : st[m]) {
-          size_t gi = cidx(sp.x, sp.y, sp.z);
-          if (sol[gi]) continue;
+          size_t k = K(sp);
+          if (sol[k]) continue;
           double cf[3] = {imp[3 * m + 0] * sp.w, imp[3 * m + 1] * sp.w, imp[3 * m + 2] * sp.w};
           for (int a = 0; a < 3; ++a) {
-            spread[3 * gi + a] += cf[a];
+            spread[3 * k + a] += cf[a];
             fluid_force[a] += cf[a];
             rep[a] += cf[a];
           }
-          double inv_rho = 1.0 / std::max((double)rho[gi], 1.0e-30);
-          for (int a = 0; a < 3; ++a) du[3 * gi + a] += cf[a] * inv_rho;
+          double inv_rho = 1.0 / std::max(rho_k[k], 1.0e-30);
+          for (int a = 0; a < 3; ++a) du[3 * k + a] += cf[a] * inv_rho;
         }
         for (int a = 0; a < 3; ++a) marker_force[a] += rep[a];
         const double* p = &pos[3 * m];
@@ -1422,10 +1512,10 @@ struct Solver {
         double um[3] = {0, 0, 0};
         bool active = false;
         for (auto& sp : st[m]) {
-          size_t gi = cidx(sp.x, sp.y, sp.z);
-          if (sol[gi]) continue;
+          size_t k = K(sp);
+          if (sol[k]) continue;
           active = true;
-          for (int a = 0; a < 3; ++a) um[a] += sp.w * (u_now[3 * gi + a] + du[3 * gi + a]);
+          for (int a = 0; a < 3; ++a) um[a] += sp.w * (u_now[3 * k + a] + du[3 * k + a]);
         }
         if (!active) continue;
         double s[3] = {t[0] - um[0], t[1] - um[1], t[2] - um[2]};
@@ -1442,13 +1532,17 @@ struct Solver {
       mom_err = norm3(err) / std::max(norm3(marker_force), 1.0e-12);
       if (slip_max_rel <= tol) break;
     }
-    // force_field += T(spread)
+    // force_field += T(spread): every cell, as the reference adds the whole
+    // spread field (zero off the stencils)
     if (!has_ff) {
       ff.assign(nc * 3, T(0));
       has_ff = true;
     }
-    for (size_t c = 0; c < nc; ++c)
-      for (int a = 0; a < 3; ++a) ff[3 * c + a] = ff[3 * c + a] + (T)spread[3 * c + a];
+#pragma omp parallel for schedule(static)
+    for (long c = 0; c < (long)nc; ++c) {
+      const long k = ibm_slot[c];
+      for (int a = 0; a < 3; ++a) ff[3 * c + a] = ff[3 * c + a] + (T)(k >= 0 ? spread[3 * k + a] : 0.0);
+    }
     double d[15] = {iterations,    slip_max,      slip_rms,        slip_max_rel,    slip_rms_rel,
                     torque[0],     torque[1],     torque[2],       fluid_force[0],  fluid_force[1],
                     fluid_force[2], marker_force[0], marker_force[1], marker_force[2], mom_err};
@@ -1761,6 +1855,8 @@ void orc_cm_transform(int Q, double* raw_from_pop, double* pop_from_raw, uint8_t
 // once decaying deviations and tracer tails underflow below 2.2e-308, where
 // x86 arithmetic takes a microcode path; values that small cannot change a
 // quantity of interest.  Off unless a caller asks (fixture generation).
+void orc_set_cm_factorised(int on) { g_cm_factorised = on != 0; }
+
 void orc_flush_subnormals(int on) {
   const unsigned bits = 0x8040u;  // FTZ (bit 15) | DAZ (bit 6)
 #pragma omp parallel

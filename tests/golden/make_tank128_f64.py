@@ -7,7 +7,9 @@ with a D3Q7 tracer (Gaussian sigma = 2 lu at 0.8 H on the axis, Sc = 1),
 and stores the shaft torque of every step and the tracer CV every 10 steps.
 Restatement-generated: the reference ships no such series.
 
-usage: python tests/golden/make_tank128_f64.py [steps]   (about 0.5 s/step on 8 cores)
+usage: python tests/golden/make_tank128_f64.py [steps]   (about 0.25 s/step on 8 cores;
+the committed series: 17,600 steps = 7 impeller revolutions, the upper end
+of the reference's mixing-time band, vb_02_passive_scalar_mixing.rs:7-14)
 """
 import json
 import os
@@ -35,7 +37,7 @@ def tank_params():
 
 
 def main():
-    steps = int(sys.argv[1]) if len(sys.argv) > 1 else 3000
+    steps = int(sys.argv[1]) if len(sys.argv) > 1 else 17600
     # subnormal f64 values (< 2.2e-308: far tracer tails, decayed quiescent
     # deviations) would put x86 on its slow path; flushing them changes no
     # torque or CV digit (checked against an unflushed run of 300 steps)
@@ -44,7 +46,7 @@ def main():
     tc = cases.tank_case(N, nu=nu, collision="cm", Q=27, sigma=2.0)
     t0 = time.time()
     torque, cv, s = cases.run_tank_oracle(tc, steps, prec=8, every=1, scalar_D=nu, d3q7=True,
-                                          cv_every=EVERY_CV, progress=100)
+                                          cv_every=EVERY_CV, progress=500)
     meta = {"n": N, "steps": steps, "nu": nu, "omega": tc["omega"], "D": tc["D"], "T": tc["T"], "sigma": 2.0,
             "scalar": "d3q7", "cv_every": EVERY_CV, "collision": "cm", "Q": 27, "precision": "f64",
             "seconds": round(time.time() - t0, 1)}

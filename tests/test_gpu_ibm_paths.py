@@ -1,0 +1,77 @@
+"""The two device paths of the rotating-IBM update agree to the bit: the
+fused path (one block per body, all sweeps in one launch, state staged in
+LDS) and the wide path (one kernel per phase across the GPU), on one tank
+and on a batch of tanks, over spans with and without the captured graph.
+Force fields, populations and every recorded diagnostic must be identical;
+both are checked against the oracle by the other IBM tests."""
+import os
+
+import numpy as np
+import pytest
+
+import cases
+
+pytestmark = pytest.mark.gpu
+
+
+def _tank(n, tip, nu, path, tol=1e-2):
+    import lbmflow_amd as L
+
+    tc = cases.tank_case(n, nu=nu, tip=tip, collision="trt", Q=19)
+    old = os.environ.get("LBM_IBM_PATH")
+    if path:
+        os.environ["LBM_IBM_PATH"] = path
+    try:
+        s = L.Solver(19, (n, n, n), nu=nu, collision="trt", solid=tc["solid"])
+        s.set_rotating_body(tc["pos"], tc["w"], tc["c"], (0.0, 0.0, tc["omega"]),
+                            L.DirectForcingConfig(slip_tolerance=tol))
+    finally:
+        if old is None:
+            os.environ.pop("LBM_IBM_PATH", None)
+        else:
+            os.environ["LBM_IBM_PATH"] = old
+    return s
+
+
+@pytest.mark.parametrize("path", ["fused1", "fused2"])
+@pytest.mark.parametrize("tol", [1e-2, 1e-12])  # one sweep per step, and all three
+def test_fused_and_wide_ibm_paths_agree(tol, path):
+    """fused1: cell state and impulses staged in LDS; fused2: the marker
+    stencils too (the default when they fit)."""
+    nu = 0.02
+    a = _tank(32, 0.05, nu, path if path != "fused2" else None, tol)
+    b = _tank(32, 0.05, nu, "wide", tol)
+    for steps, graph in [(3, False), (4, True), (5, True)]:
+        ta, da = a.run_with_ibm(steps, graph=graph)
+        tb, db = b.run_with_ibm(steps, graph=graph)
+        assert np.array_equal(ta, tb), f"torque series differ ({steps} steps, graph={graph})"
+        assert repr(da) == repr(db)
+        assert np.array_equal(a.body_force_field(), b.body_force_field())
+        assert np.array_equal(a.populations(), b.populations())
+    print(f"tol={tol} {path}: iterations {da['iterations']}")
+    assert da["iterations"] == (3 if tol < 1e-6 else 1)
+
+
+def test_fused_and_wide_ibm_paths_agree_in_a_batch():
+    import lbmflow_amd as L
+
+    specs = [(24, 0.04, 0.01), (30, 0.02, 0.003), (20, 0.06, 0.02)]
+    runs = []
+    for wide in (False, True):
+        members = [_tank(n, tip, nu, "wide" if wide else None) for n, tip, nu in specs]
+        old = os.environ.get("LBM_IBM_PATH")
+        if wide:
+            os.environ["LBM_IBM_PATH"] = "wide"
+        try:
+            batch = L.Batch(members)
+        finally:
+            if old is None:
+                os.environ.pop("LBM_IBM_PATH", None)
+            else:
+                os.environ["LBM_IBM_PATH"] = old
+        tq = np.concatenate([batch.run(3, graph=False), batch.run(4, graph=True)], axis=1)
+        runs.append((tq, [m.populations() for m in members]))
+        batch.close()
+    assert np.array_equal(runs[0][0], runs[1][0])
+    for pa, pb in zip(runs[0][1], runs[1][1]):
+        assert np.array_equal(pa, pb)
