@@ -131,7 +131,7 @@ def np_error_vs_f64(storages, collision, Q):
     z = np.load(path)
     meta = json.loads(bytes(z["meta"]).decode())
     steps = int(meta["steps"])
-    cfg = tank_config(meta["n"])
+    cfg = dict(tank_config(meta["n"]), nu=meta["nu"])
     rec = np.arange(1, steps + 1)
 
     def np_of(torque_z):
@@ -173,6 +173,60 @@ def small_tank(n, tip, nu, Q=19, collision="trt"):
     pos, w, vel, c = L.radial_impeller((n, n, n), 1.0, T, T, D, T / 3.0, 6, 0.0, omega)
     s.set_rotating_body(pos, w, c, (0.0, 0.0, omega))
     return s, {"n": n, "T": T, "D": D, "omega": omega, "tip": tip, "nu": nu, "markers": len(w)}
+
+
+def workload_surface(args):
+    """The single-phase stirred tank at 256^3 (D3Q27 central moment, fp32,
+    T = H = 240 lu, 4 baffles T/10, Rushton D = C = T/3, tip speed 0.05,
+    Re_imp 5000) with the seeded surface sampling of the blades (one marker
+    per 0.1 lu^2: about 12k markers), 20k steps; Np from the last-half mean
+    shaft torque (runner.rs:1514-1546, qoi.rs:477-505).  The WALE closure
+    runs before every step (les.rs): without a turbulence model this tank
+    (nu = 2.5e-4 lu) diverges after about 2,400 steps."""
+    import lbmflow_amd as L
+
+    n = 256
+    cfg = tank_config(n)
+    s = L.Solver(27, (n, n, n), nu=cfg["nu"], collision="cm", storage=args.storage)
+    s.build_stirred_tank(1.0, cfg["T"], cfg["H"], 4, cfg["baffle_width"], cfg["baffle_thickness"])
+    pos, w, vel, c = L.impeller_surface_markers((n, n, n), 1.0, cfg["T"], cfg["H"], cfg["D"], cfg["clearance"], 6,
+                                                0.0, cfg["omega"], 0.04, 0.1, args.seed)
+    s.set_rotating_body(pos, w, c, (0.0, 0.0, cfg["omega"]))
+    s.set_wale(True)
+    fluid = s.fluid_cells()
+    steps = args.steps
+    torque = []
+    chunk = 1000
+    s.synchronize()
+    t0 = time.perf_counter()
+    done = 0
+    while done < steps:
+        k = min(chunk, steps - done)
+        tq, diag = s.run_with_ibm(k, graph=True)
+        torque.append(-tq[:, 2])
+        done += k
+        print(f"[bench] surface tank: {done}/{steps} steps", file=sys.stderr, flush=True)
+    s.synchronize()
+    dt = time.perf_counter() - t0
+    torque = np.concatenate(torque)
+    rec = np.arange(1, steps + 1)
+    tmean = float(np.mean(torque[rec >= steps // 2]))
+    q = L.power_qois(tmean, cfg["omega"], 1.0, cfg["D"], 1.0)
+    mass = s.total_mass_f64()
+    s.close()
+    return {
+        "metric": "MLUPS and Np, D3Q27 CM stirred tank 256^3 with surface-sampled Rushton markers",
+        "value": round(fluid * steps / dt / 1e6, 2), "unit": "MLUPS", "n_gpus": 1, "steps": steps, "warmup": 0,
+        "ms_per_step": round(1e3 * dt / steps, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32" if args.storage == "f32" else f"f32 math, {args.storage} storage",
+        "data": f"synthetic (tank geometry; surface markers SplitMix64 seed {args.seed}; flow from rest)",
+        "config": {"workload": "stirred tank D3Q27 cm 256^3 + per-step WALE, surface-sampled Rushton IBM",
+                   "grid": [n] * 3,
+                   "markers": int(len(w)), "marker_area_lu2": 0.1, "tip_speed_lu": 0.05, "re_imp": 5000,
+                   "nu_lu": cfg["nu"], "fluid_cells": int(fluid), "parallelism": "single GPU"},
+        "np": q["np"], "torque_mean_lu": tmean, "revolutions": steps * cfg["omega"] / (2 * np.pi),
+        "checks": {"mass_finite": bool(np.isfinite(mass)), "ibm_iterations_last": diag["iterations"]},
+    }
 
 
 def workload_batch(args):
@@ -393,9 +447,10 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=100)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-sweep", action="store_true", help="skip the 16-bit storage and Np-accuracy sweep")
-    ap.add_argument("--workload", default="tank", choices=["tank", "batch", "adjoint", "les", "channel"],
+    ap.add_argument("--workload", default="tank", choices=["tank", "batch", "adjoint", "les", "channel", "surface"],
                     help="tank: the 512^3 benchmark (default); batch: 64-tank sweep; adjoint: torque gradient; "
-                         "les: the tank with per-step WALE; channel: open-face channel")
+                         "les: the tank with per-step WALE; channel: open-face channel; surface: the 256^3 tank "
+                         "with surface-sampled markers, 20k steps, Np")
     ap.add_argument("--seed", type=int, default=20260512)
     ap.add_argument("--snapshots", type=int, default=128, help="adjoint: binomial checkpoint snapshots")
     args = ap.parse_args()
@@ -404,12 +459,13 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     if args.steps is None:
-        args.steps = {"tank": 200, "batch": 5000, "adjoint": 4000, "les": 100, "channel": 200}[args.workload]
+        args.steps = {"tank": 200, "batch": 5000, "adjoint": 4000, "les": 100, "channel": 200,
+                      "surface": 20000}[args.workload]
     if args.warmup is None:
-        args.warmup = {"tank": 500, "batch": 100, "adjoint": 0, "les": 200, "channel": 100}[args.workload]
+        args.warmup = {"tank": 500, "batch": 100, "adjoint": 0, "les": 200, "channel": 100, "surface": 0}[args.workload]
     if args.workload != "tank":
         fn = {"batch": workload_batch, "adjoint": workload_adjoint, "les": workload_les,
-              "channel": workload_channel}[args.workload]
+              "channel": workload_channel, "surface": workload_surface}[args.workload]
         print(json.dumps(fn(args)), flush=True)
         return
 

@@ -332,6 +332,17 @@ int lbm_geom_radial_impeller(const int dims[3], double dx, double vessel_diamete
                              double diameter, double clearance, int blades, double blade_angle_deg,
                              double omega_z, double thickness_fraction, double* pos, double* weight,
                              double* velocity, double* center, int64_t capacity, int64_t* count);
+/* No counterpart in the reference: a seeded surface sampler for the same
+ * blades (the marker set of the stirred-tank benchmark configuration).  The
+ * blade boxes of build_radial_impeller (geometry.rs:815-906) are sampled at
+ * uniform areal density: per blade N_b = max(1, round(A_b / (area_per_marker
+ * dx^2))) markers, each on a face chosen with probability proportional to its
+ * area at a uniform point of it, weight A_b / N_b (m^2), three SplitMix64
+ * draws per marker from `seed`.  Output layout as lbm_geom_radial_impeller. */
+int lbm_geom_impeller_surface(const int dims[3], double dx, double vessel_diameter, double liquid_height,
+                              double diameter, double clearance, int blades, double blade_angle_deg, double omega_z,
+                              double thickness_fraction, double area_per_marker, uint64_t seed, double* pos,
+                              double* weight, double* velocity, double* center, int64_t capacity, int64_t* count);
 
 /* ---- rotating IBM (Solver::apply_rotating_ibm, solver.rs:3016-3282) ----- */
 /* Register a rigid marker body (RotatingBody::from_markers,

@@ -12,6 +12,7 @@ from .solver import (  # noqa: F401
     DirectForcingConfig,
     Solver,
     default_context,
+    impeller_surface_markers,
     mixing_time_from_cv,
     omegas,
     power_qois,
@@ -23,4 +24,4 @@ from ._lib import LbmError, lib  # noqa: F401
 
 __all__ = ["Solver", "Batch", "Context", "DirectForcingConfig", "omegas", "radial_impeller", "power_qois",
            "mixing_time_from_cv", "MAGIC_STD", "LbmError", "lib", "default_context", "revolve_schedule",
-           "WaleLes", "WALE_CW"]
+           "WaleLes", "WALE_CW", "impeller_surface_markers"]

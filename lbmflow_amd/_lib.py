@@ -119,6 +119,8 @@ SIGNATURES = {
     "lbm_geom_stirred_tank": (_I, [_P, _D, _D, _D, _I, _D, _D]),
     "lbm_geom_radial_impeller": (_I, [_pI, _D, _D, _D, _D, _D, _I, _D, _D, _D, _pD, _pD, _pD, _pD, C.c_int64,
                                       _pI64]),
+    "lbm_geom_impeller_surface": (_I, [_pI, _D, _D, _D, _D, _D, _I, _D, _D, _D, _D, C.c_uint64, _pD, _pD, _pD, _pD,
+                                       C.c_int64, _pI64]),
     "lbm_ibm_set_body": (_I, [_P, C.c_int64, _pD, _pD, _pD, _pD, C.POINTER(IbmConfig)]),
     "lbm_ibm_apply": (_I, [_P, C.POINTER(IbmDiag)]),
     "lbm_run_ibm_span": (_I, [_P, _I, _I, _pD, C.POINTER(IbmDiag)]),

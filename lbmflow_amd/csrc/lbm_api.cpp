@@ -1642,6 +1642,104 @@ int lbm_geom_radial_impeller(const int dims[3], double dx, double vessel_diamete
   return k == 0 ? LBM_ERR_INVALID : LBM_OK;
 }
 
+int lbm_geom_impeller_surface(const int dims[3], double dx, double vessel_diameter, double liquid_height,
+                              double diameter, double clearance, int blades, double blade_angle_deg, double omega_z,
+                              double thickness_fraction, double area_per_marker, uint64_t seed, double* pos,
+                              double* weight, double* velocity, double* center, int64_t capacity, int64_t* count) {
+  if (!dims || !count) return LBM_ERR_INVALID;
+  if (!(blade_angle_deg >= 0.0 && blade_angle_deg <= 90.0) || blades <= 0 || !(diameter > 0.0) || !(dx > 0.0) ||
+      !(area_per_marker > 0.0) || !std::isfinite(area_per_marker))
+    return LBM_ERR_INVALID;
+  // The blade solids of build_radial_impeller (geometry.rs:815-906): per
+  // blade a box in the blade frame, radial [r_in, R], tangential [-t/2, t/2],
+  // axial [-z_span/2, z_span/2], sheared axially by (r - r_in) sin(pitch) 0.18.
+  // Markers are drawn at uniform areal density over the six faces of each
+  // box (in the unsheared frame): a face with probability proportional to its
+  // area, then a uniform point on it; N_b = max(1, round(A_b / area)) per
+  // blade, each marker weighted A_b / N_b (m^2).  Three SplitMix64 draws per
+  // marker, one stream from `seed`, blades in order.  Markers outside the
+  // liquid are dropped as in the reference generator.
+  const double TAU = 6.283185307179586;
+  const double cxy[2] = {0.5 * (double)dims[0] * dx, 0.5 * (double)dims[1] * dx};
+  const double radius = 0.5 * diameter;
+  const double ctr[3] = {cxy[0], cxy[1], clearance};
+  const double radial_inner = blade_angle_deg == 0.0 ? 0.65 * radius : 0.35 * radius;
+  const double thickness = std::max(thickness_fraction * diameter, dx);
+  const double pitch = blade_angle_deg * (3.141592653589793 / 180.0);
+  const double z_span =
+      blade_angle_deg == 0.0 ? thickness : thickness + (radius - radial_inner) * std::sin(pitch) * 0.18;
+  const double L = radius - radial_inner;
+  // faces: r_in, r_out (t x z), t-, t+ (L x z), z-, z+ (L x t)
+  const double fa[6] = {thickness * z_span, thickness * z_span, L * z_span, L * z_span, L * thickness,
+                        L * thickness};
+  double cum[6];
+  double ab = 0.0;
+  for (int f = 0; f < 6; ++f) {
+    ab += fa[f];
+    cum[f] = ab;
+  }
+  const double area_lu = area_per_marker * dx * dx;
+  const int64_t nb = std::max<int64_t>(1, (int64_t)std::llround(ab / area_lu));
+  const double w = ab / (double)nb;
+  uint64_t state = seed;
+  auto uniform = [&state]() {  // SplitMix64, top 53 bits
+    state += 0x9E3779B97F4A7C15ull;
+    uint64_t z = state;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return (double)(z >> 11) / 9007199254740992.0;
+  };
+  int64_t k = 0;
+  for (int b = 0; b < blades; ++b) {
+    const double th = TAU * (double)b / (double)blades;
+    const double er[2] = {std::cos(th), std::sin(th)};
+    const double et[2] = {-std::sin(th), std::cos(th)};
+    for (int64_t j = 0; j < nb; ++j) {
+      const double u1 = uniform() * ab, u2 = uniform(), u3 = uniform();
+      int f = 0;
+      while (f < 5 && u1 >= cum[f]) ++f;
+      double rr, ot, oz;
+      if (f < 2) {
+        rr = f == 0 ? radial_inner : radius;
+        ot = -0.5 * thickness + thickness * u2;
+        oz = -0.5 * z_span + z_span * u3;
+      } else if (f < 4) {
+        ot = f == 2 ? -0.5 * thickness : 0.5 * thickness;
+        rr = radial_inner + L * u2;
+        oz = -0.5 * z_span + z_span * u3;
+      } else {
+        oz = f == 4 ? -0.5 * z_span : 0.5 * z_span;
+        rr = radial_inner + L * u2;
+        ot = -0.5 * thickness + thickness * u3;
+      }
+      const double pz = oz + (rr - radial_inner) * std::sin(pitch) * 0.18;
+      const double p[3] = {ctr[0] + rr * er[0] + ot * et[0], ctr[1] + rr * er[1] + ot * et[1], ctr[2] + pz};
+      const double rel = std::sqrt((p[0] - cxy[0]) * (p[0] - cxy[0]) + (p[1] - cxy[1]) * (p[1] - cxy[1]));
+      if (rel >= 0.5 * vessel_diameter || p[2] <= 0.0 || p[2] >= liquid_height) continue;
+      if (pos && k < capacity) {
+        const double r3[3] = {p[0] - ctr[0], p[1] - ctr[1], p[2] - ctr[2]};
+        const double om[3] = {0.0, 0.0, omega_z};
+        velocity[3 * k + 0] = om[1] * r3[2] - om[2] * r3[1];
+        velocity[3 * k + 1] = om[2] * r3[0] - om[0] * r3[2];
+        velocity[3 * k + 2] = om[0] * r3[1] - om[1] * r3[0];
+        pos[3 * k + 0] = p[0] / dx;
+        pos[3 * k + 1] = p[1] / dx;
+        pos[3 * k + 2] = p[2] / dx;
+        weight[k] = w;
+      }
+      ++k;
+    }
+  }
+  if (center) {
+    center[0] = ctr[0] / dx;
+    center[1] = ctr[1] / dx;
+    center[2] = ctr[2] / dx;
+  }
+  *count = k;
+  return k == 0 ? LBM_ERR_INVALID : LBM_OK;
+}
+
 // ---- rotating IBM -----------------------------------------------------------
 int lbm_ibm_set_body(lbm_fields* fl, int64_t n, const double* pos, const double* weight, const double center[3],
                      const double omega[3], const lbm_ibm_config* cfg) {

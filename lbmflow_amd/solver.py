@@ -542,6 +542,32 @@ def radial_impeller(dims, dx, vessel_diameter, liquid_height, diameter, clearanc
     return pos, w, vel, c
 
 
+def impeller_surface_markers(dims, dx, vessel_diameter, liquid_height, diameter, clearance, blades=6,
+                             blade_angle_deg=0.0, omega_z=0.0, thickness_fraction=0.04, area_per_marker=0.1,
+                             seed=20260512):
+    """Seeded uniform surface sampling of the same blades (no reference
+    counterpart): markers on the blade boxes of build_radial_impeller at one
+    marker per `area_per_marker` lattice units^2, each weighted by its area
+    (m^2).  Returns (pos, weight, velocity, center) in lattice units."""
+    lib = _L.lib()
+    d = np.array(dims, np.int32)
+    cnt = C.c_int64()
+    args = (_ip(d), dx, vessel_diameter, liquid_height, diameter, clearance, blades, blade_angle_deg, omega_z,
+            thickness_fraction, area_per_marker, int(seed) & 0xFFFFFFFFFFFFFFFF)
+    rc = lib.lbm_geom_impeller_surface(*args, None, None, None, None, 0, C.byref(cnt))
+    if rc != 0:
+        raise _L.LbmError(rc, "impeller surface sampling produced no markers")
+    n = cnt.value
+    pos = np.zeros((n, 3))
+    w = np.zeros(n)
+    vel = np.zeros((n, 3))
+    c = np.zeros(3)
+    rc = lib.lbm_geom_impeller_surface(*args, _dp(pos), _dp(w), _dp(vel), _dp(c), n, C.byref(cnt))
+    if rc != 0:
+        raise _L.LbmError(rc, "impeller surface sampling failed")
+    return pos, w, vel, c
+
+
 def power_qois(torque, omega, rho, diameter, volume):
     """``qoi::power_qois`` (qoi.rs:477-505)."""
     out = np.zeros(4)

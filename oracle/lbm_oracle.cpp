@@ -1790,6 +1790,85 @@ static int radial_impeller(const int dims[3], double dx, double vessel_diameter,
   return (int)weight.size();
 }
 
+// Seeded uniform surface sampling of the blade boxes of
+// build_radial_impeller (the benchmark's marker set; it has no reference
+// counterpart, so this restates its specification: per blade
+// N_b = max(1, round(A_b / (area dx^2))) markers, each drawing three
+// SplitMix64 uniforms u1, u2, u3 from one stream: u1 A_b picks the face by
+// cumulative area in the order r_in, r_out, t-, t+, z-, z+, and (u2, u3)
+// place the point on it; weight A_b / N_b; markers outside the liquid are
+// dropped; the axial shear (r - r_in) sin(pitch) 0.18 is applied after).
+struct SplitMix64 {
+  uint64_t s;
+  double next() {
+    s += 0x9E3779B97F4A7C15ull;
+    uint64_t z = s;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return (double)(z >> 11) / 9007199254740992.0;  // 2^53
+  }
+};
+
+static int impeller_surface(const int dims[3], double dx, double vessel_diameter, double liquid_height,
+                            double diameter, double clearance, int blades, double blade_angle_deg, double omega_z,
+                            double thickness_fraction, double area_per_marker, uint64_t seed,
+                            std::vector<double>& pos, std::vector<double>& weight, std::vector<double>& vel,
+                            double center_out[3]) {
+  const double R = 0.5 * diameter;
+  const double rin = blade_angle_deg == 0.0 ? 0.65 * R : 0.35 * R;
+  const double t = std::max(thickness_fraction * diameter, dx);
+  const double ang = blade_angle_deg * (3.141592653589793 / 180.0);
+  const double h = blade_angle_deg == 0.0 ? t : t + (R - rin) * std::sin(ang) * 0.18;
+  const double len = R - rin;
+  const double face_area[6] = {t * h, t * h, len * h, len * h, len * t, len * t};
+  double bound[6], total = 0.0;
+  for (int f = 0; f < 6; ++f) bound[f] = (total += face_area[f]);
+  const long n_per_blade = std::max(1L, (long)std::llround(total / (area_per_marker * dx * dx)));
+  const double w_marker = total / (double)n_per_blade;
+  const double cx = 0.5 * (double)dims[0] * dx, cy = 0.5 * (double)dims[1] * dx;
+  SplitMix64 rng{seed};
+  pos.clear();
+  weight.clear();
+  vel.clear();
+  for (int b = 0; b < blades; ++b) {
+    const double phi = 6.283185307179586 * (double)b / (double)blades;
+    const double c = std::cos(phi), sn = std::sin(phi);
+    for (long j = 0; j < n_per_blade; ++j) {
+      const double a = rng.next() * total, u = rng.next(), v = rng.next();
+      int f = 0;
+      while (f < 5 && !(a < bound[f])) ++f;
+      // (radial, tangential, axial) in the blade frame
+      double r, tt, zz;
+      switch (f) {
+        case 0: r = rin; tt = -0.5 * t + t * u; zz = -0.5 * h + h * v; break;
+        case 1: r = R; tt = -0.5 * t + t * u; zz = -0.5 * h + h * v; break;
+        case 2: tt = -0.5 * t; r = rin + len * u; zz = -0.5 * h + h * v; break;
+        case 3: tt = 0.5 * t; r = rin + len * u; zz = -0.5 * h + h * v; break;
+        case 4: zz = -0.5 * h; r = rin + len * u; tt = -0.5 * t + t * v; break;
+        default: zz = 0.5 * h; r = rin + len * u; tt = -0.5 * t + t * v; break;
+      }
+      const double x = cx + r * c + tt * -sn;
+      const double y = cy + r * sn + tt * c;
+      const double z = clearance + (zz + (r - rin) * std::sin(ang) * 0.18);
+      const double rho = std::sqrt((x - cx) * (x - cx) + (y - cy) * (y - cy));
+      if (rho >= 0.5 * vessel_diameter || z <= 0.0 || z >= liquid_height) continue;
+      const double rx = x - cx, ry = y - cy, rz = z - clearance;
+      vel.push_back(0.0 * rz - omega_z * ry);
+      vel.push_back(omega_z * rx - 0.0 * rz);
+      vel.push_back(0.0 * ry - 0.0 * rx);
+      pos.push_back(x / dx);
+      pos.push_back(y / dx);
+      pos.push_back(z / dx);
+      weight.push_back(w_marker);
+    }
+  }
+  center_out[0] = cx / dx;
+  center_out[1] = cy / dx;
+  center_out[2] = clearance / dx;
+  return (int)weight.size();
+}
+
 }  // namespace orc
 
 using namespace orc;
@@ -2244,6 +2323,24 @@ void orc_tank_mask(const int dims[3], double dx, double diameter, double height,
 void orc_baffle_mask(const int dims[3], double dx, double diameter, double height, double width, double thickness,
                      uint8_t* solid, uint8_t* baffle) {
   baffle_mask(dims, dx, diameter, height, width, thickness, solid, baffle);
+}
+
+// Returns marker count; if pos == NULL only counts.
+int orc_impeller_surface(const int dims[3], double dx, double vessel_diameter, double liquid_height, double diameter,
+                         double clearance, int blades, double blade_angle_deg, double omega_z,
+                         double thickness_fraction, double area_per_marker, uint64_t seed, double* pos,
+                         double* weight, double* vel, double* center) {
+  std::vector<double> P, W, V;
+  double c[3];
+  int n = impeller_surface(dims, dx, vessel_diameter, liquid_height, diameter, clearance, blades, blade_angle_deg,
+                           omega_z, thickness_fraction, area_per_marker, seed, P, W, V, c);
+  if (pos) {
+    std::memcpy(pos, P.data(), P.size() * 8);
+    std::memcpy(weight, W.data(), W.size() * 8);
+    std::memcpy(vel, V.data(), V.size() * 8);
+    std::memcpy(center, c, 24);
+  }
+  return n;
 }
 
 // Returns marker count; if pos == NULL only counts.

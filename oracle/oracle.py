@@ -86,6 +86,8 @@ def lib():
         L.orc_radial_impeller.argtypes = [i, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int,
                                           C.c_double, C.c_double, C.c_double, C.c_int, d, d, d, d]
         L.orc_stamp_wall_velocity.argtypes = [i, C.c_int, d, d, d]
+        L.orc_impeller_surface.argtypes = [i, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int,
+                                           C.c_double, C.c_double, C.c_double, C.c_double, C.c_uint64, d, d, d, d]
         L.orc_marker_stencil.argtypes = [d, i, C.c_int, C.POINTER(C.c_long), d]
         L.orc_power_qois.argtypes = [C.c_double] * 5 + [d]
         L.orc_scalar_cv.argtypes = [C.c_long, d, u8, d]
@@ -389,6 +391,22 @@ def radial_impeller(dims, dx, vessel_diameter, liquid_height, diameter, clearanc
     lib().orc_radial_impeller(_i(dims), dx, vessel_diameter, liquid_height, diameter, clearance, blades,
                               blade_angle_deg, omega_z, thickness_fraction, inner_kind, _d(pos), _d(w), _d(vel),
                               _d(c))
+    return pos, w, vel, c
+
+
+def impeller_surface(dims, dx, vessel_diameter, liquid_height, diameter, clearance, blades=6, blade_angle_deg=0.0,
+                     omega_z=0.0, thickness_fraction=0.04, area_per_marker=0.1, seed=20260512):
+    """Seeded uniform surface sampling of the blade boxes (restated from its
+    specification; the reference has no surface sampler)."""
+    dims = np.array(dims, np.int32)
+    args = (_i(dims), dx, vessel_diameter, liquid_height, diameter, clearance, blades, blade_angle_deg, omega_z,
+            thickness_fraction, area_per_marker, int(seed) & 0xFFFFFFFFFFFFFFFF)
+    n = lib().orc_impeller_surface(*args, None, None, None, None)
+    pos = np.zeros((n, 3))
+    w = np.zeros(n)
+    vel = np.zeros((n, 3))
+    c = np.zeros(3)
+    lib().orc_impeller_surface(*args, _d(pos), _d(w), _d(vel), _d(c))
     return pos, w, vel, c
 
 

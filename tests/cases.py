@@ -82,15 +82,20 @@ def tank_geometry(n, *, T=None, H=None, baffle_width=None, baffle_thickness=None
     return solid, (T, H, bw, bt)
 
 
-def tank_case(n, *, nu=0.01, tip=0.05, collision="cm", Q=27, sigma=None):
+def tank_case(n, *, nu=0.01, tip=0.05, collision="cm", Q=27, sigma=None, markers="lattice", seed=20260512):
     """Small stirred tank (same ratios as the benchmark): geometry, markers
-    and a Gaussian tracer near the surface."""
+    and a Gaussian tracer near the surface.  markers: "lattice" (the
+    reference's volumetric blade lattice) or "surface" (the seeded surface
+    sampler, one marker per 0.1 lu^2)."""
     T = 15.0 * n / 16.0
     H = T
     solid, _ = tank_geometry(n, T=T, H=H, baffle_width=T / 10.0, baffle_thickness=T / 48.0)
     D = T / 3.0
     omega = tip / (0.5 * D)
-    pos, w, vel, c = O.radial_impeller((n, n, n), 1.0, T, H, D, T / 3.0, 6, 0.0, omega)
+    if markers == "surface":
+        pos, w, vel, c = O.impeller_surface((n, n, n), 1.0, T, H, D, T / 3.0, 6, 0.0, omega, 0.04, 0.1, seed)
+    else:
+        pos, w, vel, c = O.radial_impeller((n, n, n), 1.0, T, H, D, T / 3.0, 6, 0.0, omega)
     x, y, z = coords((n, n, n))
     sig = (8.0 * n / 512.0 * 8.0) if sigma is None else sigma
     cz = 0.8 * H

@@ -10,9 +10,12 @@ from tests import cases
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("collision,Q", [("cm", 27), ("trt", 19)])
-def test_tank_torque_np_and_cv_match_oracle(collision, Q):
-    tc = cases.tank_case(40, nu=0.02, collision=collision, Q=Q, sigma=3.0)
+@pytest.mark.parametrize("collision,Q,markers", [("cm", 27, "lattice"), ("trt", 19, "lattice"),
+                                                ("cm", 27, "surface"), ("trt", 19, "surface")])
+def test_tank_torque_np_and_cv_match_oracle(collision, Q, markers):
+    """markers: the reference's blade lattice, or the seeded surface sampler
+    of the 256^3 benchmark configuration (one marker per 0.1 lu^2)."""
+    tc = cases.tank_case(40, nu=0.02, collision=collision, Q=Q, sigma=3.0, markers=markers)
     steps = 240
     t_o, cv_o, so = cases.run_tank_oracle(tc, steps, prec=4, scalar_D=0.02)
     t_g, cv_g, sg = cases.run_tank_gpu(tc, steps, scalar_D=0.02)
