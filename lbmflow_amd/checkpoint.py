@@ -1,7 +1,8 @@
 """Checkpoint directories in the reference's format v3 (Solver::save /
 Solver::restore, crates/lbm-core/src/solver.rs:4536-4830): manifest.json plus
-one rank file per part.  The rank file (populations, moments, solid flags,
-force field, FNV-1a 64 payload hash) is written and read by the library
+one rank file per part.  The rank file (populations, the previous step's
+post-collision populations, moments, solid flags, force field, scalar
+distributions, FNV-1a 64 payload hash) is written and read by the library
 (lbm_checkpoint_write_part / lbm_checkpoint_read_part); this module builds
 the manifest and the scenario and decomposition hashes (solver.rs:883-965)
 and checks them on restore, before any state is installed."""
@@ -11,6 +12,8 @@ import ctypes as C
 import json
 import os
 import struct
+
+import numpy as np
 
 from . import _lib as _L
 
@@ -47,12 +50,29 @@ def _f32(v: float) -> bytes:
     return struct.pack("<f", v)
 
 
+_FACE_KIND_IDS = {"closed": 0, "velocity": 1, "pressure": 2, "outflow": 3, "convective": 4}
+
+
+def _face_bc_bytes(kind: str, value) -> bytes:
+    """hash_face_bc (solver.rs:860-880): kind id, then its f32 parameters."""
+    data = _u64(_FACE_KIND_IDS[kind])
+    v = [0.0, 0.0, 0.0] if value is None else list(np.atleast_1d(np.asarray(value, np.float64)))
+    if kind == "velocity":
+        v = (v + [0.0, 0.0, 0.0])[:3]
+        for x in v:
+            data += _f32(float(x))
+    elif kind in ("pressure", "convective"):
+        data += _f32(float(v[0]))
+    return data
+
+
 def spec_hash(Q: int, dims, nu: float, collision: str, magic: float, omega_shear, periodic, force,
-              sources=()) -> int:
-    """spec_hash (solver.rs:883-945) for a closed-face scenario with no face
-    patches, fp32 run (reals hashed as f32).  `sources` are the volume
-    sources in order, each (lo, hi, q_lu) for a mass flow or
-    (lo, hi, q_lu, u) for a jet (SourceKind ids 1 and 2)."""
+              sources=(), faces=None, patches=()) -> int:
+    """spec_hash (solver.rs:883-945), fp32 run (reals hashed as f32).
+    `sources` are the volume sources in order, each (lo, hi, q_lu) for a
+    mass flow or (lo, hi, q_lu, u) for a jet (SourceKind ids 1 and 2);
+    `faces` the six (kind, value) face rules in Face::ALL order (None: all
+    closed); `patches` (face, lo, hi, kind, value) in order."""
     data = _u64(3) + _u64(Q)
     for v in dims:
         data += _u64(int(v))
@@ -65,8 +85,8 @@ def spec_hash(Q: int, dims, nu: float, collision: str, magic: float, omega_shear
         data += _u64(3) + _f64(float(omega_shear))
     for p in periodic:
         data += _u64(int(bool(p)))
-    for _ in range(6):  # FaceBC::Closed on every face
-        data += _u64(0)
+    for kind, value in (faces or [("closed", None)] * 6):
+        data += _face_bc_bytes(kind, value)
     for v in force:
         data += _f32(float(v))
     for src in sources:
@@ -78,6 +98,11 @@ def spec_hash(Q: int, dims, nu: float, collision: str, magic: float, omega_shear
                 data += _f32(float(v))
         else:
             data += _u64(1) + _f32(float(src[2]))
+    for face, lo, hi, kind, value in patches:
+        data += _u64(int(face))
+        for v in list(lo) + list(hi):
+            data += _u64(int(v))
+        data += _face_bc_bytes(kind, value)
     return fnv1a64(data)
 
 
@@ -107,7 +132,7 @@ def _lattice(Q: int) -> str:
 
 def _solver_spec_hash(s) -> int:
     return spec_hash(s.Q, s.dims, s.nu, s.collision, s.magic, s.params.omega_shear, s.periodic, s.force,
-                     getattr(s, "sources", ()))
+                     getattr(s, "sources", ()), getattr(s, "faces", None), getattr(s, "patches", ()))
 
 
 def save(solver, directory: str) -> dict:

@@ -2475,7 +2475,40 @@ int lbm_adjoint_torque(lbm_fields* fl, int steps, int window, int snapshots, dou
 namespace {
 
 constexpr uint32_t kCkptVersion = 3;
-constexpr uint32_t kSecF = 1, kSecStash = 2, kSecMoments = 3, kSecSolid = 4, kSecForce = 5;
+constexpr uint32_t kSecF = 1, kSecStash = 2, kSecMoments = 3, kSecSolid = 4, kSecForce = 5, kSecScalars = 6;
+
+// Open-face stash geometry on the host (the device's face_cell / unk_index
+// of lbm_open.hip): face cells in (t2 outer, t1 inner) order, unknowns are
+// the directions entering the domain through the face, ascending q.
+struct FaceMap {
+  int a, nin, t1, t2;
+  uint32_t ext[3], nf;
+  FaceMap(const Dom& d, int face)
+      : a(face / 2), nin(face % 2 == 0 ? 1 : -1), t1(face / 2 == 0 ? 1 : 0), t2(face / 2 == 2 ? 1 : 2) {
+    ext[0] = d.nx;
+    ext[1] = d.ny;
+    ext[2] = d.nz;
+    nf = ext[t1] * ext[t2];
+  }
+  void cell(uint32_t coord, int64_t xyz[3]) const {
+    xyz[a] = nin > 0 ? 0 : (int64_t)ext[a] - 1;
+    xyz[t1] = coord % ext[t1];
+    xyz[t2] = coord / ext[t1];
+  }
+  std::vector<int> unknowns(int Q) const {
+    std::vector<int> u;
+    for (int q = 0; q < Q; ++q)
+      if (lat_c(q, a) * nin > 0) u.push_back(q);
+    return u;
+  }
+};
+
+void put_u32(std::vector<uint8_t>& b, uint32_t v) { b.insert(b.end(), (uint8_t*)&v, (uint8_t*)&v + 4); }
+void put_u64(std::vector<uint8_t>& b, uint64_t v) { b.insert(b.end(), (uint8_t*)&v, (uint8_t*)&v + 8); }
+void put_f64(std::vector<uint8_t>& b, double v) { b.insert(b.end(), (uint8_t*)&v, (uint8_t*)&v + 8); }
+void put_f32s(std::vector<uint8_t>& b, const float* v, size_t n) {
+  b.insert(b.end(), (const uint8_t*)v, (const uint8_t*)(v + n));
+}
 
 uint64_t fnv1a(const void* data, size_t n, uint64_t h = 0xcbf29ce484222325ull) {
   const uint8_t* p = (const uint8_t*)data;
@@ -2602,10 +2635,6 @@ uint64_t lbm_fnv1a64(const void* data, size_t n) { return fnv1a(data, n); }
 
 int lbm_checkpoint_write_part(lbm_fields* fl, const char* path, lbm_ckpt_info* info) {
   if (!fl || !path || !info) return LBM_ERR_INVALID;
-  if (!fl->scalars.empty())
-    return fail(fl->ctx, LBM_ERR_UNSUPPORTED, "CKPT_UNSUPPORTED: scalar distributions are not checkpointed");
-  if (open_bits(fl))
-    return fail(fl->ctx, LBM_ERR_UNSUPPORTED, "CKPT_UNSUPPORTED: open-face stale slots are not checkpointed");
   const PadGeom g(fl->d);
   const int per[3] = {(int)fl->d.per[0], (int)fl->d.per[1], (int)fl->d.per[2]};
   std::vector<float> f, wall, mom, force;
@@ -2614,9 +2643,69 @@ int lbm_checkpoint_write_part(lbm_fields* fl, const char* path, lbm_ckpt_info* i
   int rc = ckpt_host_state(fl, f, solid, wall, mom, force, &present);
   if (rc) return rc;
   const size_t Q = (size_t)fl->Q;
+  // STALE_STASH: the previous step's post-collision populations (the
+  // reference's second buffer); at open faces the entering unknowns are the
+  // stash the next step restores
+  std::vector<float> post(Q * g.nc);
+  {
+    float* dpost = nullptr;
+    rc = dev_alloc(fl, (void**)&dpost, Q * g.nc * 4);
+    if (rc) return rc;
+    hipError_t e = launch_poststate(fl->Q, fl->storage, fl->parity, fl->d, fl->f, fl->mask, fl->wall_u, dpost,
+                                    fl->ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(post.data(), dpost, Q * g.nc * 4, hipMemcpyDeviceToHost, fl->ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(fl->ctx->stream);
+    dev_free(fl, dpost, Q * g.nc * 4);
+    if (e != hipSuccess) return fail(fl->ctx, LBM_ERR_DEVICE, "post-collision state: %s", hipGetErrorString(e));
+  }
+  for (int face = 0; face < 6; ++face) {
+    if (!fl->stash[face][0]) continue;
+    const FaceMap fm(fl->d, face);
+    const std::vector<int> unk = fm.unknowns(fl->Q);
+    std::vector<float> st(unk.size() * fm.nf);
+    HIPCHK(fl->ctx, hipMemcpyAsync(st.data(), fl->stash[face][fl->stash_sel], st.size() * 4, hipMemcpyDeviceToHost,
+                                   fl->ctx->stream));
+    HIPCHK(fl->ctx, hipStreamSynchronize(fl->ctx->stream));
+    for (uint32_t coord = 0; coord < fm.nf; ++coord) {
+      int64_t c[3];
+      fm.cell(coord, c);
+      const size_t i = g.cidx(c[0], c[1], c[2]);
+      if (solid[i]) continue;
+      for (size_t k = 0; k < unk.size(); ++k) post[(size_t)unk[k] * g.nc + i] = st[k * fm.nf + coord];
+    }
+  }
+  // SCALAR_FIELDS (scalar_checkpoint_bytes, solver.rs:1202-1215): count, then
+  // per scalar name, diffusivity, h (padded planes), its ping-pong partner,
+  // and the compact concentration
+  std::vector<uint8_t> scal;
+  if (!fl->scalars.empty()) {
+    put_u32(scal, (uint32_t)fl->scalars.size());
+    for (size_t k = 0; k < fl->scalars.size(); ++k) {
+      int Qs = 0;
+      double D = 0.0;
+      scalar_info(fl->scalars[k], &Qs, &D);
+      std::vector<float> h((size_t)Qs * g.nc), h2((size_t)Qs * g.nc), C(g.nc), plane(g.np);
+      rc = scalar_get_state(view_of(fl), fl->scalars[k], h.data(), h2.data(), C.data());
+      if (rc) return rc;
+      const std::string name = "scalar" + std::to_string(k);
+      put_u32(scal, (uint32_t)name.size());
+      scal.insert(scal.end(), name.begin(), name.end());
+      put_f64(scal, D);
+      for (const std::vector<float>* buf : {&h, &h2}) {
+        put_u64(scal, (uint64_t)Qs * g.np);
+        for (int q = 0; q < Qs; ++q) {
+          pad_plane(g, per, buf->data() + (size_t)q * g.nc, plane.data());
+          put_f32s(scal, plane.data(), g.np);
+        }
+      }
+      put_u64(scal, g.nc);
+      put_f32s(scal, C.data(), g.nc);
+    }
+  }
   std::vector<std::pair<uint32_t, uint64_t>> sec = {{kSecF, Q * g.np * 4}, {kSecStash, Q * g.np * 4},
                                                     {kSecMoments, 4 * g.nc * 4}, {kSecSolid, g.np}};
   if (present) sec.push_back({kSecForce, 3 * g.nc * 4});
+  if (!scal.empty()) sec.push_back({kSecScalars, scal.size()});
   CkptWriter w;
   w.fp = fopen(path, "wb");
   if (!w.fp) return fail(fl->ctx, LBM_ERR_INVALID, "CKPT_IO: cannot create %s", path);
@@ -2642,9 +2731,9 @@ int lbm_checkpoint_write_part(lbm_fields* fl, const char* path, lbm_ckpt_info* i
     off += e.second;
   }
   std::vector<float> plane(g.np);
-  for (int pass = 0; pass < 2; ++pass)  // populations, then the stash copy
+  for (int pass = 0; pass < 2; ++pass)  // populations, then the post-collision stash
     for (size_t q = 0; q < Q; ++q) {
-      pad_plane(g, per, f.data() + q * g.nc, plane.data());
+      pad_plane(g, per, (pass == 0 ? f.data() : post.data()) + q * g.nc, plane.data());
       w.payload(plane.data(), g.np * 4);
     }
   w.payload(mom.data(), 4 * g.nc * 4);
@@ -2652,6 +2741,7 @@ int lbm_checkpoint_write_part(lbm_fields* fl, const char* path, lbm_ckpt_info* i
   pad_plane(g, per, solid.data(), ps.data());
   w.payload(ps.data(), g.np);
   if (present) w.payload(force.data(), 3 * g.nc * 4);
+  if (!scal.empty()) w.payload(scal.data(), scal.size());
   const uint64_t ph = w.h;
   w.raw(&ph, 8);
   const long bytes = ftell(w.fp);
@@ -2666,10 +2756,6 @@ int lbm_checkpoint_write_part(lbm_fields* fl, const char* path, lbm_ckpt_info* i
 
 int lbm_checkpoint_read_part(lbm_fields* fl, const char* path, int64_t step, lbm_ckpt_info* info) {
   if (!fl || !path || !info || step < 0) return LBM_ERR_INVALID;
-  if (!fl->scalars.empty())
-    return fail(fl->ctx, LBM_ERR_UNSUPPORTED, "CKPT_UNSUPPORTED: scalar distributions are not checkpointed");
-  if (open_bits(fl))
-    return fail(fl->ctx, LBM_ERR_UNSUPPORTED, "CKPT_UNSUPPORTED: open-face stale slots are not checkpointed");
   const PadGeom g(fl->d);
   const int per[3] = {(int)fl->d.per[0], (int)fl->d.per[1], (int)fl->d.per[2]};
   // the manifest's mask hash against the current masks, before the rank file
@@ -2760,14 +2846,92 @@ int lbm_checkpoint_read_part(lbm_fields* fl, const char* path, int64_t step, lbm
     return LBM_OK;
   };
   const size_t Q = (size_t)fl->Q;
-  const uint8_t *sf = nullptr, *sm = nullptr, *ss = nullptr, *sff = nullptr;
+  const uint8_t *sf = nullptr, *sm = nullptr, *ss = nullptr, *sff = nullptr, *sst = nullptr, *ssc = nullptr;
+  uint64_t sclen = 0;
   rc = section(kSecF, Q * g.np * 4, &sf);
   if (!rc) rc = section(kSecMoments, 4 * g.nc * 4, &sm);
   if (!rc) rc = section(kSecSolid, g.np, &ss);
   if (!rc) rc = section(kSecForce, 3 * g.nc * 4, &sff);
+  if (!rc) rc = section(kSecStash, Q * g.np * 4, &sst);
   if (rc) return rc;
+  for (auto& e : tab)
+    if (e.id == kSecScalars) {
+      if (e.off > plen || e.len > plen - e.off)
+        return fail(fl->ctx, LBM_ERR_INVALID, "CKPT_TRUNCATED: section %u exceeds the payload", e.id);
+      ssc = payload + e.off;
+      sclen = e.len;
+    }
   if (!sf) return fail(fl->ctx, LBM_ERR_INVALID, "CKPT_TRUNCATED: checkpoint is missing required section F_PRIMARY");
   if (!sm) return fail(fl->ctx, LBM_ERR_INVALID, "CKPT_TRUNCATED: checkpoint is missing required section MOMENTS");
+  bool any_stash = false;
+  for (int face = 0; face < 6; ++face) any_stash = any_stash || fl->stash[face][0] != nullptr;
+  if (any_stash && !sst)
+    return fail(fl->ctx, LBM_ERR_INVALID, "CKPT_TRUNCATED: open faces need the STALE_STASH section");
+  // scalars (restore_scalar_checkpoint, solver.rs:1217-1270): parsed and
+  // checked before anything is installed; this run must hold the same number
+  // of scalars on the same lattices with the same diffusivities
+  struct ScalarIn {
+    std::vector<float> h, h2, C;
+  };
+  std::vector<ScalarIn> sin_(fl->scalars.size());
+  if (!fl->scalars.empty() && !ssc)
+    return fail(fl->ctx, LBM_ERR_INVALID, "CKPT_TRUNCATED: checkpoint holds no scalar fields for this run's scalars");
+  if (ssc) {
+    size_t pos2 = 0;
+    auto need = [&](size_t n) { return sclen - pos2 >= n; };
+    auto rd = [&](void* dst, size_t n) {
+      memcpy(dst, ssc + pos2, n);
+      pos2 += n;
+    };
+    uint32_t count = 0;
+    if (!need(4)) return fail(fl->ctx, LBM_ERR_INVALID, "CKPT_TRUNCATED: scalar section");
+    rd(&count, 4);
+    if (count != fl->scalars.size())
+      return fail(fl->ctx, LBM_ERR_INVALID, "CKPT_SCENARIO_MISMATCH: checkpoint holds %u scalars, this run %zu", count,
+                  fl->scalars.size());
+    for (uint32_t k = 0; k < count; ++k) {
+      int Qs = 0;
+      double D = 0.0;
+      scalar_info(fl->scalars[k], &Qs, &D);
+      uint32_t nl = 0;
+      if (!need(4)) return fail(fl->ctx, LBM_ERR_INVALID, "CKPT_TRUNCATED: scalar section");
+      rd(&nl, 4);
+      if (!need((size_t)nl + 8)) return fail(fl->ctx, LBM_ERR_INVALID, "CKPT_TRUNCATED: scalar section");
+      pos2 += nl;
+      double Df = 0.0;
+      rd(&Df, 8);
+      if (Df != D)
+        return fail(fl->ctx, LBM_ERR_INVALID, "CKPT_SCENARIO_MISMATCH: scalar %u diffusivity %.17g, this run %.17g", k,
+                    Df, D);
+      ScalarIn& si = sin_[k];
+      for (std::vector<float>* buf : {&si.h, &si.h2}) {
+        uint64_t len = 0;
+        if (!need(8)) return fail(fl->ctx, LBM_ERR_INVALID, "CKPT_TRUNCATED: scalar section");
+        rd(&len, 8);
+        if (len != (uint64_t)Qs * g.np)
+          return fail(fl->ctx, LBM_ERR_INVALID, "CKPT_GEOM_MISMATCH: scalar checkpoint section length differs");
+        if (!need(len * 4)) return fail(fl->ctx, LBM_ERR_INVALID, "CKPT_TRUNCATED: scalar section");
+        buf->resize((size_t)Qs * g.nc);
+        for (int q = 0; q < Qs; ++q)
+          for (int64_t z = 0; z < g.n[2]; ++z)
+            for (int64_t y = 0; y < g.n[1]; ++y)
+              for (int64_t x = 0; x < g.n[0]; ++x)
+                memcpy(&(*buf)[(size_t)q * g.nc + g.cidx(x, y, z)], ssc + pos2 + 4 * ((size_t)q * g.np + g.pidx(x, y, z)),
+                       4);
+        pos2 += len * 4;
+      }
+      uint64_t clen = 0;
+      if (!need(8)) return fail(fl->ctx, LBM_ERR_INVALID, "CKPT_TRUNCATED: scalar section");
+      rd(&clen, 8);
+      if (clen != g.nc)
+        return fail(fl->ctx, LBM_ERR_INVALID, "CKPT_GEOM_MISMATCH: scalar checkpoint section length differs");
+      if (!need(clen * 4)) return fail(fl->ctx, LBM_ERR_INVALID, "CKPT_TRUNCATED: scalar section");
+      si.C.resize(g.nc);
+      rd(si.C.data(), g.nc * 4);
+    }
+    if (pos2 != sclen)
+      return fail(fl->ctx, LBM_ERR_INVALID, "CKPT_PAYLOAD_CORRUPT: scalar checkpoint section has trailing bytes");
+  }
   if (ss) {
     std::vector<uint8_t> mine(g.np);
     pad_plane(g, per, solid.data(), mine.data());
@@ -2793,6 +2957,28 @@ int lbm_checkpoint_read_part(lbm_fields* fl, const char* path, int64_t step, lbm
   }
   if (!rc) rc = refresh_moments(fl);
   if (rc) return rc;
+  for (size_t k = 0; k < sin_.size(); ++k) {
+    rc = scalar_put_state(view_of(fl), fl->scalars[k], sin_[k].h.data(), sin_[k].h2.data(), sin_[k].C.data());
+    if (rc) return rc;
+  }
+  // open faces: the entering unknowns of the face cells are the stash the
+  // next step restores
+  for (int face = 0; face < 6 && sst; ++face) {
+    if (!fl->stash[face][0]) continue;
+    const FaceMap fm(fl->d, face);
+    const std::vector<int> unk = fm.unknowns(fl->Q);
+    std::vector<float> st(unk.size() * fm.nf, 0.0f);
+    for (uint32_t coord = 0; coord < fm.nf; ++coord) {
+      int64_t c[3];
+      fm.cell(coord, c);
+      for (size_t k = 0; k < unk.size(); ++k)
+        memcpy(&st[k * fm.nf + coord], sst + 4 * ((size_t)unk[k] * g.np + g.pidx(c[0], c[1], c[2])), 4);
+    }
+    HIPCHK(fl->ctx, hipMemcpyAsync(fl->stash[face][fl->stash_sel], st.data(), st.size() * 4, hipMemcpyHostToDevice,
+                                   fl->ctx->stream));
+    HIPCHK(fl->ctx, hipStreamSynchronize(fl->ctx->stream));
+  }
+  fl->gen++;
   fl->time = step;
   // the cached moments recomputed from the installed state must be the saved ones
   std::vector<float> mom(4 * g.nc);

@@ -1521,6 +1521,33 @@ int scalar_set(const FieldsView& v, ScalarState* s, const float* conc) {
   return LBM_OK;
 }
 
+// checkpoint access: the distribution h (current), its ping-pong partner h2
+// (after a step: the post-collision populations), the concentration, and
+// the lattice size and diffusivity
+int scalar_info(const ScalarState* s, int* Qs, double* D) {
+  *Qs = s->Qs;
+  *D = s->D;
+  return LBM_OK;
+}
+
+int scalar_get_state(const FieldsView& v, ScalarState* s, float* h, float* h2, float* C) {
+  const size_t N = v.d.N, bytes = (size_t)s->Qs * N * 4;
+  AUXCHK(v, hipMemcpyAsync(h, s->h, bytes, hipMemcpyDeviceToHost, v.stream));
+  AUXCHK(v, hipMemcpyAsync(h2, s->h2, bytes, hipMemcpyDeviceToHost, v.stream));
+  AUXCHK(v, hipMemcpyAsync(C, s->C, N * 4, hipMemcpyDeviceToHost, v.stream));
+  AUXCHK(v, hipStreamSynchronize(v.stream));
+  return LBM_OK;
+}
+
+int scalar_put_state(const FieldsView& v, ScalarState* s, const float* h, const float* h2, const float* C) {
+  const size_t N = v.d.N, bytes = (size_t)s->Qs * N * 4;
+  AUXCHK(v, hipMemcpyAsync(s->h, h, bytes, hipMemcpyHostToDevice, v.stream));
+  AUXCHK(v, hipMemcpyAsync(s->h2, h2, bytes, hipMemcpyHostToDevice, v.stream));
+  AUXCHK(v, hipMemcpyAsync(s->C, C, N * 4, hipMemcpyHostToDevice, v.stream));
+  AUXCHK(v, hipStreamSynchronize(v.stream));
+  return LBM_OK;
+}
+
 int scalar_gather(const FieldsView& v, ScalarState* s, float* conc) {
   AUXCHK(v, hipMemcpyAsync(conc, s->C, (size_t)v.d.N * 4, hipMemcpyDeviceToHost, v.stream));
   AUXCHK(v, hipStreamSynchronize(v.stream));

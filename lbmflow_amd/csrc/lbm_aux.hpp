@@ -83,6 +83,9 @@ int scalar_set(const FieldsView& v, ScalarState* s, const float* conc);
 int scalar_gather(const FieldsView& v, ScalarState* s, float* conc);
 int scalar_step(const FieldsView& v, ScalarState* s);
 int scalar_cv(const FieldsView& v, ScalarState* s, double* cv, int* defined);
+int scalar_info(const ScalarState* s, int* Qs, double* D);
+int scalar_get_state(const FieldsView& v, ScalarState* s, float* h, float* h2, float* C);
+int scalar_put_state(const FieldsView& v, ScalarState* s, const float* h, const float* h2, const float* C);
 int scalar_compartment_cv(const FieldsView& v, ScalarState* s, int has_imp, double imp_z, double cv[3],
                           int defined[3], int64_t counts[3]);
 

@@ -174,6 +174,9 @@ hipError_t launch_stage(int Q, int storage, int dir, int odd, const Dom& d, void
                         const uint32_t* mask, hipStream_t s);
 hipError_t launch_init_with(int Q, int storage, const Dom& d, void* f, const uint32_t* mask, const float* rho,
                             const float* u, float tau, hipStream_t s);
+// previous step's post-collision populations (compact Q x N), see k_poststate
+hipError_t launch_poststate(int Q, int storage, int odd, const Dom& d, const void* f, const uint32_t* mask,
+                            const float* wall_u, float* out, hipStream_t s);
 hipError_t launch_moments(int Q, int storage, int odd, const Dom& d, const void* f, const uint32_t* mask,
                           const float* F_mom, HydroParams p, float* rho, float* ux, float* uy, float* uz,
                           hipStream_t s);

@@ -1083,6 +1083,61 @@ __global__ __launch_bounds__(256) void k_reduce(Dom d, const S* f, const uint32_
   if (threadIdx.x == 0) partials[blockIdx.x] = acc;
 }
 
+// The previous step's post-collision populations f*_q(j) (what the
+// reference's second buffer holds between steps: fields.rs ftmp, the
+// checkpoint's STALE_STASH section), recovered from the current state by
+// undoing the streaming: f*_q(j) = f_q(j + c_q) when j + c_q is fluid; the
+// reflected population f_opp(q)(j), less the moving-wall term 6 w rho
+// (c_opp . u_wall) (rho of the current state), when j + c_q is solid; 0 for
+// solid cells and where j + c_q lies beyond a non-periodic face (a slot the
+// next stream overwrites).  Exact copies on resting walls.
+template <int Q, typename S>
+__global__ void k_poststate(Dom d, const S* f, const uint32_t* mask, const float* wall_u, int odd, float* out) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= d.N) return;
+  const uint32_t m = mask[j];
+  if (m & kMaskSolid) {
+    for (int q = 0; q < Q; ++q) out[(size_t)q * d.N + j] = 0.0f;
+    return;
+  }
+  uint32_t x, y, z;
+  cell_xyz(d, j, x, y, z);
+  const Nbr nb(d, x, y, z);
+  float rho = 1.0f;
+  if (wall_u && (m & kMaskMoving)) {
+    float dr = 0.0f;
+    sfor<0, Q>([&](auto qc) { dr = dr + read_pop<Q, S, decltype(qc)::value>(f, d.ps, nb, j, m, odd != 0); });
+    rho = 1.0f + dr;
+  }
+  sfor<0, Q>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    constexpr int o = lat_opp(q);
+    constexpr int cx = lat_c(q, 0), cy = lat_c(q, 1), cz = lat_c(q, 2);
+    float v = 0.0f;
+    if (q != 0 && ((m >> o) & 1u)) {  // j + c_q solid: reflected into f_opp(j)
+      v = read_pop<Q, S, o>(f, d.ps, nb, j, m, odd != 0);
+      if (wall_u && (m & kMaskMoving)) {
+        const uint32_t t = j + (uint32_t)nb.template off<cx, cy, cz>();
+        const float* wu = wall_u + 3 * (size_t)t;
+        v = v - 6.0f * lat_wf(Q, o) * rho * cdot<o>(wu[0], wu[1], wu[2]);
+      }
+    } else {
+      const int64_t tx = (int64_t)x + cx, ty = (int64_t)y + cy, tz = (int64_t)z + cz;
+      const bool out_x = (tx < 0 || tx >= (int64_t)d.nx) && !d.per[0];
+      const bool out_y = (ty < 0 || ty >= (int64_t)d.ny) && !d.per[1];
+      const bool out_z = (tz < 0 || tz >= (int64_t)d.nz) && !d.per[2];
+      if (!(out_x || out_y || out_z)) {
+        const uint32_t t = j + (uint32_t)nb.template off<cx, cy, cz>();
+        uint32_t ux, uy, uz;
+        cell_xyz(d, t, ux, uy, uz);
+        const Nbr nt(d, ux, uy, uz);
+        v = read_pop<Q, S, q>(f, d.ps, nt, t, mask[t], odd != 0);
+      }
+    }
+    out[(size_t)q * d.N + j] = v;
+  });
+}
+
 // masks: links in the pull convention plus cell flags
 template <int Q>
 __global__ void k_build_mask(Dom d, const uint8_t* solid, const float* wall_u, uint32_t* mask,
@@ -1409,6 +1464,25 @@ static hipError_t moments_s(int Q, int odd, const Dom& d, const void* f, const u
     hipLaunchKernelGGL((k_moments<27, S>), dim3(blocks_for(d.N)), dim3(256), 0, s, d, (const S*)f, mask, F_mom, p,
                        odd, rho, ux, uy, uz);
   return hipGetLastError();
+}
+
+template <typename S>
+static hipError_t poststate_s(int Q, int odd, const Dom& d, const void* f, const uint32_t* mask, const float* wall_u,
+                              float* out, hipStream_t s) {
+  if (Q == 19)
+    hipLaunchKernelGGL((k_poststate<19, S>), dim3(blocks_for(d.N)), dim3(256), 0, s, d, (const S*)f, mask, wall_u, odd,
+                       out);
+  else
+    hipLaunchKernelGGL((k_poststate<27, S>), dim3(blocks_for(d.N)), dim3(256), 0, s, d, (const S*)f, mask, wall_u, odd,
+                       out);
+  return hipGetLastError();
+}
+
+hipError_t launch_poststate(int Q, int storage, int odd, const Dom& d, const void* f, const uint32_t* mask,
+                            const float* wall_u, float* out, hipStream_t s) {
+  if (storage == kF32) return poststate_s<float>(Q, odd, d, f, mask, wall_u, out, s);
+  if (storage == kF16) return poststate_s<_Float16>(Q, odd, d, f, mask, wall_u, out, s);
+  return poststate_s<bf16s>(Q, odd, d, f, mask, wall_u, out, s);
 }
 
 hipError_t launch_moments(int Q, int storage, int odd, const Dom& d, const void* f, const uint32_t* mask,

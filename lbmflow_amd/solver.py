@@ -153,6 +153,8 @@ class Solver:
         self._ibm_key = None
         self.scalars = []
         self.sources = []
+        self.faces = [("closed", None)] * 6
+        self.patches = []
 
     # ---- lifetime ---------------------------------------------------------
     def close(self):
@@ -277,6 +279,7 @@ class Solver:
             vv = np.atleast_1d(np.asarray(value, np.float64))
             v[:vv.size] = vv
         self._chk(self.lib.lbm_set_face_bc(self.h, f, self.FACE_KINDS[kind], _dp(v)))
+        self.faces[f] = (kind, None if kind in ("closed", "outflow") else v.copy())
 
     def set_inlet_profile(self, face, values):
         """``Solver::set_inlet_profile`` (solver.rs:3322-3350): ``values`` of
@@ -320,6 +323,8 @@ class Solver:
                 for c in range(vv.size):
                     e.value[c] = float(vv[c])
         self._chk(self.lib.lbm_set_face_patches(self.h, len(patches), arr))
+        self.patches = [((self.FACES[fc] if isinstance(fc, str) else int(fc)), tuple(lo), tuple(hi), kind, value)
+                        for fc, lo, hi, kind, value in patches]
 
     def wale_update(self, cw: float = WALE_CW, tau_eff_max: float | None = None, want_nu_t: bool = False):
         """One ``WaleLes::update`` (les.rs:120-211): install the WALE omega

@@ -1550,6 +1550,86 @@ struct Solver {
     return 0;
   }
 
+  // ---- checkpoint rank file (write_part_checkpoint, solver.rs:4536-4630) ----
+  // The reference's v3 layout from this solver's own buffers: header, section
+  // table, F_PRIMARY (f), STALE_STASH (ftmp), MOMENTS, SOLID, FORCE_FIELD (if
+  // a field is set), SCALAR_FIELDS (if any; names "scalar<k>"), FNV-1a 64
+  // trailer of the payload.  Returns 0, or -1 if the file cannot be written.
+  int write_checkpoint(const char* path) const {
+    auto u32 = [](std::vector<uint8_t>& b, uint32_t v) { b.insert(b.end(), (uint8_t*)&v, (uint8_t*)&v + 4); };
+    auto u64 = [](std::vector<uint8_t>& b, uint64_t v) { b.insert(b.end(), (uint8_t*)&v, (uint8_t*)&v + 8); };
+    auto reals = [](std::vector<uint8_t>& b, const T* v, size_t n) {
+      b.insert(b.end(), (const uint8_t*)v, (const uint8_t*)(v + n));
+    };
+    std::vector<std::pair<uint32_t, std::vector<uint8_t>>> sec;
+    std::vector<uint8_t> b;
+    reals(b, f.data(), f.size());
+    sec.push_back({1, b});
+    b.clear();
+    reals(b, ftmp.data(), ftmp.size());
+    sec.push_back({2, b});
+    b.clear();
+    for (const std::vector<T>* m : {&rho, &ux, &uy, &uz}) reals(b, m->data(), m->size());
+    sec.push_back({3, b});
+    b.clear();
+    for (uint8_t v : solid) b.push_back(v ? 1 : 0);
+    sec.push_back({4, b});
+    if (has_ff) {
+      b.clear();
+      reals(b, ff.data(), ff.size());
+      sec.push_back({5, b});
+    }
+    if (!scalars.empty()) {
+      b.clear();
+      u32(b, (uint32_t)scalars.size());
+      for (size_t k = 0; k < scalars.size(); ++k) {
+        const ScalarDist<T>& sc = scalars[k];
+        const std::string name = "scalar" + std::to_string(k);
+        u32(b, (uint32_t)name.size());
+        b.insert(b.end(), name.begin(), name.end());
+        const double D = sc.diffusivity;
+        b.insert(b.end(), (const uint8_t*)&D, (const uint8_t*)&D + 8);
+        u64(b, sc.h.size());
+        reals(b, sc.h.data(), sc.h.size());
+        u64(b, sc.htmp.size());
+        reals(b, sc.htmp.data(), sc.htmp.size());
+        u64(b, sc.conc.size());
+        reals(b, sc.conc.data(), sc.conc.size());
+      }
+      sec.push_back({6, b});
+    }
+    std::vector<uint8_t> payload, head;
+    head.insert(head.end(), (const uint8_t*)"LBMKPT\0\0", (const uint8_t*)"LBMKPT\0\0" + 8);
+    u32(head, 3);
+    head.push_back(0);
+    head.push_back(sizeof(T) == 4 ? 0 : 1);
+    const uint16_t lat = L.Q == 19 ? 319 : 327, q16 = (uint16_t)L.Q, d16 = 3;
+    head.insert(head.end(), (const uint8_t*)&lat, (const uint8_t*)&lat + 2);
+    head.insert(head.end(), (const uint8_t*)&q16, (const uint8_t*)&q16 + 2);
+    head.insert(head.end(), (const uint8_t*)&d16, (const uint8_t*)&d16 + 2);
+    u64(head, np);
+    u64(head, nc);
+    u32(head, (uint32_t)sec.size());
+    for (auto& e : sec) {
+      u32(head, e.first);
+      u64(head, payload.size());
+      u64(head, e.second.size());
+      payload.insert(payload.end(), e.second.begin(), e.second.end());
+    }
+    uint64_t h = 0xcbf29ce484222325ull;
+    for (uint8_t v : payload) {
+      h ^= v;
+      h *= 0x100000001b3ull;
+    }
+    FILE* fp = fopen(path, "wb");
+    if (!fp) return -1;
+    bool ok = fwrite(head.data(), 1, head.size(), fp) == head.size();
+    ok = ok && fwrite(payload.data(), 1, payload.size(), fp) == payload.size();
+    ok = ok && fwrite(&h, 1, 8, fp) == 8;
+    ok = fclose(fp) == 0 && ok;
+    return ok ? 0 : -1;
+  }
+
   // ---- passive scalar ADE, solver.rs:2113-2205 / scalar.rs:6-15 --------------
   void scalar_eq(const ScalarDist<T>& s, T conc, const T u[3], T* out) const {
     if (!s.d3q7) {
@@ -1935,6 +2015,13 @@ void orc_cm_transform(int Q, double* raw_from_pop, double* pop_from_raw, uint8_t
 // x86 arithmetic takes a microcode path; values that small cannot change a
 // quantity of interest.  Off unless a caller asks (fixture generation).
 void orc_set_cm_factorised(int on) { g_cm_factorised = on != 0; }
+
+int orc_write_checkpoint(void* p, const char* path) {
+  Handle* h = (Handle*)p;
+  int r = 0;
+  DISPATCH(h, r = S->write_checkpoint(path));
+  return r;
+}
 
 void orc_flush_subnormals(int on) {
   const unsigned bits = 0x8040u;  // FTZ (bit 15) | DAZ (bit 6)

@@ -50,6 +50,7 @@ def lib():
         L.orc_free.argtypes = [vp]
         L.orc_flush_subnormals.argtypes = [C.c_int]
         L.orc_set_cm_factorised.argtypes = [C.c_int]
+        L.orc_write_checkpoint.argtypes = [vp, C.c_char_p]
         L.orc_step.argtypes = [vp, C.c_int]
         L.orc_collide_only.argtypes = [vp]
         L.orc_update_moments.argtypes = [vp]
@@ -309,6 +310,12 @@ class OracleSolver:
         om, nt, mx = np.zeros(self.n), np.zeros(self.n), np.zeros(1)
         k = lib().orc_wale(self.h, cw, 0.0 if tau_eff_max is None else tau_eff_max, _d(om), _d(nt), _d(mx))
         return om, nt, int(k), float(mx[0])
+
+    def write_checkpoint(self, path):
+        """The reference's v3 rank file (solver.rs:4536-4630) from this
+        solver's own buffers (f, ftmp, moments, solid, force field, scalars)."""
+        if lib().orc_write_checkpoint(self.h, str(path).encode()) != 0:
+            raise OSError(f"cannot write {path}")
 
     def apply_ibm(self, pos, weight, center, omega, max_iter=3, tol=1e-2, radius=1, relax=1.0, stencil_dim=3):
         """Solver::apply_rotating_ibm (solver.rs:3016-3282).  stencil_dim=2

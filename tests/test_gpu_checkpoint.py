@@ -206,3 +206,150 @@ def test_section_offset_past_the_payload_is_truncated(tmp_path):
     with pytest.raises(K.CheckpointError) as e:
         _cavity().restore_checkpoint(str(tmp_path))
     assert e.value.code == "CKPT_TRUNCATED"
+
+
+def _sections_equal(ra, rb, what):
+    """Same header, same section table, and section contents equal value for
+    value (float sections compared as numbers: +0.0 and -0.0 equal)."""
+    for key in ("ver", "endian", "dtype", "lat", "q", "d", "np", "nc"):
+        assert ra[key] == rb[key], f"{what}: header {key}"
+    assert sorted(ra["sec"]) == sorted(rb["sec"]), f"{what}: sections {sorted(ra['sec'])} vs {sorted(rb['sec'])}"
+    for sid in ra["sec"]:
+        a, b = ra["sec"][sid], rb["sec"][sid]
+        assert len(a) == len(b), f"{what}: section {sid} length"
+        if sid == 4:
+            assert a == b, f"{what}: solid section"
+        elif sid == 6:
+            _scalar_sections_equal(a, b, what)
+        else:
+            fa, fb = np.frombuffer(a, np.float32), np.frombuffer(b, np.float32)
+            bad = np.flatnonzero(fa != fb)
+            assert bad.size == 0, f"{what}: section {sid}: {bad.size} values differ, first {bad[:5]}"
+
+
+def _scalar_sections_equal(a, b, what):
+    def parse(buf):
+        pos = 0
+        (count,) = struct.unpack_from("<I", buf, pos)
+        pos += 4
+        out = []
+        for _ in range(count):
+            (nl,) = struct.unpack_from("<I", buf, pos)
+            name = buf[pos + 4:pos + 4 + nl].decode()
+            pos += 4 + nl
+            (D,) = struct.unpack_from("<d", buf, pos)
+            pos += 8
+            arrs = []
+            for _ in range(3):
+                (n,) = struct.unpack_from("<Q", buf, pos)
+                arrs.append(np.frombuffer(buf, np.float32, n, pos + 8))
+                pos += 8 + 4 * n
+            out.append((name, D, arrs))
+        assert pos == len(buf)
+        return out
+
+    pa, pb = parse(a), parse(b)
+    assert len(pa) == len(pb)
+    for (na, Da, xa), (nb, Db, xb) in zip(pa, pb):
+        assert na == nb and Da == Db, what
+        for u, v in zip(xa, xb):
+            assert np.array_equal(u, v), f"{what}: scalar arrays differ by {np.abs(u - v).max()}"
+
+
+@pytest.mark.parametrize("scalar", ["hydro", "d3q7"])
+@pytest.mark.parametrize("steps", [6, 7])
+def test_rank_file_matches_the_oracle_writer(tmp_path, scalar, steps):
+    """The library's rank file against the oracle's (the reference's buffers
+    written in its v3 layout, solver.rs:4536-4630) for the same state: a
+    closed D3Q19 TRT tank with the rotating IBM (force field section) and a
+    tracer (scalar section), after an even and an odd number of steps.  The
+    stale-stash section is the previous step's post-collision populations,
+    recovered by the library from its single in-place buffer."""
+    import lbmflow_amd as L
+    from oracle import oracle as O
+
+    tc = cases.tank_case(20, nu=0.02, collision="trt", Q=19, sigma=2.0)
+    n = tc["n"]
+    orc = O.OracleSolver(19, (n,) * 3, prec=4, collision=O.TRT, nu=tc["nu"], solid=tc["solid"])
+    hip = L.Solver(19, (n,) * 3, nu=tc["nu"], collision="trt", solid=tc["solid"])
+    hip.set_rotating_body(tc["pos"], tc["w"], tc["c"], (0.0, 0.0, tc["omega"]))
+    ko = orc.scalar_enable(0.02, scalar == "d3q7")
+    orc.scalar_set(ko, tc["blob"])
+    kh = hip.enable_scalar_distribution(0.02, scalar)
+    hip.set_scalar_concentration(kh, tc["blob"])
+    for _ in range(steps):
+        orc.clear_force_field()
+        orc.apply_ibm(tc["pos"], tc["w"], tc["c"], (0.0, 0.0, tc["omega"]))
+        orc.step()
+    hip.run_with_ibm(steps, graph=False)
+    orc.write_checkpoint(tmp_path / "orc.bin")
+    man = hip.save_checkpoint(str(tmp_path / "hip"))
+    ra = _parse_rank(os.path.join(str(tmp_path / "hip"), man["ranks"][0]["file"]))
+    rb = _parse_rank(str(tmp_path / "orc.bin"))
+    _sections_equal(ra, rb, f"{scalar}, {steps} steps")
+
+
+def test_scalar_resume_bit_exact(tmp_path):
+    import lbmflow_amd as L
+
+    tc = cases.tank_case(20, nu=0.02, collision="trt", Q=19, sigma=2.0)
+
+    def mk():
+        s = L.Solver(19, (20,) * 3, nu=tc["nu"], collision="trt", solid=tc["solid"])
+        s.set_rotating_body(tc["pos"], tc["w"], tc["c"], (0.0, 0.0, tc["omega"]))
+        k = s.enable_scalar_distribution(0.02, "d3q7")
+        return s, k
+
+    a, ka = mk()
+    a.set_scalar_concentration(ka, tc["blob"])
+    a.run_with_ibm(7, graph=False)
+    a.save_checkpoint(str(tmp_path))
+    a.run_with_ibm(5, graph=False)
+    b, kb = mk()
+    assert b.restore_checkpoint(str(tmp_path)) == 7
+    b.run_with_ibm(5, graph=False)
+    assert np.array_equal(a.populations(), b.populations())
+    assert np.array_equal(a.gather_scalar_concentration(ka), b.gather_scalar_concentration(kb))
+    # a run with another diffusivity is refused before anything is installed
+    c = L.Solver(19, (20,) * 3, nu=tc["nu"], collision="trt", solid=tc["solid"])
+    c.set_rotating_body(tc["pos"], tc["w"], tc["c"], (0.0, 0.0, tc["omega"]))
+    c.enable_scalar_distribution(0.03, "d3q7")
+    with pytest.raises(K.CheckpointError) as e:
+        c.restore_checkpoint(str(tmp_path))
+    assert e.value.code == "CKPT_SCENARIO_MISMATCH"
+
+
+@pytest.mark.parametrize("outlet", ["pressure", "convective"])
+def test_open_channel_resume_bit_exact(tmp_path, outlet):
+    """Open faces: the entering unknowns of the face cells travel in the
+    stale-stash section, so a restored run continues bit for bit (the
+    convective outlet reads its previous values)."""
+    import lbmflow_amd as L
+
+    dims = (24, 10, 6)
+    nx, ny, nz = dims
+    solid = np.zeros((nz, ny, nx), np.uint8)
+    solid[:, 0, :] = 1
+    solid[:, ny - 1, :] = 1
+
+    def mk():
+        s = L.Solver(19, dims, nu=0.02, collision="trt", periodic=(False, False, True), solid=solid.reshape(-1))
+        s.set_face_bc("x-", "velocity", (0.03, 0.0, 0.0))
+        s.set_face_bc("x+", outlet, (1.0,) if outlet == "pressure" else (0.03,))
+        return s
+
+    a = mk()
+    a.run(9)
+    a.save_checkpoint(str(tmp_path))
+    a.run(6)
+    b = mk()
+    assert b.restore_checkpoint(str(tmp_path)) == 9
+    b.run(6)
+    assert np.array_equal(a.populations(), b.populations())
+    # other face rules: scenario hash
+    c = L.Solver(19, dims, nu=0.02, collision="trt", periodic=(False, False, True), solid=solid.reshape(-1))
+    c.set_face_bc("x-", "velocity", (0.04, 0.0, 0.0))
+    c.set_face_bc("x+", outlet, (1.0,) if outlet == "pressure" else (0.03,))
+    with pytest.raises(K.CheckpointError) as e:
+        c.restore_checkpoint(str(tmp_path))
+    assert e.value.code == "CKPT_SCENARIO_MISMATCH"
