@@ -175,6 +175,44 @@ def small_tank(n, tip, nu, Q=19, collision="trt"):
     return s, {"n": n, "T": T, "D": D, "omega": omega, "tip": tip, "nu": nu, "markers": len(w)}
 
 
+def workload_cavity(args):
+    """The plain-method configuration: 3D lid-driven cavity, D3Q19 TRT
+    (magic 3/16), 64^3, fp32, Re 100, lid 0.05 lu, quiescent start, 1000
+    steps; MLUPS and the centreline velocity profiles of the reference's
+    cavity test (t15_5_cavity3d.rs:183-243)."""
+    import lbmflow_amd as L
+    from lbmflow_amd import cavity
+
+    n, u_lid, re = args.size if args.size != 512 else 64, 0.05, 100.0
+    nu = u_lid * (n - 2) / re
+    s = L.Solver(19, (n, n, n), nu=nu, collision="trt")
+    u = np.zeros((6, 3))
+    u[5] = [u_lid, 0.0, 0.0]
+    s.build_wall_rims([1] * 6, u)
+    fluid = s.fluid_cells()
+    s.synchronize()
+    t0 = time.perf_counter()
+    s.run(args.steps)
+    s.synchronize()
+    dt = time.perf_counter() - t0
+    rho, ux, uy, uz = s.moments()
+    zf, uu, xf, ww = cavity.centrelines(ux, uz, n, u_lid)
+    mass = s.mass_deviation()
+    s.close()
+    return {
+        "metric": "MLUPS, 3D lid-driven cavity D3Q19 TRT 64^3 Re 100 (1000 steps)",
+        "value": round(n ** 3 * args.steps / dt / 1e6, 2), "unit": "MLUPS", "n_gpus": 1, "steps": args.steps,
+        "warmup": 0, "ms_per_step": round(1e3 * dt / args.steps, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic (quiescent start)",
+        "config": {"workload": f"lid-driven cavity D3Q19 TRT {n}^3, Re 100, lid 0.05", "grid": [n] * 3,
+                   "fluid_cells": int(fluid), "parallelism": "single GPU"},
+        "mlups_fluid": round(fluid * args.steps / dt / 1e6, 2),
+        "centreline_u_vs_z": [[float(a), float(b)] for a, b in zip(zf, uu)],
+        "centreline_w_vs_x": [[float(a), float(b)] for a, b in zip(xf, ww)],
+        "checks": {"mass_deviation": float(mass)},
+    }
+
+
 def workload_surface(args):
     """The single-phase stirred tank at 256^3 (D3Q27 central moment, fp32,
     T = H = 240 lu, 4 baffles T/10, Rushton D = C = T/3, tip speed 0.05,
@@ -447,10 +485,11 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=100)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-sweep", action="store_true", help="skip the 16-bit storage and Np-accuracy sweep")
-    ap.add_argument("--workload", default="tank", choices=["tank", "batch", "adjoint", "les", "channel", "surface"],
+    ap.add_argument("--workload", default="tank", choices=["tank", "batch", "adjoint", "les", "channel", "surface",
+                                                              "cavity"],
                     help="tank: the 512^3 benchmark (default); batch: 64-tank sweep; adjoint: torque gradient; "
                          "les: the tank with per-step WALE; channel: open-face channel; surface: the 256^3 tank "
-                         "with surface-sampled markers, 20k steps, Np")
+                         "with surface-sampled markers, 20k steps, Np; cavity: the 64^3 lid-driven cavity")
     ap.add_argument("--seed", type=int, default=20260512)
     ap.add_argument("--snapshots", type=int, default=128, help="adjoint: binomial checkpoint snapshots")
     args = ap.parse_args()
@@ -460,12 +499,14 @@ def main():
         sys.exit(2)
     if args.steps is None:
         args.steps = {"tank": 200, "batch": 5000, "adjoint": 4000, "les": 100, "channel": 200,
-                      "surface": 20000}[args.workload]
+                      "surface": 20000, "cavity": 1000}[args.workload]
     if args.warmup is None:
-        args.warmup = {"tank": 500, "batch": 100, "adjoint": 0, "les": 200, "channel": 100, "surface": 0}[args.workload]
+        args.warmup = {"tank": 500, "batch": 100, "adjoint": 0, "les": 200, "channel": 100, "surface": 0,
+                       "cavity": 0}[args.workload]
     if args.workload != "tank":
         fn = {"batch": workload_batch, "adjoint": workload_adjoint, "les": workload_les,
-              "channel": workload_channel, "surface": workload_surface}[args.workload]
+              "channel": workload_channel, "surface": workload_surface,
+              "cavity": workload_cavity}[args.workload]
         print(json.dumps(fn(args)), flush=True)
         return
 
