@@ -20,8 +20,8 @@ committed rocprofv3 counter summary for this workload, if there is one.
 `storage_sweep` repeats the timing with the other population storages and
 compares the 128^3 tank's power number with each storage against the FP64
 oracle series in tests/golden/.  `cpu_baseline` times the CPU oracle (a
-restatement of the reference's CpuScalar path) on the same tank at a
-reduced size, on this host's cores.
+restatement of the reference's CpuScalar path) on the same tank, a short
+window from the GPU run's developed state, on this host's cores.
 """
 from __future__ import annotations
 
@@ -79,12 +79,30 @@ def build_gpu(cfg, storage, collision, Q):
     return s, len(w)
 
 
-def cpu_baseline(n_cpu, steps, collision, Q):
-    """The oracle (CpuScalar restatement) on the same tank at n_cpu^3."""
+def cpu_info():
+    """The host the CPU leg runs on: model, logical CPUs, OMP_NUM_THREADS."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count(), "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
+
+
+def cpu_baseline(cfg, f_dev, gpu_steps, steps, collision, Q):
+    """The oracle (the CpuScalar restatement in fp32, OpenMP over rows) on
+    the benchmark's own tank, started from the populations the GPU run
+    reached after `gpu_steps` steps: a flow from rest exercises no storage
+    rounding and converges the IBM in fewer sweeps.  One untimed step, then
+    `steps` timed runner steps (clear force, rotating IBM, collide-stream)."""
     from oracle import oracle as O
 
-    cfg = tank_config(n_cpu)
-    dims = (n_cpu,) * 3
+    n = cfg["n"]
+    dims = (n,) * 3
     solid = O.tank_mask(dims, 1.0, cfg["T"], cfg["H"])
     solid, _ = O.baffle_mask(dims, 1.0, cfg["T"], cfg["H"], cfg["baffle_width"], cfg["baffle_thickness"], solid)
     pos, w, vel, c = O.radial_impeller(dims, 1.0, cfg["T"], cfg["H"], cfg["D"], cfg["clearance"], 6, 0.0,
@@ -92,6 +110,8 @@ def cpu_baseline(n_cpu, steps, collision, Q):
     coll = {"cm": O.CM, "trt": O.TRT, "bgk": O.BGK}[collision]
     orc = O.OracleSolver(Q, dims, prec=4, collision=coll, nu=cfg["nu"], solid=solid)
     fluid = orc.fluid_cells()
+    orc.set_f(f_dev)
+    orc.update_moments()
 
     def one():
         orc.clear_force_field()
@@ -100,10 +120,12 @@ def cpu_baseline(n_cpu, steps, collision, Q):
 
     one()
     t0 = time.perf_counter()
-    for _ in range(steps):
+    for k in range(steps):
         one()
+        print(f"[bench] cpu oracle {n}^3: step {k + 1}/{steps}", file=sys.stderr, flush=True)
     dt = time.perf_counter() - t0
-    return fluid * steps / dt / 1e6, dt
+    info = dict(cpu_info(), threads_used=O.max_threads())
+    return fluid * steps / dt / 1e6, dt, info
 
 
 def timed_span(s, steps):
@@ -235,20 +257,27 @@ def workload_surface(args):
     steps = args.steps
     torque = []
     chunk = 1000
+    # the QOI window is steps steps//2 .. steps (1-based): the device's FP64
+    # torque integral is cleared after step steps//2 - 1
+    w0 = max(steps // 2 - 1, 0)
     s.synchronize()
     t0 = time.perf_counter()
     done = 0
     while done < steps:
-        k = min(chunk, steps - done)
+        k = min(chunk, (w0 if done < w0 else steps) - done)
         tq, diag = s.run_with_ibm(k, graph=True)
         torque.append(-tq[:, 2])
         done += k
+        if done == w0:
+            s.torque_integral(reset=True)
         print(f"[bench] surface tank: {done}/{steps} steps", file=sys.stderr, flush=True)
     s.synchronize()
     dt = time.perf_counter() - t0
+    tsum, tcount = s.torque_integral()
+    tmean = float(-tsum[2] / tcount)
     torque = np.concatenate(torque)
     rec = np.arange(1, steps + 1)
-    tmean = float(np.mean(torque[rec >= steps // 2]))
+    tmean_host = float(np.mean(torque[rec >= steps // 2]))
     q = L.power_qois(tmean, cfg["omega"], 1.0, cfg["D"], 1.0)
     mass = s.total_mass_f64()
     s.close()
@@ -263,7 +292,9 @@ def workload_surface(args):
                    "markers": int(len(w)), "marker_area_lu2": 0.1, "tip_speed_lu": 0.05, "re_imp": 5000,
                    "nu_lu": cfg["nu"], "fluid_cells": int(fluid), "parallelism": "single GPU"},
         "np": q["np"], "torque_mean_lu": tmean, "revolutions": steps * cfg["omega"] / (2 * np.pi),
-        "checks": {"mass_finite": bool(np.isfinite(mass)), "ibm_iterations_last": diag["iterations"]},
+        "checks": {"mass_finite": bool(np.isfinite(mass)), "ibm_iterations_last": diag["iterations"],
+                   "window_updates": tcount, "torque_mean_host_series": tmean_host,
+                   "torque_mean_rel_diff": abs(tmean - tmean_host) / max(abs(tmean_host), 1e-300)},
     }
 
 
@@ -481,8 +512,7 @@ def main():
     ap.add_argument("--storage", default="f32", choices=["f32", "f16", "bf16"])
     ap.add_argument("--collision", default="cm", choices=["cm", "trt", "bgk"])
     ap.add_argument("--lattice", type=int, default=27, choices=[19, 27])
-    ap.add_argument("--cpu-size", type=int, default=128)
-    ap.add_argument("--cpu-steps", type=int, default=100)
+    ap.add_argument("--cpu-steps", type=int, default=10, help="timed oracle steps of the tank on the host")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-sweep", action="store_true", help="skip the 16-bit storage and Np-accuracy sweep")
     ap.add_argument("--workload", default="tank", choices=["tank", "batch", "adjoint", "les", "channel", "surface",
@@ -546,6 +576,8 @@ def main():
     copy_gbps = s.ctx.stream_copy_gbps(4 << 30, 20)
     prof = profile_name(args.size, args.lattice, args.collision, args.storage)
     traffic = profiled_traffic(prof)
+    steps_done = s.time
+    f_dev = None if args.no_cpu_baseline else s.populations()
     s.close()
 
     sweep = None
@@ -564,15 +596,14 @@ def main():
             sweep["np_vs_f64_oracle"] = nperr
 
     cpu = None
-    if not args.no_cpu_baseline:
-        threads = os.cpu_count() or 1
-        omp = os.environ.get("OMP_NUM_THREADS")
-        if omp:
-            threads = int(omp)
-        v, secs = cpu_baseline(args.cpu_size, args.cpu_steps, args.collision, Q)
-        cpu = {"value": round(v, 3), "unit": "MLUPS", "cores": threads, "kind": "port",
-               "sample": f"oracle D3Q{Q} {args.collision} fp32 stirred tank {args.cpu_size}^3 (same ratios), "
-                         f"{args.cpu_steps} steps incl. IBM, {secs:.1f}s"}
+    if f_dev is not None:
+        v, secs, info = cpu_baseline(cfg, f_dev, steps_done, args.cpu_steps, args.collision, Q)
+        del f_dev
+        cpu = {"value": round(v, 3), "unit": "MLUPS", "cores": info["threads_used"], "kind": "port",
+               "sample": f"oracle D3Q{Q} {args.collision} fp32 on the same {args.size}^3 tank (IBM + "
+                         f"collide-stream), {args.cpu_steps} steps timed after one untimed step, started from "
+                         f"the GPU run's populations after {steps_done} steps; {secs:.1f}s",
+               **info}
 
     out = {
         "metric": f"MLUPS & % HBM roofline, D3Q{Q} tank {args.size}^3",

@@ -364,6 +364,15 @@ int lbm_ibm_apply(lbm_fields* fl, lbm_ibm_diag* diag);
  * eager launches.  The reference's equivalent seam is Backend::run_chunk_size
  * / finish_run_chunk (backend.rs:340-349). */
 int lbm_run_ibm_span(lbm_fields* fl, int steps, int use_graph, double* torque, lbm_ibm_diag* last);
+/* Running FP64 torque integral of the registered body, kept on the device
+ * and advanced by every rotating-IBM update (lbm_ibm_apply, spans, batches):
+ * out[0..2] = sum of IbmDiagnostics::torque over the updates since the body
+ * was registered or the integral last reset, out[3] = their count.  reset
+ * != 0 clears it after the read.  Replaces the host-side accumulation the
+ * runner does for the power number (runner.rs:1514-1530, mean shaft torque
+ * over the QOI window: qoi.rs:477-505), so a span needs no per-step torque
+ * readback. */
+int lbm_ibm_torque_integral(lbm_fields* fl, int reset, double out[4]);
 
 /* ---- batched domains ------------------------------------------------------
  * No single reference counterpart: lbm-cli's sweep runs its cases one after
@@ -388,6 +397,11 @@ int lbm_batch_run_span(lbm_batch* b, int steps, int use_graph, double* torque);
  * scratch copies of the populations (the members' state is untouched) and
  * keeps the fastest. */
 int lbm_batch_autotune(lbm_batch* b, int trial_steps, uint32_t* best_grid, double* best_ms);
+/* The running FP64 torque integrals of the members' bodies (as
+ * lbm_ibm_torque_integral), out = members x 4 in member order (zeros for a
+ * member without a body); they advance only while the members step through
+ * this batch. */
+int lbm_batch_torque_integral(lbm_batch* b, int reset, double* out);
 
 /* ---- adjoint of the time-averaged shaft torque --------------------------
  * No reference counterpart (the reference differentiates only by finite

@@ -124,12 +124,14 @@ SIGNATURES = {
     "lbm_ibm_set_body": (_I, [_P, C.c_int64, _pD, _pD, _pD, _pD, C.POINTER(IbmConfig)]),
     "lbm_ibm_apply": (_I, [_P, C.POINTER(IbmDiag)]),
     "lbm_run_ibm_span": (_I, [_P, _I, _I, _pD, C.POINTER(IbmDiag)]),
+    "lbm_ibm_torque_integral": (_I, [_P, _I, _pD]),
     "lbm_batch_create": (_I, [_P, C.POINTER(_P), _I, C.POINTER(_P)]),
     "lbm_batch_destroy": (None, [_P]),
     "lbm_batch_info": (_I, [_P, _pI, _pU32, _pU32, _pU32]),
     "lbm_batch_set_grid": (_I, [_P, C.c_uint32]),
     "lbm_batch_run_span": (_I, [_P, _I, _I, _pD]),
     "lbm_batch_autotune": (_I, [_P, _I, _pU32, _pD]),
+    "lbm_batch_torque_integral": (_I, [_P, _I, _pD]),
     "lbm_adjoint_torque": (_I, [_P, _I, _I, _I, _pD, _pD, _pD, C.POINTER(C.c_int64)]),
     "lbm_revolve_schedule": (_I, [_I, _I, _pI, _I, _pI]),
     "lbm_checkpoint_write_part": (_I, [_P, C.c_char_p, C.POINTER(CkptInfo)]),

@@ -1764,6 +1764,12 @@ int lbm_ibm_set_body(lbm_fields* fl, int64_t n, const double* pos, const double*
   return grow_force_list(fl, cells);
 }
 
+int lbm_ibm_torque_integral(lbm_fields* fl, int reset, double out[4]) {
+  if (!fl || !out) return LBM_ERR_INVALID;
+  if (!fl->ibm) return fail(fl->ctx, LBM_ERR_STATE, "no IBM body registered");
+  return ibm_torque_integral(view_of(fl), fl->ibm, reset, out);
+}
+
 int lbm_ibm_apply(lbm_fields* fl, lbm_ibm_diag* diag) {
   if (!fl || !diag) return LBM_ERR_INVALID;
   if (!fl->ibm) return fail(fl->ctx, LBM_ERR_STATE, "no IBM body registered");
@@ -2135,6 +2141,16 @@ int lbm_batch_run_span(lbm_batch* b, int steps, int use_graph, double* torque) {
     HIPCHK(b->ctx, hipStreamSynchronize(s));
   }
   return LBM_OK;
+}
+
+int lbm_batch_torque_integral(lbm_batch* b, int reset, double* out) {
+  if (!b || !out) return LBM_ERR_INVALID;
+  const size_t K = b->mem.size();
+  if (!b->ibm) {
+    std::fill(out, out + 4 * K, 0.0);
+    return LBM_OK;
+  }
+  return ibm_torque_integral(batch_view(b), b->ibm, reset, out);
 }
 
 int lbm_batch_autotune(lbm_batch* b, int trial_steps, uint32_t* best_grid, double* best_ms) {

@@ -115,6 +115,7 @@ struct IbmState {
   double* mslip = nullptr;   // M
   uint8_t* mact = nullptr;   // M
   IbmDevDiag* diag = nullptr;  // K
+  double* integ = nullptr;     // K x 4: running sums of the torque (x, y, z) and the update count
   // adjoint (single body): per-sweep impulses and du recorded by the
   // forward update, and the reverse-sweep work arrays
   double* imp_hist = nullptr;  // max_iterations x M x 3
@@ -344,17 +345,27 @@ __global__ void k_ibm_commit(uint32_t U, const uint32_t* uc_cell, const uint32_t
   for (int a = 0; a < 3; ++a) F[3 * i + a] = F[3 * i + a] + (float)spread[3 * (size_t)u + a];
 }
 
+__global__ void k_zero_integ(double* p, int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0.0;
+}
+
 // one block: append the K diagnostics of this update to a device ring (read
 // back after a span of steps, so the step loop needs no host
-// synchronisation), then advance the ring position
-__global__ void k_ibm_record(const IbmDevDiag* diag, int K, IbmDevDiag* ring, uint32_t cap, uint32_t* pos) {
+// synchronisation), then advance the ring position (ring may be null); the
+// running f64 torque integral of each instance advances by this update's
+// torque
+__global__ void k_ibm_record(const IbmDevDiag* diag, int K, IbmDevDiag* ring, uint32_t cap, uint32_t* pos,
+                             double* integ) {
   __shared__ uint32_t slot;
-  if (threadIdx.x == 0) slot = *pos;
+  if (threadIdx.x == 0 && ring) slot = *pos;
   __syncthreads();
-  for (int k = threadIdx.x; k < K; k += blockDim.x)
-    if (slot < cap) ring[(size_t)slot * K + k] = diag[k];
+  for (int k = threadIdx.x; k < K; k += blockDim.x) {
+    if (ring && slot < cap) ring[(size_t)slot * K + k] = diag[k];
+    for (int c = 0; c < 3; ++c) integ[4 * k + c] += diag[k].torque[c];
+    integ[4 * k + 3] += 1.0;
+  }
   __syncthreads();
-  if (threadIdx.x == 0) *pos = slot + 1;
+  if (threadIdx.x == 0 && ring) *pos = slot + 1;
 }
 
 // ---------------------------------------------------------------------------
@@ -382,6 +393,7 @@ struct IbmSweepArgs {
   const uint32_t *moff, *uoff, *mk_off, *pt_u, *uc_off, *uc_m, *uc_cell;
   const double *pt_w, *uc_w, *target, *pos, *cen, *weight, *scale, *ksum, *unow, *rho;
   IbmDevDiag* diag;
+  double* integ;  // K x 4 running torque sums and update count
   const IbmDomain* dom;
   double relax, tol;
   int max_iter;
@@ -644,6 +656,11 @@ __global__ __launch_bounds__(1024) void k_ibm_sweeps(IbmSweepArgs a) {
       for (int c = 0; c < 3; ++c) F[3 * i + c] = F[3 * i + c] + (float)V.spread[3 * l + c];
     }
   }
+  // the running f64 torque integral of this body
+  if (tid == 0) {
+    for (int c = 0; c < 3; ++c) a.integ[4 * k + c] += diag->torque[c];
+    a.integ[4 * k + 3] += 1.0;
+  }
   // append this body's diagnostics to the ring at the current slot; the last
   // block to finish advances the slot (every block has read it by then)
   if (a.ring && tid == 0) {
@@ -707,7 +724,7 @@ static hipError_t scratch(T** dst, size_t n) {
 
 void ibm_destroy(IbmState* s) {
   if (!s) return;
-  void* ptrs[] = {s->uc_m,    s->uc_w,   s->moff,    s->uoff,   s->cen,    s->scale,   s->dom,    s->pos,    s->weight, s->target, s->mk_off,
+  void* ptrs[] = {s->integ,   s->uc_m,    s->uc_w,   s->moff,    s->uoff,   s->cen,    s->scale,   s->dom,    s->pos,    s->weight, s->target, s->mk_off,
                   s->mk_inst, s->pt_u,   s->pt_m,    s->pt_w,   s->uc_cell, s->uc_inst, s->uc_off, s->uc_ent,
                   s->ksum,    s->unow,   s->rho,     s->du,     s->spread, s->imp,    s->imp_next, s->mrep, s->mtq,
                   s->mslip,   s->mact,   s->diag,    s->imp_hist, s->du_hist, s->a_T, s->a_um, s->a_mob,
@@ -825,6 +842,8 @@ static IbmState* ibm_build(const std::vector<const IbmHost*>& parts, const lbm_i
   chain(scratch(&s->mslip, s->M));
   chain(scratch(&s->mact, s->M));
   chain(scratch(&s->diag, s->K));
+  chain(scratch(&s->integ, 4 * (size_t)s->K));
+  if (e == hipSuccess) e = hipMemset(s->integ, 0, 4 * (size_t)s->K * sizeof(double));
   if (e != hipSuccess) {
     *err = std::string("IBM device setup failed: ") + hipGetErrorString(e);
     ibm_destroy(s);
@@ -1053,10 +1072,8 @@ static int ibm_enqueue_wide(const FieldsView& v, IbmState* s, void* ring, uint32
     hipLaunchKernelGGL(k_ibm_commit, dim3(gu), dim3(256), 0, st, s->U, s->uc_cell, s->uc_inst, s->dom, s->spread);
     AUXCHK(v, hipGetLastError());
   }
-  if (ring) {
-    hipLaunchKernelGGL(k_ibm_record, dim3(1), dim3(64), 0, st, s->diag, s->K, (IbmDevDiag*)ring, cap, pos);
-    AUXCHK(v, hipGetLastError());
-  }
+  hipLaunchKernelGGL(k_ibm_record, dim3(1), dim3(64), 0, st, s->diag, s->K, (IbmDevDiag*)ring, cap, pos, s->integ);
+  AUXCHK(v, hipGetLastError());
   return LBM_OK;
 }
 
@@ -1105,6 +1122,7 @@ static int ibm_enqueue_impl(const FieldsView& v, IbmState* s, void* ring, uint32
   a.unow = s->unow;
   a.rho = s->rho;
   a.diag = s->diag;
+  a.integ = s->integ;
   a.dom = s->dom;
   a.relax = s->cfg.relaxation;
   a.tol = s->cfg.slip_tolerance;
@@ -1326,6 +1344,16 @@ int ibm_apply(const FieldsView& v, IbmState* s, lbm_ibm_diag* out) {
   AUXCHK(v, hipMemcpyAsync(h.data(), s->diag, s->K * sizeof(IbmDevDiag), hipMemcpyDeviceToHost, v.stream));
   AUXCHK(v, hipStreamSynchronize(v.stream));
   for (int k = 0; k < s->K; ++k) diag_to_abi(h[k], out + k);
+  return LBM_OK;
+}
+
+int ibm_torque_integral(const FieldsView& v, IbmState* s, int reset, double* out) {
+  AUXCHK(v, hipMemcpyAsync(out, s->integ, 4 * (size_t)s->K * sizeof(double), hipMemcpyDeviceToHost, v.stream));
+  AUXCHK(v, hipStreamSynchronize(v.stream));
+  if (reset) {
+    hipLaunchKernelGGL(k_zero_integ, dim3(1), dim3(64), 0, v.stream, s->integ, 4 * s->K);
+    AUXCHK(v, hipGetLastError());
+  }
   return LBM_OK;
 }
 

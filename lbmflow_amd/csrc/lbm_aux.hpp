@@ -64,6 +64,9 @@ int ibm_apply(const FieldsView& v, IbmState* s, lbm_ibm_diag* diag);
 int ibm_enqueue(const FieldsView& v, IbmState* s, void* ring, uint32_t cap, uint32_t* pos,
                 const uint32_t* clear_list = nullptr, uint32_t clear_n = 0);
 size_t ibm_ring_entry_bytes();
+// Running f64 torque integral per instance: out[4k + c] = sum of torque c
+// over the updates since the last reset, out[4k + 3] = their count.
+int ibm_torque_integral(const FieldsView& v, IbmState* s, int reset, double* out);
 // Adjoint support (single body).  ibm_enqueue_recorded is ibm_enqueue that
 // also keeps every sweep's impulses and du for ibm_adjoint, which runs the
 // sweeps backwards: Fbar (+ Gbar, nullable) is the adjoint of the force

@@ -455,6 +455,15 @@ class Solver:
         self._chk(self.lib.lbm_run_ibm_span(self.h, steps, int(bool(graph)), _dp(tq), C.byref(d)))
         return tq, (_diag_dict(d) if steps > 0 else None)
 
+    def torque_integral(self, reset: bool = False):
+        """Running FP64 sum of the body's torque kept on the device since the
+        body was registered or last reset (lbm_ibm_torque_integral): returns
+        (sum[3], updates).  The mean shaft torque of a window is
+        sum / updates (runner.rs:1514-1530)."""
+        out = np.zeros(4)
+        self._chk(self.lib.lbm_ibm_torque_integral(self.h, int(bool(reset)), _dp(out)))
+        return out[:3].copy(), int(out[3])
+
     def adjoint_torque(self, steps: int, window: int, snapshots: int = 16, gradients: bool = True):
         """Mean shaft torque J over the last ``window`` of ``steps`` runner
         steps from the current state and, with ``gradients``, its discrete
@@ -632,6 +641,13 @@ class Batch:
         tq = np.zeros((len(self.solvers), max(steps, 0), 3))
         self.ctx.check(self.lib.lbm_batch_run_span(self.h, steps, int(bool(graph)), _dp(tq)))
         return tq
+
+    def torque_integrals(self, reset: bool = False):
+        """The members' running FP64 torque sums over the updates stepped in
+        this batch (lbm_batch_torque_integral): (sums[members, 3], updates[members])."""
+        out = np.zeros((len(self.solvers), 4))
+        self.ctx.check(self.lib.lbm_batch_torque_integral(self.h, int(bool(reset)), _dp(out)))
+        return out[:, :3].copy(), out[:, 3].astype(np.int64)
 
     def close(self):
         if getattr(self, "h", None):

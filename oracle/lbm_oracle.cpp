@@ -16,6 +16,7 @@
 //   patches (the reference's impinging-jet and full-face acceptance tests,
 //   exact per-rect rules); force-probe momentum balance.
 #include <xmmintrin.h>
+#include <omp.h>
 
 #include <algorithm>
 #include <array>
@@ -2022,6 +2023,10 @@ int orc_write_checkpoint(void* p, const char* path) {
   DISPATCH(h, r = S->write_checkpoint(path));
   return r;
 }
+
+// threads an OpenMP region of the oracle runs on (OMP_NUM_THREADS, else
+// the cores the process may use)
+int orc_max_threads() { return omp_get_max_threads(); }
 
 void orc_flush_subnormals(int on) {
   const unsigned bits = 0x8040u;  // FTZ (bit 15) | DAZ (bit 6)

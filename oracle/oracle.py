@@ -49,6 +49,7 @@ def lib():
         L.orc_new.argtypes = [C.c_int, C.c_int, i, i, C.c_int, C.c_double, C.c_double, C.c_double, d, u8, d]
         L.orc_free.argtypes = [vp]
         L.orc_flush_subnormals.argtypes = [C.c_int]
+        L.orc_max_threads.restype = C.c_int
         L.orc_set_cm_factorised.argtypes = [C.c_int]
         L.orc_write_checkpoint.argtypes = [vp, C.c_char_p]
         L.orc_step.argtypes = [vp, C.c_int]
@@ -114,6 +115,11 @@ def set_cm_factorised(on=True):
     """D3Q27 central moments through the factorised 1-D line transforms
     (default) or the reference's generic matrix path (kernels.rs:227-392)."""
     lib().orc_set_cm_factorised(int(bool(on)))
+
+
+def max_threads():
+    """Threads of the oracle's OpenMP regions (omp_get_max_threads)."""
+    return int(lib().orc_max_threads())
 
 
 def flush_subnormals(on=True):

@@ -123,3 +123,32 @@ def test_domega_matches_f64_oracle_difference(tank):
     ad = res["dJ_domega"]
     print(f"dJ/dOmega adjoint {ad:.8e}  f64 oracle FD {fd64:.8e}")
     assert abs(ad - fd64) <= 1e-3 * abs(fd64)
+
+
+def test_autograd_function_carries_the_adjoint(tank):
+    """TankTorque (the torch.autograd.Function over lbm_adjoint_torque): J
+    and the gradients that backward hands to torch equal the adjoint's own,
+    through the chain rule of a loss built on J."""
+    import torch
+
+    from lbmflow_amd.autograd import TankTorque
+
+    tc, nu, res = tank
+    omega = torch.tensor(tc["omega"], dtype=torch.float64, requires_grad=True)
+    nu_t = torch.tensor(nu.reshape(N, N, N), dtype=torch.float64, requires_grad=True)
+
+    def make(w):
+        return _solver(tc, w, nu), STEPS, WINDOW, 3
+
+    J = TankTorque.apply(omega, nu_t, make)
+    assert J.item() == res["J"]
+    loss = 2.0 * J + J * J
+    loss.backward()
+    scale = 2.0 + 2.0 * res["J"]
+    assert omega.grad.item() == pytest.approx(scale * res["dJ_domega"], rel=1e-15)
+    assert nu_t.grad.shape == nu_t.shape
+    np.testing.assert_allclose(nu_t.grad.numpy().reshape(-1), scale * res["dJ_dnu"], rtol=1e-15, atol=0)
+    # no gradient requested: the forward runs without the reverse sweep
+    with torch.no_grad():
+        J2 = TankTorque.apply(omega.detach(), nu_t.detach(), make)
+    assert J2.item() == pytest.approx(res["J"], rel=1e-12)

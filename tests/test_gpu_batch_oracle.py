@@ -6,8 +6,8 @@ together in one persistent launch with the autotuned grid and the
 graph-captured step, and each must match the oracle running that tank alone
 as the reference's sweep does (crates/lbm-cli/src/sweep.rs:61-67): flags and
 bounce-back links bit-exact, populations bit-exact (TRT keeps the
-reference's operand order), per-step torque within 1e-10 relative, and the
-power number of the last-half window (runner.rs:1514-1546) within 0.5%."""
+reference's operand order), per-step torque and the device's running torque
+integral within 1e-10 relative, and the power number of the last-half window (runner.rs:1514-1546) within 0.5%."""
 import numpy as np
 import pytest
 
@@ -55,6 +55,8 @@ def test_sweep_members_match_the_oracle():
     steps = 20
     tq = batch.run(steps, graph=True)
     assert tq.shape == (len(picked), steps, 3)
+    tsum, tcount = batch.torque_integrals()
+    assert (tcount == steps).all()
     for k, ((n, tip, nu), s) in enumerate(zip(picked, members)):
         print(f"[batch-oracle] oracle tank {n}^3", flush=True)
         orc, solid, pos, w, c, omega, D = oracle_member(n, tip, nu)
@@ -71,6 +73,9 @@ def test_sweep_members_match_the_oracle():
         t_o = np.array(t_o)
         np.testing.assert_allclose(tq[k], t_o, rtol=1e-10, atol=1e-12 * np.abs(t_o).max(),
                                    err_msg=f"member {k}: torque")
+        # the device's running FP64 torque integral against the oracle's sum
+        np.testing.assert_allclose(tsum[k], t_o.sum(axis=0), rtol=1e-10, atol=1e-12 * np.abs(t_o).sum(),
+                                   err_msg=f"member {k}: torque integral")
         fl = solid == 0
         a = s.populations()[:, fl].astype(np.float64)
         b = orc.get_f()[:, fl]

@@ -75,3 +75,32 @@ def test_fused_and_wide_ibm_paths_agree_in_a_batch():
     assert np.array_equal(runs[0][0], runs[1][0])
     for pa, pb in zip(runs[0][1], runs[1][1]):
         assert np.array_equal(pa, pb)
+
+
+def _sequential_sum(tq):
+    acc = np.zeros(3)
+    for row in tq:
+        acc = acc + row  # one f64 add per update, in step order, as on the device
+    return acc
+
+
+@pytest.mark.parametrize("path", [None, "wide"])
+def test_device_torque_integral_is_the_sum_of_the_recorded_torques(path):
+    """The running FP64 torque integral kept on the device equals, bit for
+    bit, the in-order sum of the per-step torques the spans record, on both
+    IBM paths; reset clears it after the read, and the single update
+    (lbm_ibm_apply) advances it too."""
+    s = _tank(24, 0.04, 0.01, path)
+    t1, _ = s.run_with_ibm(5, graph=False)
+    t2, _ = s.run_with_ibm(6, graph=True)
+    tsum, n = s.torque_integral(reset=True)
+    assert n == 11
+    assert np.array_equal(tsum, _sequential_sum(np.concatenate([t1, t2])))
+    zero, n0 = s.torque_integral()
+    assert n0 == 0 and not zero.any()
+    t3, _ = s.run_with_ibm(4, graph=True)
+    s.clear_body_force_field()
+    d = s.apply_rotating_ibm()
+    tsum, n = s.torque_integral()
+    assert n == 5
+    assert np.array_equal(tsum, _sequential_sum(np.concatenate([t3, np.array([d["torque"]])])))
