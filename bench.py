@@ -142,9 +142,13 @@ def timed_span(s, steps):
     return dt, k_ms / max(k_n, 1), tq, diag
 
 
-def np_error_vs_f64(storages, collision, Q):
-    """Np of the 128^3 tank (last-half mean torque, runner.rs:1514-1530)
-    against the FP64 oracle series stored in tests/golden/."""
+def qoi_error_vs_f64(storages, collision, Q):
+    """The 128^3 tank's QOIs with each population storage against the FP64
+    oracle series in tests/golden/ (Re_imp 100, D3Q7 tracer with Sc = 1, the
+    fixture's full step count): Np from the last-half mean shaft torque
+    (runner.rs:1514-1530, qoi.rs:477-505), here the device's running FP64
+    torque integral over that window, and the 95 % mixing time from the
+    tracer CV sampled every cv_every steps (qoi.rs:534-556)."""
     import lbmflow_amd as L
 
     path = os.path.join(ROOT, "tests", "golden", "tank128_cm_f64.npz")
@@ -152,22 +156,43 @@ def np_error_vs_f64(storages, collision, Q):
         return None
     z = np.load(path)
     meta = json.loads(bytes(z["meta"]).decode())
-    steps = int(meta["steps"])
-    cfg = dict(tank_config(meta["n"]), nu=meta["nu"])
+    steps, every, n = int(meta["steps"]), int(meta["cv_every"]), int(meta["n"])
+    cfg = dict(tank_config(n), nu=meta["nu"])
     rec = np.arange(1, steps + 1)
-
-    def np_of(torque_z):
-        tq = float(np.mean(torque_z[rec >= steps // 2]))
-        return L.power_qois(tq, cfg["omega"], 1.0, cfg["D"], 1.0)["np"]
-
-    ref = np_of(np.asarray(z["torque"]))
-    out = {"oracle": "f64 CPU restatement, 128^3, %d steps" % steps, "np_f64": ref}
+    t_cv = every * np.arange(1, steps // every + 1, dtype=float)
+    tq_ref = np.asarray(z["torque"])
+    np_ref = L.power_qois(float(np.mean(tq_ref[rec >= steps // 2])), cfg["omega"], 1.0, cfg["D"], 1.0)["np"]
+    t95_ref = L.mixing_time_from_cv(t_cv, np.asarray(z["cv"]))["t95"]
+    out = {"oracle": f"f64 CPU restatement, {n}^3, Re_imp {meta['re_imp']:g}, {steps} steps, D3Q7 tracer",
+           "np_f64": np_ref, "t95_f64": t95_ref}
+    w0 = steps // 2 - 1  # the torque window is steps steps//2 .. steps
     for st in storages:
         s, _ = build_gpu(cfg, st, collision, Q)
-        tq, _ = s.run_with_ibm(steps, graph=False)
+        solid = s.solid().reshape(n, n, n)
+        zz, yy, xx = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+        sig, cz = float(meta["sigma"]), 0.8 * cfg["H"]
+        blob = np.exp(-((xx + 0.5 - n / 2) ** 2 + (yy + 0.5 - n / 2) ** 2 + (zz + 0.5 - cz) ** 2) / (2 * sig ** 2))
+        blob = np.where(solid == 0, blob, 0.0).astype(np.float32).reshape(-1)
+        k = s.enable_scalar_distribution(cfg["nu"], "d3q7")
+        s.set_scalar_concentration(k, blob)
+        cv = []
+        done = 0
+        while done < steps:
+            step = min(every - done % every, w0 - done if done < w0 else every)
+            s.run_with_ibm(step, graph=True)
+            done += step
+            if done == w0:
+                s.torque_integral(reset=True)
+            if done % every == 0:
+                cv.append(s.scalar_cv(k))
+        tsum, cnt = s.torque_integral()
         s.close()
-        v = np_of(-tq[:, 2])
-        out[st] = {"np": v, "rel_err": abs(v - ref) / abs(ref)}
+        v = L.power_qois(float(-tsum[2] / cnt), cfg["omega"], 1.0, cfg["D"], 1.0)["np"]
+        t95 = L.mixing_time_from_cv(t_cv, np.array(cv))["t95"]
+        out[st] = {"np": v, "np_rel_err": abs(v - np_ref) / abs(np_ref), "t95": t95,
+                   "t95_rel_err": None if t95 is None or t95_ref is None else abs(t95 - t95_ref) / t95_ref}
+        print(f"[bench] {st} vs f64: Np rel {out[st]['np_rel_err']:.2e}, t95 {t95} vs {t95_ref}",
+              file=sys.stderr, flush=True)
     return out
 
 
@@ -591,9 +616,9 @@ def main():
             sweep[st] = {"mlups_fluid": round(fluid * 100 / dt2 / 1e6, 2), "kernel_avg_ms": round(k2, 4),
                          "speedup_vs_" + args.storage: round(dt / args.steps / (dt2 / 100), 3)}
             print(f"[bench] {st}: {1e3 * dt2 / 100:.3f} ms/step, k_hydro {k2:.3f} ms", file=sys.stderr, flush=True)
-        nperr = np_error_vs_f64(("f32", "f16", "bf16"), args.collision, Q)
-        if nperr is not None:
-            sweep["np_vs_f64_oracle"] = nperr
+        qerr = qoi_error_vs_f64(("f32", "f16", "bf16"), args.collision, Q)
+        if qerr is not None:
+            sweep["qoi_vs_f64_oracle"] = qerr
 
     cpu = None
     if f_dev is not None:

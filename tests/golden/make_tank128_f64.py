@@ -16,7 +16,7 @@ reaches the 95 % mixing criterion after 6.7 revolutions, inside the
 reference's band of 3 to 7 (vb_02_passive_scalar_mixing.rs:7-14), so the
 series carries both Np and t95.
 
-usage: python tests/golden/make_tank128_f64.py [steps]   (about 0.25 s/step on 8 cores;
+usage: python tests/golden/make_tank128_f64.py [steps]   (about 0.3 s/step on 8 cores, 88 min;
 the committed series: 17,600 steps = 7 impeller revolutions, the upper end
 of the reference's mixing-time band, vb_02_passive_scalar_mixing.rs:7-14)
 """
