@@ -576,6 +576,9 @@ def main():
           f"force-list={n_list}  device={s.device_bytes() / 2**30:.2f} GiB", file=sys.stderr, flush=True)
 
     s.run_with_ibm(args.warmup, graph=False)
+    # tile visit order per parity, timed on a scratch copy of the state
+    tune = s.autotune_tiles(10)
+    print(f"[bench] autotuned tile order: {tune}", file=sys.stderr, flush=True)
     dt, k_avg_ms, tq, diag = timed_span(s, args.steps)
     # the same span replayed from the captured two-step HIP graph
     s.run_with_ibm(2, graph=True)  # capture (not timed)
@@ -610,12 +613,17 @@ def main():
         sweep = {}
         for st in [v for v in ("f32", "f16", "bf16") if v != args.storage]:
             s2, _ = build_gpu(cfg, st, args.collision, Q)
-            s2.run_with_ibm(200, graph=False)
-            dt2, k2, _, _ = timed_span(s2, 100)
+            s2.run_with_ibm(args.warmup, graph=False)
+            tune2 = s2.autotune_tiles(10)
+            dt2, k2, _, _ = timed_span(s2, args.steps)
             s2.close()
-            sweep[st] = {"mlups_fluid": round(fluid * 100 / dt2 / 1e6, 2), "kernel_avg_ms": round(k2, 4),
-                         "speedup_vs_" + args.storage: round(dt / args.steps / (dt2 / 100), 3)}
-            print(f"[bench] {st}: {1e3 * dt2 / 100:.3f} ms/step, k_hydro {k2:.3f} ms", file=sys.stderr, flush=True)
+            sweep[st] = {"mlups_fluid": round(fluid * args.steps / dt2 / 1e6, 2), "kernel_avg_ms": round(k2, 4),
+                         "speedup_vs_" + args.storage: round(dt / dt2, 3),
+                         "kernel_speedup_vs_" + args.storage: round(k_avg_ms / k2, 3),
+                         "autotuned_tile_order": {k: (round(v, 4) if isinstance(v, float) else v)
+                                                  for k, v in tune2.items()}}
+            print(f"[bench] {st}: {1e3 * dt2 / args.steps:.3f} ms/step, k_hydro {k2:.3f} ms, tiles {tune2}",
+                  file=sys.stderr, flush=True)
         qerr = qoi_error_vs_f64(("f32", "f16", "bf16"), args.collision, Q)
         if qerr is not None:
             sweep["qoi_vs_f64_oracle"] = qerr
@@ -647,7 +655,8 @@ def main():
                    "lattice": f"D3Q{Q}", "collision": args.collision, "storage": args.storage,
                    "grid": [args.size] * 3, "tank_T_lu": cfg["T"], "impeller_D_lu": cfg["D"],
                    "tip_speed_lu": 0.05, "re_imp": 5000, "nu_lu": cfg["nu"], "markers": n_markers,
-                   "fluid_cells": int(fluid), "parallelism": "single GPU"},
+                   "fluid_cells": int(fluid), "parallelism": "single GPU",
+                   "autotuned_tile_order": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in tune.items()}},
         "mlups_total": round(mlups_total, 2),
         "mlups_fluid": round(mlups_fluid, 2),
         "graph_span": {"ms_per_step": round(1e3 * dt_graph / args.steps, 4),

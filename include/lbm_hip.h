@@ -364,6 +364,16 @@ int lbm_ibm_apply(lbm_fields* fl, lbm_ibm_diag* diag);
  * eager launches.  The reference's equivalent seam is Backend::run_chunk_size
  * / finish_run_chunk (backend.rs:340-349). */
 int lbm_run_ibm_span(lbm_fields* fl, int steps, int use_graph, double* torque, lbm_ibm_diag* last);
+/* Autotuner of the collide-stream launch on one domain: the visit order of
+ * the active tiles (slabs of zb planes, y-rows within a slab, then the
+ * planes) for each parity.  Times `trial_steps` (2..64, whole even/odd
+ * pairs) main-kernel launches for each of six candidate depths per parity on
+ * a scratch copy of the populations (HIP events on the library stream),
+ * keeps the fastest for each parity and rebuilds the tile lists; the state
+ * is untouched.  zb_out = {even, odd}; ms_out (nullable) their mean launch
+ * times.  No reference counterpart: the reference's GPU prototype fixes its
+ * workgroup shapes (backend.rs:126-363 exposes no tuning). */
+int lbm_autotune_tiles(lbm_fields* fl, int trial_steps, uint32_t zb_out[2], double ms_out[2]);
 /* Running FP64 torque integral of the registered body, kept on the device
  * and advanced by every rotating-IBM update (lbm_ibm_apply, spans, batches):
  * out[0..2] = sum of IbmDiagnostics::torque over the updates since the body

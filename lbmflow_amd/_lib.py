@@ -125,6 +125,7 @@ SIGNATURES = {
     "lbm_ibm_apply": (_I, [_P, C.POINTER(IbmDiag)]),
     "lbm_run_ibm_span": (_I, [_P, _I, _I, _pD, C.POINTER(IbmDiag)]),
     "lbm_ibm_torque_integral": (_I, [_P, _I, _pD]),
+    "lbm_autotune_tiles": (_I, [_P, _I, _pU32, _pD]),
     "lbm_batch_create": (_I, [_P, C.POINTER(_P), _I, C.POINTER(_P)]),
     "lbm_batch_destroy": (None, [_P]),
     "lbm_batch_info": (_I, [_P, _pI, _pU32, _pU32, _pU32]),

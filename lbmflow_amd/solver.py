@@ -455,6 +455,17 @@ class Solver:
         self._chk(self.lib.lbm_run_ibm_span(self.h, steps, int(bool(graph)), _dp(tq), C.byref(d)))
         return tq, (_diag_dict(d) if steps > 0 else None)
 
+    def autotune_tiles(self, trial_steps: int = 10):
+        """Pick the tile visit order of the collide-stream launch per parity
+        by timing candidates on a scratch copy of the state
+        (lbm_autotune_tiles).  Returns {"zb_even", "zb_odd", "ms_even",
+        "ms_odd"}."""
+        zb = np.zeros(2, np.uint32)
+        ms = np.zeros(2)
+        self._chk(self.lib.lbm_autotune_tiles(self.h, int(trial_steps),
+                                              zb.ctypes.data_as(C.POINTER(C.c_uint32)), _dp(ms)))
+        return {"zb_even": int(zb[0]), "zb_odd": int(zb[1]), "ms_even": float(ms[0]), "ms_odd": float(ms[1])}
+
     def torque_integral(self, reset: bool = False):
         """Running FP64 sum of the body's torque kept on the device since the
         body was registered or last reset (lbm_ibm_torque_integral): returns
