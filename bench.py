@@ -222,6 +222,35 @@ def small_tank(n, tip, nu, Q=19, collision="trt"):
     return s, {"n": n, "T": T, "D": D, "omega": omega, "tip": tip, "nu": nu, "markers": len(w)}
 
 
+def workload_calib(args):
+    """Counter calibration, not a throughput benchmark: one GiB copied with
+    each lane access form of the collide-stream kernels (16-byte; 4-byte
+    aligned and shifted one element, as the odd step's x-neighbour rows;
+    2-byte aligned and shifted), `steps` launches each, then the 4 GiB
+    16-byte stream copy.  Under rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE the
+    per-dispatch counts divided by the bytes below give each form's factor."""
+    import lbmflow_amd as L
+
+    ctx = L.default_context()
+    nbytes = 1 << 30
+    forms = [(16, 0), (4, 0), (4, 1), (2, 0), (2, 1)]
+    out = {}
+    for w, o in forms:
+        out[f"w{w}_off{o}"] = round(ctx.access_copy_gbps(w, o, nbytes, args.steps), 1)
+        print(f"[bench] access copy width {w} offset {o}: {out[f'w{w}_off{o}']} GB/s", file=sys.stderr, flush=True)
+    out["stream_copy_4GiB"] = round(ctx.stream_copy_gbps(4 << 30, args.steps), 1)
+    return {"metric": "copy GB/s per lane access form (counter calibration)", "value": out["w16_off0"],
+            "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": 1, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "u8", "data": "synthetic (zeros)",
+            "config": {"workload": "access-form copies, 1 GiB read + 1 GiB written per launch",
+                       "bytes_read_per_launch": nbytes, "bytes_written_per_launch": nbytes,
+                       "kernels": {"k_access_copy<uint4, 0>": "w16_off0", "k_access_copy<unsigned int, 0>": "w4_off0",
+                                   "k_access_copy<unsigned int, 1>": "w4_off1",
+                                   "k_access_copy<unsigned short, 0>": "w2_off0",
+                                   "k_access_copy<unsigned short, 1>": "w2_off1"}},
+            "gbps": out}
+
+
 def workload_cavity(args):
     """The plain-method configuration: 3D lid-driven cavity, D3Q19 TRT
     (magic 3/16), 64^3, fp32, Re 100, lid 0.05 lu, quiescent start, 1000
@@ -541,10 +570,11 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-sweep", action="store_true", help="skip the 16-bit storage and Np-accuracy sweep")
     ap.add_argument("--workload", default="tank", choices=["tank", "batch", "adjoint", "les", "channel", "surface",
-                                                              "cavity"],
+                                                              "cavity", "calib"],
                     help="tank: the 512^3 benchmark (default); batch: 64-tank sweep; adjoint: torque gradient; "
                          "les: the tank with per-step WALE; channel: open-face channel; surface: the 256^3 tank "
-                         "with surface-sampled markers, 20k steps, Np; cavity: the 64^3 lid-driven cavity")
+                         "with surface-sampled markers, 20k steps, Np; cavity: the 64^3 lid-driven cavity; "
+                         "calib: copies per lane access form for counter calibration")
     ap.add_argument("--seed", type=int, default=20260512)
     ap.add_argument("--snapshots", type=int, default=128, help="adjoint: binomial checkpoint snapshots")
     args = ap.parse_args()
@@ -554,14 +584,14 @@ def main():
         sys.exit(2)
     if args.steps is None:
         args.steps = {"tank": 200, "batch": 5000, "adjoint": 4000, "les": 100, "channel": 200,
-                      "surface": 20000, "cavity": 1000}[args.workload]
+                      "surface": 20000, "cavity": 1000, "calib": 5}[args.workload]
     if args.warmup is None:
         args.warmup = {"tank": 500, "batch": 100, "adjoint": 0, "les": 200, "channel": 100, "surface": 0,
-                       "cavity": 0}[args.workload]
+                       "cavity": 0, "calib": 1}[args.workload]
     if args.workload != "tank":
         fn = {"batch": workload_batch, "adjoint": workload_adjoint, "les": workload_les,
               "channel": workload_channel, "surface": workload_surface,
-              "cavity": workload_cavity}[args.workload]
+              "cavity": workload_cavity, "calib": workload_calib}[args.workload]
         print(json.dumps(fn(args)), flush=True)
         return
 

@@ -122,6 +122,14 @@ const char* lbm_build_info(void);
  * two device buffers, `iterations` times; *gbps counts read + write bytes.
  * The measured HBM ceiling that the collide-stream roofline is read against. */
 int lbm_stream_copy(lbm_ctx* ctx, size_t bytes, int iterations, double* gbps);
+/* Diagnostic with no reference counterpart: copy of `bytes` with one
+ * element of `width` bytes per lane access (16; 4; 2), the element run
+ * starting `offset_elems` elements (0, or 1 for widths 4 and 2) past
+ * 256-byte-aligned bases: the access forms of the collide-stream kernels
+ * (4-byte planes, their x-shifted neighbour rows, 2-byte planes), against
+ * which the HBM traffic counters are calibrated.  *gbps counts read + write
+ * bytes. */
+int lbm_access_copy(lbm_ctx* ctx, int width, int offset_elems, size_t bytes, int iterations, double* gbps);
 
 /* ---- fields ------------------------------------------------------------- */
 /* Backend::alloc, backend.rs:138-139: quiescent fields (f = 0, rho = 1,

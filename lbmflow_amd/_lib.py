@@ -82,6 +82,7 @@ SIGNATURES = {
     "lbm_capabilities": (_I, [_P, _pU32]),
     "lbm_build_info": (C.c_char_p, []),
     "lbm_stream_copy": (_I, [_P, C.c_size_t, _I, _pD]),
+    "lbm_access_copy": (_I, [_P, _I, _I, C.c_size_t, _I, _pD]),
     "lbm_fields_alloc": (_I, [_P, _I, _I, _pI, _pI, C.POINTER(_P)]),
     "lbm_fields_free": (None, [_P]),
     "lbm_fields_device_bytes": (C.c_size_t, [_P]),

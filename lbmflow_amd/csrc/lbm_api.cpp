@@ -545,6 +545,40 @@ int lbm_stream_copy(lbm_ctx* ctx, size_t bytes, int iterations, double* gbps) {
   return LBM_OK;
 }
 
+int lbm_access_copy(lbm_ctx* ctx, int width, int offset_elems, size_t bytes, int iterations, double* gbps) {
+  if (!ctx || !gbps || iterations <= 0 || bytes < 256) return LBM_ERR_INVALID;
+  if (!((width == 16 && offset_elems == 0) || ((width == 4 || width == 2) && (offset_elems == 0 || offset_elems == 1))))
+    return fail(ctx, LBM_ERR_INVALID, "access copy: width %d offset %d", width, offset_elems);
+  const size_t n = bytes / (size_t)width;
+  const size_t alloc = n * (size_t)width + 256;  // room for the offset element
+  void *a = nullptr, *b = nullptr;
+  if (hipMalloc(&a, alloc) != hipSuccess) return fail(ctx, LBM_ERR_NOMEM, "access copy: hipMalloc failed");
+  if (hipMalloc(&b, alloc) != hipSuccess) {
+    (void)hipFree(a);
+    return fail(ctx, LBM_ERR_NOMEM, "access copy: hipMalloc failed");
+  }
+  hipStream_t s = ctx->stream;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = LBM_OK;
+  float ms = 0.0f;
+  if (hipMemsetAsync(a, 0, alloc, s) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+      hipEventCreate(&e1) != hipSuccess || launch_access_copy(width, offset_elems, a, b, n, s) != hipSuccess ||
+      hipEventRecord(e0, s) != hipSuccess)
+    rc = LBM_ERR_DEVICE;
+  for (int k = 0; rc == LBM_OK && k < iterations; ++k)
+    if (launch_access_copy(width, offset_elems, a, b, n, s) != hipSuccess) rc = LBM_ERR_DEVICE;
+  if (rc == LBM_OK && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                       hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+    rc = LBM_ERR_DEVICE;
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  if (rc) return fail(ctx, rc, "access copy failed");
+  *gbps = 2.0 * (double)(n * (size_t)width) * iterations / (ms * 1e-3) / 1e9;
+  return LBM_OK;
+}
+
 int lbm_capabilities(const lbm_ctx* ctx, uint32_t* flags) {
   if (!ctx || !flags) return LBM_ERR_INVALID;
   *flags = LBM_CAP_TRT | LBM_CAP_CENTRAL_MOMENT | LBM_CAP_F16_STORAGE | LBM_CAP_BF16_STORAGE | LBM_CAP_FORCE_FIELD |

@@ -189,6 +189,9 @@ hipError_t launch_rims(const Dom& d, const int is_wall[6], const double u[18], u
                        double* best, hipStream_t s);
 hipError_t launch_fill_f32(float* p, size_t n, float v, hipStream_t s);
 hipError_t launch_stream_copy(const void* a, void* b, size_t bytes, hipStream_t s);
+// n elements of `width` bytes (16, 4 or 2) from a + offset_elems to
+// b + offset_elems (offset 0, or 1 for widths 4 and 2)
+hipError_t launch_access_copy(int width, int offset_elems, const void* a, void* b, size_t n, hipStream_t s);
 hipError_t launch_copy_force_list(const uint32_t* flist, uint32_t n, float* F_dst, const float* F_src,
                                   hipStream_t s);
 

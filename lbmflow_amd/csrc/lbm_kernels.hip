@@ -1547,6 +1547,35 @@ hipError_t launch_rims(const Dom& d, const int is_wall[6], const double u[18], u
   return hipSuccess;
 }
 
+// Copy n elements of T, each lane one naturally aligned element per
+// iteration, starting OFF elements past 256-byte-aligned bases: the access
+// widths and line offsets of the collide-stream kernels (4-byte planes, and
+// their x-shifted neighbour rows that put a wave across three lines; 2-byte
+// planes of 16-bit storage) next to the 16-byte copy, for calibrating the
+// memory counters per access form.
+template <typename T, int OFF>
+__global__ __launch_bounds__(256) void k_access_copy(const T* __restrict__ a, T* __restrict__ b, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
+    b[OFF + i] = a[OFF + i];
+}
+
+hipError_t launch_access_copy(int width, int offset_elems, const void* a, void* b, size_t n, hipStream_t s) {
+  const uint32_t grid = (uint32_t)std::min<size_t>((n + 255) / 256, 256 * 8 * 64);
+  if (width == 16 && offset_elems == 0)
+    hipLaunchKernelGGL((k_access_copy<uint4, 0>), dim3(grid), dim3(256), 0, s, (const uint4*)a, (uint4*)b, n);
+  else if (width == 4 && offset_elems == 0)
+    hipLaunchKernelGGL((k_access_copy<uint32_t, 0>), dim3(grid), dim3(256), 0, s, (const uint32_t*)a, (uint32_t*)b, n);
+  else if (width == 4 && offset_elems == 1)
+    hipLaunchKernelGGL((k_access_copy<uint32_t, 1>), dim3(grid), dim3(256), 0, s, (const uint32_t*)a, (uint32_t*)b, n);
+  else if (width == 2 && offset_elems == 0)
+    hipLaunchKernelGGL((k_access_copy<uint16_t, 0>), dim3(grid), dim3(256), 0, s, (const uint16_t*)a, (uint16_t*)b, n);
+  else if (width == 2 && offset_elems == 1)
+    hipLaunchKernelGGL((k_access_copy<uint16_t, 1>), dim3(grid), dim3(256), 0, s, (const uint16_t*)a, (uint16_t*)b, n);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
 hipError_t launch_stream_copy(const void* a, void* b, size_t bytes, hipStream_t s) {
   const size_t n = bytes / 16;
   const uint32_t grid = (uint32_t)std::min<size_t>((n + 255) / 256, 256 * 8 * 64);

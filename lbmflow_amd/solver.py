@@ -78,6 +78,14 @@ class Context:
         self.check(self.lib.lbm_stream_copy(self.h, int(nbytes), int(iterations), C.byref(v)))
         return v.value
 
+    def access_copy_gbps(self, width: int, offset_elems: int = 0, nbytes: int = 1 << 30, iterations: int = 5) -> float:
+        """Copy with `width`-byte lane accesses (16, 4, 2) starting
+        `offset_elems` elements past aligned bases (lbm_access_copy)."""
+        v = C.c_double()
+        self.check(self.lib.lbm_access_copy(self.h, int(width), int(offset_elems), int(nbytes), int(iterations),
+                                            C.byref(v)))
+        return v.value
+
     def close(self):
         if getattr(self, "h", None):
             self.lib.lbm_ctx_destroy(self.h)
