@@ -2077,28 +2077,45 @@ void orc_update_moments(void* p) {
 }
 
 // Populations, q-major compact core planes (deviation form).
-void orc_get_f(void* p, double* out) {
+// Population exchange in compact order f[q][cell] (f64 or f32 on the host
+// side), one OpenMP thread per (q, z) plane row block.
+extern "C++" {
+template <typename H>
+static void get_f_any(void* p, H* out) {
   Handle* h = (Handle*)p;
   DISPATCH(h, {
-    for (int q = 0; q < S->L.Q; ++q)
-      for (long z = 0; z < S->nz; ++z)
+    const long Q = S->L.Q, nz = S->nz;
+    _Pragma("omp parallel for collapse(2) schedule(static)")
+    for (long q = 0; q < Q; ++q)
+      for (long z = 0; z < nz; ++z)
         for (long y = 0; y < S->ny; ++y)
           for (long x = 0; x < S->nx; ++x)
-            out[q * S->nc + S->cidx(x, y, z)] = (double)S->f[q * S->np + S->pidx(x, y, z)];
+            out[q * S->nc + S->cidx(x, y, z)] = (H)S->f[q * S->np + S->pidx(x, y, z)];
   });
 }
 
-void orc_set_f(void* p, const double* in) {
+template <typename H>
+static void set_f_any(void* p, const H* in) {
   Handle* h = (Handle*)p;
   DISPATCH(h, {
     using TT = std::remove_reference_t<decltype(S->f[0])>;
-    for (int q = 0; q < S->L.Q; ++q)
-      for (long z = 0; z < S->nz; ++z)
+    const long Q = S->L.Q, nz = S->nz;
+    _Pragma("omp parallel for collapse(2) schedule(static)")
+    for (long q = 0; q < Q; ++q)
+      for (long z = 0; z < nz; ++z)
         for (long y = 0; y < S->ny; ++y)
           for (long x = 0; x < S->nx; ++x)
             S->f[q * S->np + S->pidx(x, y, z)] = (TT)in[q * S->nc + S->cidx(x, y, z)];
   });
 }
+}  // extern "C++"
+
+void orc_get_f(void* p, double* out) { get_f_any(p, out); }
+void orc_set_f(void* p, const double* in) { set_f_any(p, in); }
+// f32 host arrays: an f32 oracle exchanges its populations without the f64
+// copy (512^3 D3Q27: 14.5 GB instead of 29 GB)
+void orc_get_f32(void* p, float* out) { get_f_any(p, out); }
+void orc_set_f32(void* p, const float* in) { set_f_any(p, in); }
 
 void orc_get_moments(void* p, double* rho, double* ux, double* uy, double* uz) {
   Handle* h = (Handle*)p;

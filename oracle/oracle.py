@@ -57,6 +57,8 @@ def lib():
         L.orc_update_moments.argtypes = [vp]
         L.orc_get_f.argtypes = [vp, d]
         L.orc_set_f.argtypes = [vp, d]
+        L.orc_get_f32.argtypes = [vp, C.POINTER(C.c_float)]
+        L.orc_set_f32.argtypes = [vp, C.POINTER(C.c_float)]
         L.orc_get_moments.argtypes = [vp, d, d, d, d]
         L.orc_set_moments.argtypes = [vp, d, d, d, d]
         L.orc_reduce.restype = C.c_double
@@ -184,12 +186,22 @@ class OracleSolver:
     def update_moments(self):
         lib().orc_update_moments(self.h)
 
-    def get_f(self):
+    def get_f(self, dtype=np.float64):
+        """Populations f[q][cell]; dtype float32 skips the f64 copy (an f32
+        oracle holds f32 values, so nothing is lost)."""
+        if np.dtype(dtype) == np.float32:
+            out = np.empty((self.Q, self.n), np.float32)
+            lib().orc_get_f32(self.h, out.ctypes.data_as(C.POINTER(C.c_float)))
+            return out
         out = np.zeros((self.Q, self.n))
         lib().orc_get_f(self.h, _d(out))
         return out
 
     def set_f(self, f):
+        if isinstance(f, np.ndarray) and f.dtype == np.float32:
+            f = np.ascontiguousarray(f)
+            lib().orc_set_f32(self.h, f.ctypes.data_as(C.POINTER(C.c_float)))
+            return
         f = np.ascontiguousarray(f, np.float64)
         lib().orc_set_f(self.h, _d(f))
 

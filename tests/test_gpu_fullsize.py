@@ -1,0 +1,83 @@
+"""The headline configuration at its full size against the oracle: the
+512^3 D3Q27 central-moment stirred tank with the Rushton IBM (the benchmark
+workload), fp32.  The GPU runs 300 steps from rest; its populations and
+force field are handed to the oracle, and both advance two runner steps
+(clear force, rotating IBM, collide-stream: one odd and one even step).
+Every fluid population must agree within the central-moment gates (1e-6 on
+the stored deviations and 1e-6 of the largest population), the shaft
+torque within 1e-6 relative (all three components within 1e-6 of its
+magnitude), and the total mass stays within 1e-6 relative of the
+cell count over the GPU's run.  Host memory: about 50 GB at the peak."""
+import sys
+
+import numpy as np
+import pytest
+
+import bench
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(f"[fullsize] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def test_tank512_cm_two_steps_from_a_developed_state():
+    import lbmflow_amd as L
+
+    n = 512
+    cfg = bench.tank_config(n)
+    _say("GPU 512^3 tank, 300 steps")
+    s, _ = bench.build_gpu(cfg, "f32", "cm", 27)
+    fluid = s.fluid_cells()
+    s.run_with_ibm(300, graph=True)
+    mass = s.total_mass_f64()
+    assert abs(mass - fluid) <= 1e-6 * fluid, f"mass {mass} vs {fluid} cells"
+    solid_gpu = s.solid()
+    f = s.populations()
+    F = s.body_force_field()
+    assert np.isfinite(f).all()
+    _say("oracle from the GPU state")
+    dims = (n,) * 3
+    solid = O.tank_mask(dims, 1.0, cfg["T"], cfg["H"])
+    solid, _ = O.baffle_mask(dims, 1.0, cfg["T"], cfg["H"], cfg["baffle_width"], cfg["baffle_thickness"], solid)
+    assert np.array_equal(solid.reshape(-1), solid_gpu), "tank flags"
+    pos, w, vel, c = O.radial_impeller(dims, 1.0, cfg["T"], cfg["H"], cfg["D"], cfg["clearance"], 6, 0.0,
+                                       cfg["omega"])
+    orc = O.OracleSolver(27, dims, prec=4, collision=O.CM, nu=cfg["nu"], solid=solid)
+    orc.set_f(f)
+    del f
+    if F is not None:
+        orc.set_force_field(F.astype(np.float64).reshape(-1))
+    del F
+    orc.update_moments()
+    t_orc = []
+    for k in range(2):
+        _say(f"oracle step {k + 1}/2")
+        orc.clear_force_field()
+        d = orc.apply_ibm(pos, w, c, (0.0, 0.0, cfg["omega"]))
+        orc.step()
+        t_orc.append(d["torque"])
+    tq, _ = s.run_with_ibm(2, graph=False)
+    # the shaft (z) component within 1e-6 relative; the x and y components
+    # are near zero by symmetry, so all three within 1e-6 of the magnitude
+    t_orc = np.array(t_orc)
+    np.testing.assert_allclose(tq[:, 2], t_orc[:, 2], rtol=1e-6)
+    np.testing.assert_allclose(tq, t_orc, rtol=0, atol=1e-6 * np.abs(t_orc).max())
+    _say("compare populations")
+    g = s.populations()
+    s.close()
+    og = orc.get_f(np.float32)
+    del orc
+    fl = solid.reshape(-1) == 0
+    _, wq, _, _ = O.lattice(27)
+    worst, big = 0.0, 0.0
+    for q in range(27):
+        a = g[q, fl].astype(np.float64)
+        b = og[q, fl].astype(np.float64)
+        worst = max(worst, float(np.abs(a - b).max()))
+        big = max(big, float(np.abs(wq[q] + b).max()))
+    _say(f"max |d| {worst:.3e}, largest population {big:.4f}")
+    assert worst <= 1e-6 and worst <= 1e-6 * big
