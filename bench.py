@@ -374,16 +374,27 @@ def workload_batch(args):
           file=sys.stderr, flush=True)
     batch.run(args.warmup, graph=True)
     solvers[0].synchronize()
+    # two spans: the second is the QOI window, over which the device's FP64
+    # torque integrals run (reset between the spans)
+    h = args.steps // 2
     t0 = time.perf_counter()
-    tq = batch.run(args.steps, graph=True)
+    tq1 = batch.run(h, graph=True)
+    batch.torque_integrals(reset=True)
+    tq2 = batch.run(args.steps - h, graph=True)
     solvers[0].synchronize()
     dt = time.perf_counter() - t0
-    nps = []
+    tsum, tcnt = batch.torque_integrals()
+    tq = np.concatenate([tq1, tq2], axis=1)
+    nps, dev = [], 0.0
     for k, m in enumerate(meta):
         tz = -tq[k, :, 2]
-        tmean = float(np.mean(tz[len(tz) // 2:]))
+        t_host = float(np.mean(tz[h:]))
+        tmean = float(-tsum[k, 2] / tcnt[k])
+        dev = max(dev, abs(tmean - t_host) / max(abs(t_host), 1e-300))
         nps.append(L.power_qois(tmean, m["omega"], 1.0, m["D"], 1.0)["np"])
     batch.close()
+    B = 2 * 19 * 4
+    achieved = fluid * B / (trial_ms * 1e-3) / 1e9
     return {
         "metric": "aggregate MLUPS, 64-tank batched sweep (D3Q19 TRT fp32, one persistent launch)",
         "value": round(fluid * args.steps / dt / 1e6, 2), "unit": "MLUPS", "n_gpus": 1, "steps": args.steps,
@@ -394,6 +405,13 @@ def workload_batch(args):
                    "cells": int(total), "fluid_cells": int(fluid), "persistent_grid": grid,
                    "autotune_trial_ms": round(trial_ms, 4), "parallelism": "single GPU"},
         "mlups_total": round(total * args.steps / dt / 1e6, 2),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_hydro_batch (persistent collide-stream over all tanks)",
+                     "kernel_avg_ms": round(trial_ms, 4), "bytes_per_cell": B,
+                     "algorithmic_bytes_per_launch": int(fluid * B),
+                     "timing": "mean launch time of the autotuner's best grid (HIP events, 10 launches)"},
+        "checks": {"np_window_steps": args.steps - h, "device_vs_host_torque_mean_rel": dev},
         "per_instance": [{"n": m["n"], "tip": round(m["tip"], 5), "nu": round(m["nu"], 6), "np": round(v, 5)}
                          for m, v in zip(meta, nps)],
     }

@@ -66,6 +66,7 @@ struct lbm_fields {
   std::vector<uint32_t> tiles_host;  // the even-step visit order
   uint32_t* tiles_odd = nullptr;     // the same tiles in the odd-step visit order
   uint32_t zb_set[2] = {0, 0};       // visit-order slab depth per parity (0: default), set by the autotuner
+  bool tiles_linear = false;         // tiles are 64-cell blocks of the linear index
   uint32_t* flist = nullptr;
   size_t flist_cap = 0;
   // reduction scratch
@@ -267,16 +268,23 @@ int grow_force_list(lbm_fields* fl, const std::vector<uint32_t>& extra) {
   return rebuild_tiles(fl);
 }
 
-// Rebuild the list of 64-cell tiles the main kernel must visit.
+// Rebuild the list of 64-cell tiles the main kernel must visit.  Row tiles
+// are x-segments of one row; when no axis wraps and the row length is not
+// a multiple of 64, tiles are 64-cell blocks of the linear index instead
+// (rows packed end to end: on the 64-tank sweep, edges 48 to 160, 14 %
+// fewer tiles for the same cells), which is exact because without wrapping
+// every neighbour shift is the linear one.
 int rebuild_tiles(lbm_fields* fl) {
   const uint32_t nseg = (fl->d.nx + 63) / 64;
-  const size_t nt = (size_t)nseg * fl->d.ny * fl->d.nz;
+  const bool linear = fl->d.nx % 64 != 0 && !fl->d.per[0] && !fl->d.per[1] && !fl->d.per[2] && !fl->open_face_cells &&
+                      !(getenv("LBM_ROW_TILES") && getenv("LBM_ROW_TILES")[0] == '1');
+  const size_t nt = linear ? ((size_t)fl->d.N + 63) / 64 : (size_t)nseg * fl->d.ny * fl->d.nz;
   uint8_t* dflags = nullptr;
   int rc = dev_alloc(fl, (void**)&dflags, nt);
   if (rc) return rc;
   hipStream_t s = fl->ctx->stream;
   std::vector<uint8_t> flags(nt);
-  HIPCHK(fl->ctx, launch_tile_flags(fl->d, fl->mask, dflags, s));
+  HIPCHK(fl->ctx, launch_tile_flags(fl->d, fl->mask, dflags, linear ? 1 : 0, s));
   HIPCHK(fl->ctx, hipMemcpyAsync(flags.data(), dflags, nt, hipMemcpyDeviceToHost, s));
   HIPCHK(fl->ctx, hipStreamSynchronize(s));
   dev_free(fl, dflags, nt);
@@ -295,15 +303,19 @@ int rebuild_tiles(lbm_fields* fl) {
   }
   if (fl->zb_set[0]) zb_even = fl->zb_set[0];
   if (fl->zb_set[1]) zb_odd = fl->zb_set[1];
-  const uint32_t ny = fl->d.ny, nz = fl->d.nz;
+  const uint32_t ny = fl->d.ny, nz = fl->d.nz, nx = fl->d.nx;
   auto build = [&](uint32_t zb) {
     std::vector<uint32_t> list;
     list.reserve(nt);
     for (uint32_t z0 = 0; z0 < nz; z0 += zb)
       for (uint32_t y = 0; y < ny; ++y)
-        for (uint32_t z = z0; z < std::min(nz, z0 + zb); ++z)
-          for (uint32_t sg = 0; sg < nseg; ++sg) {
-            const size_t t = ((size_t)z * ny + y) * nseg + sg;
+        for (uint32_t z = z0; z < std::min(nz, z0 + zb); ++z) {
+          // the row's tiles: its x-segments, or the linear tiles whose
+          // first cell lies in the row
+          const size_t r0 = ((size_t)z * ny + y) * nx;
+          const size_t ta = linear ? (r0 + 63) / 64 : ((size_t)z * ny + y) * nseg;
+          const size_t tb = linear ? (r0 + nx + 63) / 64 : ta + nseg;
+          for (size_t t = ta; t < tb; ++t) {
             const uint8_t fg = flags[t];
             if (!(fg & 1)) continue;
             uint32_t e = (uint32_t)t | ((fg & 2) ? kTilePlain : 0u);
@@ -314,6 +326,7 @@ int rebuild_tiles(lbm_fields* fl) {
             if (fl->storage == kF32) e |= ((fg & 4) ? kTileDeadLo : 0u) | ((fg & 8) ? kTileDeadHi : 0u);
             list.push_back(e);
           }
+        }
     return list;
   };
   std::vector<uint32_t> list = build(zb_even);
@@ -334,6 +347,7 @@ int rebuild_tiles(lbm_fields* fl) {
   }
   fl->ntiles = (uint32_t)list.size();
   fl->tiles_host.swap(list);
+  fl->tiles_linear = linear;
   fl->gen++;
   return LBM_OK;
 }
@@ -447,6 +461,7 @@ HydroArgs hydro_args(lbm_fields* fl) {
   a.F_mom = fl->F_mom;
   a.F_cur = fl->F_cur;
   a.open = fl->open_face_cells ? 1 : 0;
+  a.lin = fl->tiles_linear ? 1 : 0;
   return a;
 }
 

@@ -65,6 +65,7 @@ struct HydroArgs {
   float* F_mom;             // N x 3: force of the cached moments
   const float* F_cur;       // N x 3: current field
   int open;                 // fluid on a non-periodic (open) face: wrap every offset
+  int lin;                  // tiles are 64-cell blocks of the linear index (no axis wraps)
 };
 
 // Open faces (lbm_open.hip): FaceBC kinds of params.rs:232-252, faces in
@@ -169,7 +170,7 @@ hipError_t launch_wale(int Q, int storage, int odd, const Dom& d, const void* f,
 hipError_t launch_build_mask(int Q, const Dom& d, const uint8_t* solid, const float* wall_u, uint32_t* mask,
                              unsigned int* err_count, hipStream_t s);
 hipError_t launch_mark_forced(const uint32_t* flist, uint32_t n, uint32_t* mask, hipStream_t s);
-hipError_t launch_tile_flags(const Dom& d, const uint32_t* mask, uint8_t* flags, hipStream_t s);
+hipError_t launch_tile_flags(const Dom& d, const uint32_t* mask, uint8_t* flags, int linear, hipStream_t s);
 hipError_t launch_stage(int Q, int storage, int dir, int odd, const Dom& d, void* f, float* host_layout,
                         const uint32_t* mask, hipStream_t s);
 hipError_t launch_init_with(int Q, int storage, const Dom& d, void* f, const uint32_t* mask, const float* rho,

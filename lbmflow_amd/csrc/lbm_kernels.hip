@@ -650,7 +650,8 @@ __device__ __forceinline__ void slot_st(S* p, float v) {
 template <int Q, typename S, int KIND, bool ODD, bool FON, bool WRAP, bool FILL, bool PLAIN = false>
 __device__ __forceinline__ void cell_step(const HydroArgs& a, uint32_t i, uint32_t x, uint32_t y, uint32_t z,
                                           const uint32_t* mask_ptr, uint32_t skip, const float Fm[3],
-                                          const float Fc[3], uint32_t xlo = 0u, uint32_t xhi = 0u) {
+                                          const float Fc[3], uint32_t xlo = 0u, uint32_t xhi = 0u,
+                                          uint32_t xr = 0xFFFFFFFFu) {
   constexpr bool fon = FON;
   const Slots<S, WRAP> sl(a, i, x, y, z);
   // The mask word and the Q populations are loaded together: the loads do
@@ -739,7 +740,11 @@ __device__ __forceinline__ void cell_step(const HydroArgs& a, uint32_t i, uint32
       constexpr int q = decltype(qc)::value;
       constexpr int o = lat_opp(q);
       if ((m >> o) & 1u) {
-        const uint32_t fwd = i + (uint32_t)sl.nb.template off<lat_c(q, 0), lat_c(q, 1), lat_c(q, 2)>();
+        // (no wrapping axis: a fluid cell's neighbours are inside the domain,
+        // so the linear offset is the wrapped one)
+        const uint32_t fwd =
+            WRAP ? i + (uint32_t)sl.nb.template off<lat_c(q, 0), lat_c(q, 1), lat_c(q, 2)>()
+                 : i + (uint32_t)(lat_c(q, 0) + lat_c(q, 1) * (int32_t)a.d.nx + lat_c(q, 2) * (int32_t)(a.d.nx * a.d.ny));
         const float* wu = a.wall_u + 3 * (size_t)fwd;
         const float cu = cdot<o>(wu[0], wu[1], wu[2]);
         constexpr float six_w = 6.0f * lat_wf(Q, o);
@@ -780,7 +785,10 @@ __device__ __forceinline__ void cell_step(const HydroArgs& a, uint32_t i, uint32
       constexpr int q = decltype(qc)::value;
       constexpr int o = lat_opp(q);
       constexpr int cx = lat_c(q, 0);
-      const bool inr = cx == 0 || ((uint32_t)(x + cx) >= xlo && (uint32_t)(x + cx) <= xhi);
+      // xr: the coordinate the live range is in (x for row tiles, the cell
+      // index for linear tiles)
+      const uint32_t xc = xr == 0xFFFFFFFFu ? x : xr;
+      const bool inr = cx == 0 || ((uint32_t)(xc + cx) >= xlo && (uint32_t)(xc + cx) <= xhi);
       const bool link = !dead && ((m >> o) & 1u);
       const bool plain = dead ? (!((m >> q) & 1u) && inr) : !link;
       if (link) slot_st<S, Q, q>(sl.template at<o, 0, 0, 0>(), fv[q]);
@@ -799,6 +807,22 @@ __global__ __launch_bounds__(256) void k_hydro(HydroArgs a) {
   if (t >= a.ntiles) return;
   const uint32_t te = __builtin_amdgcn_readfirstlane(a.tiles[t]);  // one tile per wave
   const uint32_t tile = te & kTileId;
+  if (!WRAP && a.lin) {
+    // linear tile: cells 64 tile .. 64 tile + 63 across row ends (no axis
+    // wraps, so every neighbour shift is the linear one)
+    const uint32_t i = tile * 64 + (threadIdx.x & 63);
+    if (i >= a.d.N || (te & ((threadIdx.x & 32) ? kTileDeadHi : kTileDeadLo))) return;
+    if (te & kTilePlain) {
+      cell_step<Q, S, KIND, ODD, FON, WRAP, true, true>(a, i, 0u, 0u, 0u, nullptr, 0u, a.p.F, a.p.F);
+    } else {
+      const uint32_t c0 = tile * 64;
+      const uint32_t lo = (te & kTileDeadLo) ? c0 + 32 : c0;
+      const uint32_t hi = min((te & kTileDeadHi) ? c0 + 31 : c0 + 63, a.d.N - 1);
+      cell_step<Q, S, KIND, ODD, FON, WRAP, true, false>(a, i, 0u, 0u, 0u, a.mask + i, kMaskSolid | kMaskForced,
+                                                         a.p.F, a.p.F, lo, hi, i);
+    }
+    return;
+  }
   const uint32_t row = tile / a.nseg, seg = tile - row * a.nseg;
   const uint32_t x = seg * 64 + (threadIdx.x & 63);
   if (x >= a.d.nx || (te & ((threadIdx.x & 32) ? kTileDeadHi : kTileDeadLo))) return;
@@ -861,6 +885,21 @@ __global__ __launch_bounds__(256) void k_hydro_batch(const HydroArgs* __restrict
     const uint32_t k = __builtin_amdgcn_readfirstlane(ent[2 * (size_t)e + 1]);
     const HydroArgs& a = tab[k];
     const uint32_t tile = te & kTileId;
+    if (a.lin) {  // linear tile (members never wrap)
+      const uint32_t i = tile * 64 + (threadIdx.x & 63);
+      if (i < a.d.N && !(te & ((threadIdx.x & 32) ? kTileDeadHi : kTileDeadLo))) {
+        if (te & kTilePlain) {
+          cell_step<Q, S, KIND, ODD, false, false, true, true>(a, i, 0u, 0u, 0u, nullptr, 0u, a.p.F, a.p.F);
+        } else {
+          const uint32_t c0 = tile * 64;
+          const uint32_t lo = (te & kTileDeadLo) ? c0 + 32 : c0;
+          const uint32_t hi = min((te & kTileDeadHi) ? c0 + 31 : c0 + 63, a.d.N - 1);
+          cell_step<Q, S, KIND, ODD, false, false, true, false>(a, i, 0u, 0u, 0u, a.mask + i,
+                                                                kMaskSolid | kMaskForced, a.p.F, a.p.F, lo, hi, i);
+        }
+      }
+      continue;
+    }
     const uint32_t row = tile / a.nseg, seg = tile - row * a.nseg;
     const uint32_t x = seg * 64 + (threadIdx.x & 63);
     if (x < a.d.nx && !(te & ((threadIdx.x & 32) ? kTileDeadHi : kTileDeadLo))) {
@@ -1184,18 +1223,28 @@ __global__ void k_build_mask(Dom d, const uint8_t* solid, const float* wall_u, u
 // flags per 64-cell x-segment: bit 0 some cell for the main kernel, bit 1
 // all 64 cells plain fluid (mask word 0), bits 2/3 lanes 0-31 / 32-63 all
 // solid (or past the row end)
-__global__ void k_tile_flags(Dom d, const uint32_t* mask, uint8_t* flags) {
+// Linear tiles (`linear`): tile t is cells 64t .. 64t + 63 of the linear
+// index, across row ends, with the same four flags.
+__global__ void k_tile_flags(Dom d, const uint32_t* mask, uint8_t* flags, int linear) {
   const uint32_t nseg = (d.nx + 63) / 64;
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nseg * d.ny * d.nz) return;
-  const uint32_t row = t / nseg, seg = t - row * nseg;
-  const uint32_t x0 = seg * 64, x1 = min(x0 + 64, d.nx);
-  bool any = false, plain = x1 - x0 == 64, dead[2] = {true, true};
-  for (uint32_t x = x0; x < x1; ++x) {
-    const uint32_t m = mask[row * d.nx + x];
+  uint32_t c0, c1;
+  if (linear) {
+    if (t >= (d.N + 63) / 64) return;
+    c0 = t * 64;
+    c1 = min(c0 + 64, d.N);
+  } else {
+    if (t >= nseg * d.ny * d.nz) return;
+    const uint32_t row = t / nseg, seg = t - row * nseg;
+    c0 = row * d.nx + seg * 64;
+    c1 = row * d.nx + min(seg * 64 + 64, d.nx);
+  }
+  bool any = false, plain = c1 - c0 == 64, dead[2] = {true, true};
+  for (uint32_t c = c0; c < c1; ++c) {
+    const uint32_t m = mask[c];
     if (!(m & (kMaskSolid | kMaskForced))) any = true;
     if (m != 0u) plain = false;
-    if (!(m & kMaskSolid)) dead[(x - x0) >> 5] = false;
+    if (!(m & kMaskSolid)) dead[(c - c0) >> 5] = false;
   }
   flags[t] = (uint8_t)((any ? 1 : 0) | (plain ? 2 : 0) | (dead[0] ? 4 : 0) | (dead[1] ? 8 : 0));
 }
@@ -1408,9 +1457,9 @@ hipError_t launch_build_mask(int Q, const Dom& d, const uint8_t* solid, const fl
   return hipGetLastError();
 }
 
-hipError_t launch_tile_flags(const Dom& d, const uint32_t* mask, uint8_t* flags, hipStream_t s) {
-  const size_t nt = (size_t)((d.nx + 63) / 64) * d.ny * d.nz;
-  hipLaunchKernelGGL(k_tile_flags, dim3(blocks_for(nt)), dim3(256), 0, s, d, mask, flags);
+hipError_t launch_tile_flags(const Dom& d, const uint32_t* mask, uint8_t* flags, int linear, hipStream_t s) {
+  const size_t nt = linear ? (size_t)(d.N + 63) / 64 : (size_t)((d.nx + 63) / 64) * d.ny * d.nz;
+  hipLaunchKernelGGL(k_tile_flags, dim3(blocks_for(nt)), dim3(256), 0, s, d, mask, flags, linear);
   return hipGetLastError();
 }
 
