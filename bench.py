@@ -625,7 +625,7 @@ def main():
 
     s.run_with_ibm(args.warmup, graph=False)
     # tile visit order per parity, timed on a scratch copy of the state
-    tune = s.autotune_tiles(10)
+    tune = s.autotune_tiles(20)
     print(f"[bench] autotuned tile order: {tune}", file=sys.stderr, flush=True)
     dt, k_avg_ms, tq, diag = timed_span(s, args.steps)
     # the same span replayed from the captured two-step HIP graph
@@ -662,7 +662,7 @@ def main():
         for st in [v for v in ("f32", "f16", "bf16") if v != args.storage]:
             s2, _ = build_gpu(cfg, st, args.collision, Q)
             s2.run_with_ibm(args.warmup, graph=False)
-            tune2 = s2.autotune_tiles(10)
+            tune2 = s2.autotune_tiles(20)
             dt2, k2, _, _ = timed_span(s2, args.steps)
             s2.close()
             sweep[st] = {"mlups_fluid": round(fluid * args.steps / dt2 / 1e6, 2), "kernel_avg_ms": round(k2, 4),
