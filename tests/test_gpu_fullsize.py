@@ -25,8 +25,6 @@ def _say(msg):
 
 
 def test_tank512_cm_two_steps_from_a_developed_state():
-    import lbmflow_amd as L
-
     n = 512
     cfg = bench.tank_config(n)
     _say("GPU 512^3 tank, 300 steps")
