@@ -449,6 +449,7 @@ __device__ __forceinline__ void marker_sums(const IbmView& V, int64_t m, const d
 __global__ __launch_bounds__(1024) void k_ibm_sweeps(IbmSweepArgs a) {
   extern __shared__ double dyn[];
   __shared__ double sh[9][256];
+  __shared__ int hand[256];  // lanes 256-511 hand their marker's presence to lanes 0-255
   __shared__ int conv;
   const int k = blockIdx.x;
   const int tid = threadIdx.x, nth = blockDim.x;
@@ -581,42 +582,79 @@ __global__ __launch_bounds__(1024) void k_ibm_sweeps(IbmSweepArgs a) {
     // velocity, written over this sweep's impulse of the same marker; the
     // lane accumulates its reduction terms as it goes
     const bool next = it + 1 < a.max_iter;
-    if (tid < 256) {
-      double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // tq[3], rep[3], slipmax, ssq, wsum
-      for (int64_t m = m0 + tid; m < m1; m += 256) {
-        double rep[3] = {0.0, 0.0, 0.0}, um[3] = {0.0, 0.0, 0.0};
-        double im[3];
-        for (int c = 0; c < 3; ++c) im[c] = V.imp[3 * (m - m0) + c];
-        double mob = 0.0;
-        if (next)
-          marker_sums<true>(V, m, im, um, mob, rep);
-        else
-          marker_sums<false>(V, m, im, um, mob, rep);
-        const double* t = V.target + 3 * (m - V.mb);
-        if (next)
-          for (int c = 0; c < 3; ++c) {
-            const double slip = t[c] - um[c];
-            V.imp[3 * (m - m0) + c] = mob == 0.0 ? 0.0 : a.relax * slip / mob;
-          }
-        const double* pm = V.pos + 3 * (m - V.mb);
-        const double r[3] = {pm[0] - a.cen[3 * k], pm[1] - a.cen[3 * k + 1], pm[2] - a.cen[3 * k + 2]};
-        const double tq[3] = {r[1] * rep[2] - r[2] * rep[1], r[2] * rep[0] - r[0] * rep[2],
-                              r[0] * rep[1] - r[1] * rep[0]};
+    // One marker's terms: torque and represented force, post-sweep slip
+    // (and, when a sweep follows, its next impulse from the same
+    // interpolated velocity, written over this sweep's impulse).
+    auto marker_terms = [&](int64_t m, double* tm, bool& haspts) {
+      double rep[3] = {0.0, 0.0, 0.0}, um[3] = {0.0, 0.0, 0.0};
+      double im[3];
+      for (int c = 0; c < 3; ++c) im[c] = V.imp[3 * (m - m0) + c];
+      double mob = 0.0;
+      if (next)
+        marker_sums<true>(V, m, im, um, mob, rep);
+      else
+        marker_sums<false>(V, m, im, um, mob, rep);
+      const double* t = V.target + 3 * (m - V.mb);
+      if (next)
         for (int c = 0; c < 3; ++c) {
-          acc[c] += tq[c];
-          acc[3 + c] += rep[c];
+          const double slip = t[c] - um[c];
+          V.imp[3 * (m - m0) + c] = mob == 0.0 ? 0.0 : a.relax * slip / mob;
         }
-        const double sl[3] = {t[0] - um[0], t[1] - um[1], t[2] - um[2]};
-        const double slip = sqrt(0.0 + sl[0] * sl[0] + sl[1] * sl[1] + sl[2] * sl[2]);
-        if (V.mk_off[m - V.mb + 1] > V.mk_off[m - V.mb]) {
-          const double w = V.weight[m - V.mb];
-          acc[6] = fmax(acc[6], slip);
-          acc[7] += w * slip * slip;
-          acc[8] += w;
+      const double* pm = V.pos + 3 * (m - V.mb);
+      const double r[3] = {pm[0] - a.cen[3 * k], pm[1] - a.cen[3 * k + 1], pm[2] - a.cen[3 * k + 2]};
+      tm[0] = r[1] * rep[2] - r[2] * rep[1];
+      tm[1] = r[2] * rep[0] - r[0] * rep[2];
+      tm[2] = r[0] * rep[1] - r[1] * rep[0];
+      for (int c = 0; c < 3; ++c) tm[3 + c] = rep[c];
+      const double sl[3] = {t[0] - um[0], t[1] - um[1], t[2] - um[2]};
+      const double slip = sqrt(0.0 + sl[0] * sl[0] + sl[1] * sl[1] + sl[2] * sl[2]);
+      haspts = V.mk_off[m - V.mb + 1] > V.mk_off[m - V.mb];
+      const double w = haspts ? V.weight[m - V.mb] : 0.0;
+      tm[6] = slip;
+      tm[7] = w * slip * slip;
+      tm[8] = w;
+    };
+    auto accumulate = [&](double* acc, const double* tm, bool haspts) {
+      for (int c = 0; c < 6; ++c) acc[c] += tm[c];
+      if (haspts) {
+        acc[6] = fmax(acc[6], tm[6]);
+        acc[7] += tm[7];
+        acc[8] += tm[8];
+      }
+    };
+    // The 256 accumulators of the reduction (lane t: markers t, t + 256,
+    // t + 512, ... in that order), with the markers computed on 512 lanes:
+    // per round, lanes 0-255 take chunk 2r (markers 512r + t) and lanes
+    // 256-511 chunk 2r + 1, whose terms lane t adds after its own through
+    // LDS; the sums are those of the one-group loop.
+    double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // tq[3], rep[3], slipmax, ssq, wsum
+    for (uint32_t r0 = 0; r0 < nm; r0 += 512) {
+      double tm[9];
+      bool haspts = false, have = false;
+      if (tid < 512) {
+        const uint32_t l = r0 + (uint32_t)tid;
+        if (l < nm) {
+          have = true;
+          marker_terms(m0 + (int64_t)l, tm, haspts);
         }
       }
-      for (int j = 0; j < 9; ++j) sh[j][tid] = acc[j];
+      if (tid >= 256 && tid < 512) {
+        for (int j = 0; j < 9; ++j) sh[j][tid - 256] = have ? tm[j] : 0.0;
+        hand[tid - 256] = (have ? 1 : 0) | (haspts ? 2 : 0);
+      }
+      __syncthreads();
+      if (tid < 256) {
+        if (have) accumulate(acc, tm, haspts);
+        if (hand[tid] & 1) {
+          double o[9];
+          for (int j = 0; j < 9; ++j) o[j] = sh[j][tid];
+          accumulate(acc, o, (hand[tid] & 2) != 0);
+        }
+      }
+      __syncthreads();
     }
+    if (tid < 256)
+      for (int j = 0; j < 9; ++j) sh[j][tid] = acc[j];
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
       if (tid < w)
@@ -803,7 +841,7 @@ static IbmState* ibm_build(const std::vector<const IbmHost*>& parts, const lbm_i
     const int force_level = path && strcmp(path, "fused1") == 0 ? 1 : 0;
     if (!wide && s->mmax > 0 && hipGetDevice(&dev) == hipSuccess &&
         hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess) {
-      const size_t fixed = 9 * 256 * sizeof(double) + 64;
+      const size_t fixed = 9 * 256 * sizeof(double) + 256 * sizeof(int) + 64;  // the kernel's static LDS
       for (int level = force_level ? force_level : 2; level >= 1 && !s->lds_bytes; --level) {
         const size_t need = ibm_lds_bytes(level, s->mmax, s->umax, s->pmax);
         if (need + fixed <= (size_t)lim) {
