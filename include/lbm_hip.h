@@ -378,8 +378,10 @@ int lbm_run_ibm_span(lbm_fields* fl, int steps, int use_graph, double* torque, l
  * pairs) main-kernel launches for each of six candidate depths per parity on
  * a scratch copy of the populations (HIP events on the library stream),
  * keeps the fastest for each parity and rebuilds the tile lists; the state
- * is untouched.  zb_out = {even, odd}; ms_out (nullable) their mean launch
- * times.  No reference counterpart: the reference's GPU prototype fixes its
+ * is untouched (the scratch copy needs device memory for a second
+ * population buffer; LBM_ERR_NOMEM if it cannot be had, with the tile
+ * lists unchanged).  zb_out = {even, odd}; ms_out (nullable) their mean
+ * launch times.  No reference counterpart: the reference's GPU prototype fixes its
  * workgroup shapes (backend.rs:126-363 exposes no tuning). */
 int lbm_autotune_tiles(lbm_fields* fl, int trial_steps, uint32_t zb_out[2], double ms_out[2]);
 /* Running FP64 torque integral of the registered body, kept on the device
