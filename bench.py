@@ -222,6 +222,76 @@ def small_tank(n, tip, nu, Q=19, collision="trt"):
     return s, {"n": n, "T": T, "D": D, "omega": omega, "tip": tip, "nu": nu, "markers": len(w)}
 
 
+def workload_precision512(args):
+    """The precision configuration at its full size: the 512^3 tank (D3Q27
+    central moments, Re_imp 5000, per-step WALE: without a turbulence model
+    this tank diverges after about 3,150 steps) with --storage population
+    storage and a D3Q7 tracer pulse (Gaussian sigma 8 lu at 0.8 H on the
+    axis, Sc = 1), run from rest until the 95 % mixing time (CV sampled every
+    100 steps, qoi.rs:534-556) or --steps; Np from the last-half mean shaft
+    torque (runner.rs:1514-1530).  The errors against the FP64 oracle are
+    measured at 128^3 (qoi_vs_f64_oracle of the default workload); this run
+    gives the storages' Np and t95 at 512^3 side by side."""
+    import lbmflow_amd as L
+
+    n = 512
+    cfg = tank_config(n)
+    s, n_markers = build_gpu(cfg, args.storage, "cm", 27)
+    s.set_wale(True)
+    fluid = s.fluid_cells()
+    solid = s.solid().reshape(n, n, n)
+    ax = np.arange(n, dtype=np.float64) + 0.5 - n / 2
+    zc = np.arange(n, dtype=np.float64) + 0.5 - 0.8 * cfg["H"]
+    sig = 8.0
+    blob = np.exp(-(ax[None, None, :] ** 2 + ax[None, :, None] ** 2 + zc[:, None, None] ** 2) / (2 * sig * sig))
+    blob = np.where(solid == 0, blob, 0.0).astype(np.float32).reshape(-1)
+    del solid
+    k = s.enable_scalar_distribution(cfg["nu"], "d3q7")
+    s.set_scalar_concentration(k, blob)
+    del blob
+    every, torque, t_cv, cv = 100, [], [], []
+    t95, done = None, 0
+    s.synchronize()
+    t0 = time.perf_counter()
+    while done < args.steps:
+        tq, diag = s.run_with_ibm(every, graph=True)
+        done += every
+        torque.append(-tq[:, 2])
+        c = s.scalar_cv(k)
+        t_cv.append(float(done))
+        cv.append(c)
+        if not np.isfinite(c) or not np.isfinite(tq).all():
+            break
+        if done % 1000 == 0:
+            print(f"[bench] precision512 {args.storage}: {done} steps, CV {c:.4f}", file=sys.stderr, flush=True)
+        mix = L.mixing_time_from_cv(np.array(t_cv), np.array(cv))
+        if mix is not None and mix["t95"] is not None:
+            t95 = mix["t95"]
+            break
+    s.synchronize()
+    dt = time.perf_counter() - t0
+    torque = np.concatenate(torque)
+    finite = bool(np.isfinite(torque).all() and np.isfinite(cv).all())
+    tmean = float(np.mean(torque[len(torque) // 2:]))
+    q = L.power_qois(tmean, cfg["omega"], 1.0, cfg["D"], 1.0)
+    s.close()
+    rev = done * cfg["omega"] / (2 * np.pi)
+    return {
+        "metric": "Np and 95 % mixing time, D3Q27 CM tank 512^3 with WALE and a D3Q7 tracer",
+        "value": round(fluid * done / dt / 1e6, 2), "unit": "MLUPS", "n_gpus": 1, "steps": done, "warmup": 0,
+        "ms_per_step": round(1e3 * dt / done, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32" if args.storage == "f32" else f"f32 math, {args.storage} storage",
+        "data": "synthetic (tank geometry, Rushton lattice markers, Gaussian tracer sigma 8 lu; flow from rest)",
+        "config": {"workload": "stirred tank D3Q27 cm 512^3 + per-step WALE + D3Q7 tracer (Sc 1), until t95",
+                   "storage": args.storage, "grid": [n] * 3, "re_imp": 5000, "nu_lu": cfg["nu"],
+                   "markers": n_markers, "fluid_cells": int(fluid), "cv_every": every,
+                   "parallelism": "single GPU"},
+        "np": q["np"], "t95_steps": t95, "t95_revolutions": None if t95 is None else t95 * cfg["omega"] / (2 * np.pi),
+        "revolutions_run": rev, "cv0": cv[0] if cv else None, "cv_last": cv[-1] if cv else None,
+        "checks": {"finite": finite},
+    }
+
+
 def workload_calib(args):
     """Counter calibration, not a throughput benchmark: one GiB copied with
     each lane access form of the collide-stream kernels (16-byte; 4-byte
@@ -588,11 +658,12 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-sweep", action="store_true", help="skip the 16-bit storage and Np-accuracy sweep")
     ap.add_argument("--workload", default="tank", choices=["tank", "batch", "adjoint", "les", "channel", "surface",
-                                                              "cavity", "calib"],
+                                                              "cavity", "calib", "precision512"],
                     help="tank: the 512^3 benchmark (default); batch: 64-tank sweep; adjoint: torque gradient; "
                          "les: the tank with per-step WALE; channel: open-face channel; surface: the 256^3 tank "
                          "with surface-sampled markers, 20k steps, Np; cavity: the 64^3 lid-driven cavity; "
-                         "calib: copies per lane access form for counter calibration")
+                         "calib: copies per lane access form for counter calibration; precision512: the 512^3 "
+                         "tank with WALE and a tracer until t95, per --storage")
     ap.add_argument("--seed", type=int, default=20260512)
     ap.add_argument("--snapshots", type=int, default=128, help="adjoint: binomial checkpoint snapshots")
     args = ap.parse_args()
@@ -602,14 +673,15 @@ def main():
         sys.exit(2)
     if args.steps is None:
         args.steps = {"tank": 200, "batch": 5000, "adjoint": 4000, "les": 100, "channel": 200,
-                      "surface": 20000, "cavity": 1000, "calib": 5}[args.workload]
+                      "surface": 20000, "cavity": 1000, "calib": 5, "precision512": 90000}[args.workload]
     if args.warmup is None:
         args.warmup = {"tank": 500, "batch": 100, "adjoint": 0, "les": 200, "channel": 100, "surface": 0,
-                       "cavity": 0, "calib": 1}[args.workload]
+                       "cavity": 0, "calib": 1, "precision512": 0}[args.workload]
     if args.workload != "tank":
         fn = {"batch": workload_batch, "adjoint": workload_adjoint, "les": workload_les,
               "channel": workload_channel, "surface": workload_surface,
-              "cavity": workload_cavity, "calib": workload_calib}[args.workload]
+              "cavity": workload_cavity, "calib": workload_calib,
+              "precision512": workload_precision512}[args.workload]
         print(json.dumps(fn(args)), flush=True)
         return
 
