@@ -187,79 +187,142 @@ __global__ void k_ibm_reset(IbmDevDiag* diag, int K) {
   if (k < K) memset(diag + k, 0, sizeof(IbmDevDiag));
 }
 
-// per marker: interpolated velocity, mobility row sum, impulse
-__global__ void k_ibm_impulse(int64_t M, const uint32_t* mk_off, const uint32_t* mk_inst, const uint32_t* pt_u,
-                              const double* pt_w, const double* target, const double* unow, const double* du,
-                              const double* rho, const double* ksum, double relax, double* imp,
-                              const IbmDevDiag* diag) {
-  const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= M || diag[mk_inst[m]].converged) return;
-  double um[3] = {0.0, 0.0, 0.0}, mob = 0.0;
-  for (uint32_t p = mk_off[m]; p < mk_off[m + 1]; ++p) {
-    const uint32_t u = pt_u[p];
-    const double w = pt_w[p];
-    for (int a = 0; a < 3; ++a) um[a] += w * (unow[3 * (size_t)u + a] + du[3 * (size_t)u + a]);
-    mob += w * ksum[u] / fmax(rho[u], 1.0e-30);
-  }
-  for (int a = 0; a < 3; ++a) {
-    const double slip = target[3 * m + a] - um[a];
-    imp[3 * m + a] = mob == 0.0 ? 0.0 : relax * slip / mob;
+// The wide path's per-marker and per-cell sums walk short lists (a marker's
+// stencil points, a cell's marker entries) whose entries point at data
+// written by another kernel, usually in another XCD's L2: one thread per
+// marker made each entry two or three dependent far-cache round trips.  Here
+// one wave takes a marker (or cell): each lane loads one entry, all loads in
+// flight at once, and the sum then runs over the entries in list order in
+// every lane from values read out of the loading lanes (v_readlane with a
+// wave-uniform index), each term in the statement form of the one-thread
+// loop, so the f64 results are the same bits.
+__device__ __forceinline__ double lane_f64(double v, int j) {
+  const uint64_t b = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, j);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), j);
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+// Per marker over its stencil points in point order: the interpolated
+// velocity sum_p w (u_now + du) and, with `mob`, the mobility row sum
+// sum_p w ksum / rho (solver.rs:3107-3208).  One wave per marker.
+__device__ __forceinline__ void marker_point_sums(uint32_t p0, uint32_t p1, const uint32_t* pt_u, const double* pt_w,
+                                                  const double* unow, const double* du, const double* rho,
+                                                  const double* ksum, bool want_mob, double um[3], double* mob,
+                                                  double* w_of_lane) {
+  const int lane = threadIdx.x & 63;
+  for (uint32_t base = p0; base < p1; base += 64) {
+    const uint32_t p = base + lane;
+    double w = 0.0, t[3] = {0.0, 0.0, 0.0}, q = 0.0;
+    if (p < p1) {
+      const uint32_t u = pt_u[p];
+      w = pt_w[p];
+      for (int a = 0; a < 3; ++a) t[a] = unow[3 * (size_t)u + a] + du[3 * (size_t)u + a];
+      if (want_mob) q = w * ksum[u] / fmax(rho[u], 1.0e-30);
+    }
+    if (w_of_lane && base == p0) *w_of_lane = w;
+    const int n = (int)min(64u, p1 - base);
+    for (int j = 0; j < n; ++j) {
+      const double wj = lane_f64(w, j);
+      for (int a = 0; a < 3; ++a) um[a] += wj * lane_f64(t[a], j);
+      if (want_mob) *mob += lane_f64(q, j);
+    }
   }
 }
 
-// per cell: spread and du accumulated over the cell's markers in marker order
-__global__ void k_ibm_spread(uint32_t U, const uint32_t* uc_off, const uint32_t* uc_inst, const uint32_t* uc_ent,
-                             const uint32_t* pt_m, const double* pt_w, const double* imp, const double* rho,
-                             double* spread, double* du, const IbmDevDiag* diag) {
-  const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+// per marker: interpolated velocity, mobility row sum, impulse (one wave
+// per marker)
+__global__ __launch_bounds__(256) void k_ibm_impulse(int64_t M, const uint32_t* mk_off, const uint32_t* mk_inst,
+                                                     const uint32_t* pt_u, const double* pt_w, const double* target,
+                                                     const double* unow, const double* du, const double* rho,
+                                                     const double* ksum, double relax, double* imp,
+                                                     const IbmDevDiag* diag) {
+  const int64_t m = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (m >= M || diag[mk_inst[m]].converged) return;
+  double um[3] = {0.0, 0.0, 0.0}, mob = 0.0;
+  marker_point_sums(mk_off[m], mk_off[m + 1], pt_u, pt_w, unow, du, rho, ksum, true, um, &mob, nullptr);
+  if ((threadIdx.x & 63) == 0)
+    for (int a = 0; a < 3; ++a) {
+      const double slip = target[3 * m + a] - um[a];
+      imp[3 * m + a] = mob == 0.0 ? 0.0 : relax * slip / mob;
+    }
+}
+
+// per cell: spread and du accumulated over the cell's markers in marker
+// order (one wave per cell)
+__global__ __launch_bounds__(256) void k_ibm_spread(uint32_t U, const uint32_t* uc_off, const uint32_t* uc_inst,
+                                                    const uint32_t* uc_ent, const uint32_t* pt_m, const double* pt_w,
+                                                    const double* imp, const double* rho, double* spread, double* du,
+                                                    const IbmDevDiag* diag) {
+  const uint64_t u = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   if (u >= U || diag[uc_inst[u]].converged) return;
+  const int lane = threadIdx.x & 63;
   const double inv_rho = 1.0 / fmax(rho[u], 1.0e-30);
   double s[3], dd[3];
   for (int a = 0; a < 3; ++a) {
     s[a] = spread[3 * (size_t)u + a];
     dd[a] = du[3 * (size_t)u + a];
   }
-  for (uint32_t e = uc_off[u]; e < uc_off[u + 1]; ++e) {
-    const uint32_t p = uc_ent[e];
-    const uint32_t m = pt_m[p];
-    const double w = pt_w[p];
-    for (int a = 0; a < 3; ++a) {
-      const double cf = imp[3 * (size_t)m + a] * w;
-      s[a] += cf;
-      dd[a] += cf * inv_rho;
+  const uint32_t e0 = uc_off[u], e1 = uc_off[u + 1];
+  for (uint32_t base = e0; base < e1; base += 64) {
+    const uint32_t e = base + lane;
+    double w = 0.0, im[3] = {0.0, 0.0, 0.0};
+    if (e < e1) {
+      const uint32_t p = uc_ent[e];
+      const uint32_t m = pt_m[p];
+      w = pt_w[p];
+      for (int a = 0; a < 3; ++a) im[a] = imp[3 * (size_t)m + a];
+    }
+    const int n = (int)min(64u, e1 - base);
+    for (int j = 0; j < n; ++j) {
+      const double wj = lane_f64(w, j);
+      for (int a = 0; a < 3; ++a) {
+        const double cf = lane_f64(im[a], j) * wj;
+        s[a] += cf;
+        dd[a] += cf * inv_rho;
+      }
     }
   }
-  for (int a = 0; a < 3; ++a) {
-    spread[3 * (size_t)u + a] = s[a];
-    du[3 * (size_t)u + a] = dd[a];
-  }
+  if (lane == 0)
+    for (int a = 0; a < 3; ++a) {
+      spread[3 * (size_t)u + a] = s[a];
+      du[3 * (size_t)u + a] = dd[a];
+    }
 }
 
 // per marker: represented force, torque contribution, post-sweep slip; with
 // imp_next, also the next sweep's impulse from the same interpolated
 // velocity (what k_ibm_impulse of the next sweep computes, operation for
-// operation), which k_ibm_reduce installs if the instance goes on
-__global__ void k_ibm_marker_diag(int64_t M, const uint32_t* mk_off, const uint32_t* mk_inst, const uint32_t* pt_u,
-                                  const double* pt_w, const double* pos, const double* target, const double* unow,
-                                  const double* du, const double* imp, const double* cen, double* mrep, double* mtq,
-                                  double* mslip, uint8_t* mact, const IbmDevDiag* diag, const double* rho,
-                                  const double* ksum, double relax, double* imp_next) {
-  const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// operation), which k_ibm_reduce installs if the instance goes on (one wave
+// per marker)
+__global__ __launch_bounds__(256) void k_ibm_marker_diag(int64_t M, const uint32_t* mk_off, const uint32_t* mk_inst,
+                                                         const uint32_t* pt_u, const double* pt_w, const double* pos,
+                                                         const double* target, const double* unow, const double* du,
+                                                         const double* imp, const double* cen, double* mrep,
+                                                         double* mtq, double* mslip, uint8_t* mact,
+                                                         const IbmDevDiag* diag, const double* rho, const double* ksum,
+                                                         double relax, double* imp_next) {
+  const int64_t m = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   if (m >= M) return;
   const uint32_t k = mk_inst[m];
   if (diag[k].converged) return;
+  const uint32_t p0 = mk_off[m], p1 = mk_off[m + 1];
   double rep[3] = {0.0, 0.0, 0.0}, um[3] = {0.0, 0.0, 0.0};
-  for (uint32_t p = mk_off[m]; p < mk_off[m + 1]; ++p) {
-    const double w = pt_w[p];
-    for (int a = 0; a < 3; ++a) rep[a] += imp[3 * m + a] * w;
-  }
   double mob = 0.0;
-  for (uint32_t p = mk_off[m]; p < mk_off[m + 1]; ++p) {
-    const uint32_t u = pt_u[p];
-    const double w = pt_w[p];
-    for (int a = 0; a < 3; ++a) um[a] += w * (unow[3 * (size_t)u + a] + du[3 * (size_t)u + a]);
-    if (imp_next) mob += w * ksum[u] / fmax(rho[u], 1.0e-30);
+  double w0 = 0.0;  // this lane's point weight of the first chunk
+  marker_point_sums(p0, p1, pt_u, pt_w, unow, du, rho, ksum, imp_next != nullptr, um, &mob, &w0);
+  double im[3];
+  for (int a = 0; a < 3; ++a) im[a] = imp[3 * m + a];
+  for (uint32_t base = p0; base < p1; base += 64) {
+    const uint32_t p = base + (threadIdx.x & 63);
+    const double w = base == p0 ? w0 : (p < p1 ? pt_w[p] : 0.0);
+    const int n = (int)min(64u, p1 - base);
+    for (int j = 0; j < n; ++j) {
+      const double wj = lane_f64(w, j);
+      for (int a = 0; a < 3; ++a) rep[a] += im[a] * wj;
+    }
   }
+  if ((threadIdx.x & 63) != 0) return;
   if (imp_next)
     for (int a = 0; a < 3; ++a) {
       const double slip = target[3 * m + a] - um[a];
@@ -272,7 +335,7 @@ __global__ void k_ibm_marker_diag(int64_t M, const uint32_t* mk_off, const uint3
   for (int a = 0; a < 3; ++a) mrep[3 * m + a] = rep[a];
   const double s[3] = {target[3 * m] - um[0], target[3 * m + 1] - um[1], target[3 * m + 2] - um[2]};
   mslip[m] = sqrt(0.0 + s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
-  mact[m] = mk_off[m + 1] > mk_off[m] ? 1 : 0;
+  mact[m] = p1 > p0 ? 1 : 0;
 }
 
 // one block per instance: fixed-tree reduction over its markers and the
@@ -289,15 +352,41 @@ __global__ __launch_bounds__(256) void k_ibm_reduce(const uint32_t* moff, const 
   __syncthreads();
   if (conv) return;
   double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // tq[3], rep[3], slipmax, ssq, wsum
-  for (int64_t m = (int64_t)moff[k] + threadIdx.x; m < (int64_t)moff[k + 1]; m += 256) {
-    for (int a = 0; a < 3; ++a) {
-      acc[a] += mtq[3 * m + a];
-      acc[3 + a] += mrep[3 * m + a];
+  // each thread's markers in their order, four at a time with every load
+  // issued before the first add (the loads are independent; one far-cache
+  // latency per four markers instead of one per marker)
+  const int64_t m1 = moff[k + 1];
+  for (int64_t mb = (int64_t)moff[k] + threadIdx.x; mb < m1; mb += 4 * 256) {
+    double tq[4][3], rp[4][3], sl[4], wt[4];
+    bool ac[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int64_t m = mb + 256 * j;
+      ac[j] = false;
+      sl[j] = wt[j] = 0.0;
+      for (int a = 0; a < 3; ++a) tq[j][a] = rp[j][a] = 0.0;
+      if (m < m1) {
+        for (int a = 0; a < 3; ++a) {
+          tq[j][a] = mtq[3 * m + a];
+          rp[j][a] = mrep[3 * m + a];
+        }
+        ac[j] = mact[m] != 0;
+        sl[j] = mslip[m];
+        wt[j] = weight[m];
+      }
     }
-    if (mact[m]) {
-      acc[6] = fmax(acc[6], mslip[m]);
-      acc[7] += weight[m] * mslip[m] * mslip[m];
-      acc[8] += weight[m];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      if (mb + 256 * j >= m1) break;
+      for (int a = 0; a < 3; ++a) {
+        acc[a] += tq[j][a];
+        acc[3 + a] += rp[j][a];
+      }
+      if (ac[j]) {
+        acc[6] = fmax(acc[6], sl[j]);
+        acc[7] += wt[j] * sl[j] * sl[j];
+        acc[8] += wt[j];
+      }
     }
   }
   for (int j = 0; j < 9; ++j) sh[j][threadIdx.x] = acc[j];
@@ -336,14 +425,6 @@ __global__ __launch_bounds__(256) void k_ibm_reduce(const uint32_t* moff, const 
 }
 
 // force_field += T(spread)
-__global__ void k_ibm_commit(uint32_t U, const uint32_t* uc_cell, const uint32_t* uc_inst, const IbmDomain* dom,
-                             const double* spread) {
-  const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
-  if (u >= U) return;
-  float* F = dom[uc_inst[u]].F_cur;
-  const size_t i = uc_cell[u];
-  for (int a = 0; a < 3; ++a) F[3 * i + a] = F[3 * i + a] + (float)spread[3 * (size_t)u + a];
-}
 
 __global__ void k_zero_integ(double* p, int n) {
   for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0.0;
@@ -354,8 +435,8 @@ __global__ void k_zero_integ(double* p, int n) {
 // synchronisation), then advance the ring position (ring may be null); the
 // running f64 torque integral of each instance advances by this update's
 // torque
-__global__ void k_ibm_record(const IbmDevDiag* diag, int K, IbmDevDiag* ring, uint32_t cap, uint32_t* pos,
-                             double* integ) {
+__device__ __forceinline__ void ibm_record_block(const IbmDevDiag* diag, int K, IbmDevDiag* ring, uint32_t cap,
+                                                 uint32_t* pos, double* integ) {
   __shared__ uint32_t slot;
   if (threadIdx.x == 0 && ring) slot = *pos;
   __syncthreads();
@@ -366,6 +447,25 @@ __global__ void k_ibm_record(const IbmDevDiag* diag, int K, IbmDevDiag* ring, ui
   }
   __syncthreads();
   if (threadIdx.x == 0 && ring) *pos = slot + 1;
+}
+
+// The wide path's last launch of an update, two independent jobs: blocks
+// 0 .. nb-1 add the spread to the force field at the stencil cells
+// (force_field += T(spread)), the last block records the diagnostics.
+__global__ __launch_bounds__(256) void k_ibm_commit_record(uint32_t U, uint32_t nb, const uint32_t* uc_cell,
+                                                           const uint32_t* uc_inst, const IbmDomain* dom,
+                                                           const double* spread, const IbmDevDiag* diag, int K,
+                                                           IbmDevDiag* ring, uint32_t cap, uint32_t* pos,
+                                                           double* integ) {
+  if (blockIdx.x == nb) {
+    ibm_record_block(diag, K, ring, cap, pos, integ);
+    return;
+  }
+  const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= U) return;
+  float* F = dom[uc_inst[u]].F_cur;
+  const size_t i = uc_cell[u];
+  for (int a = 0; a < 3; ++a) F[3 * i + a] = F[3 * i + a] + (float)spread[3 * (size_t)u + a];
 }
 
 // ---------------------------------------------------------------------------
@@ -1052,8 +1152,10 @@ __global__ void k_copy_d(double* dst, const double* src, size_t n) {
   if (k < n) dst[k] = src[k];
 }
 
-// wide path: one kernel per phase across the whole GPU (13 launches per
-// step at three sweeps), for bodies whose state does not fit in LDS
+// wide path: one kernel per phase across the whole GPU (12 launches per
+// step at three sweeps: gather, impulse, three times spread, diagnostics and
+// reduction, then commit with record), for bodies whose state does not fit
+// in LDS
 static int ibm_enqueue_wide(const FieldsView& v, IbmState* s, void* ring, uint32_t cap, uint32_t* pos, bool record,
                             const uint32_t* clr, uint32_t nclr) {
   if (!s->dom_valid) {
@@ -1085,32 +1187,31 @@ static int ibm_enqueue_wide(const FieldsView& v, IbmState* s, void* ring, uint32
     AUXCHK(v, hipGetLastError());
   }
   const uint32_t gm = nblk((size_t)s->M), gu = nblk(s->U);
+  const uint32_t gm4 = nblk((size_t)s->M, 4), gu4 = nblk(s->U, 4);  // one wave per marker / cell
   for (int it = 0; it < s->cfg.max_iterations; ++it) {
     if (record && s->U)
       hipLaunchKernelGGL(k_copy_d, dim3(nblk(3 * (size_t)s->U)), dim3(256), 0, st,
                          s->du_hist + 3 * (size_t)s->U * it, s->du, 3 * (size_t)s->U);
     if (it == 0)  // later sweeps take the impulses the previous sweep's diagnostics computed
-      hipLaunchKernelGGL(k_ibm_impulse, dim3(gm), dim3(256), 0, st, s->M, s->mk_off, s->mk_inst, s->pt_u, s->pt_w,
+      hipLaunchKernelGGL(k_ibm_impulse, dim3(gm4), dim3(256), 0, st, s->M, s->mk_off, s->mk_inst, s->pt_u, s->pt_w,
                          s->target, s->unow, s->du, s->rho, s->ksum, s->cfg.relaxation, s->imp, s->diag);
     double* nxt = it + 1 < s->cfg.max_iterations ? s->imp_next : nullptr;
     if (record)
       hipLaunchKernelGGL(k_copy_d, dim3(nblk(3 * (size_t)s->M)), dim3(256), 0, st,
                          s->imp_hist + 3 * (size_t)s->M * it, s->imp, 3 * (size_t)s->M);
     if (s->U)
-      hipLaunchKernelGGL(k_ibm_spread, dim3(gu), dim3(256), 0, st, s->U, s->uc_off, s->uc_inst, s->uc_ent, s->pt_m,
+      hipLaunchKernelGGL(k_ibm_spread, dim3(gu4), dim3(256), 0, st, s->U, s->uc_off, s->uc_inst, s->uc_ent, s->pt_m,
                          s->pt_w, s->imp, s->rho, s->spread, s->du, s->diag);
-    hipLaunchKernelGGL(k_ibm_marker_diag, dim3(gm), dim3(256), 0, st, s->M, s->mk_off, s->mk_inst, s->pt_u, s->pt_w,
+    hipLaunchKernelGGL(k_ibm_marker_diag, dim3(gm4), dim3(256), 0, st, s->M, s->mk_off, s->mk_inst, s->pt_u, s->pt_w,
                        s->pos, s->target, s->unow, s->du, s->imp, s->cen, s->mrep, s->mtq, s->mslip, s->mact,
                        s->diag, s->rho, s->ksum, s->cfg.relaxation, nxt);
     hipLaunchKernelGGL(k_ibm_reduce, dim3(s->K), dim3(256), 0, st, s->moff, s->mrep, s->mtq, s->mslip, s->mact,
                        s->weight, it, s->scale, s->cfg.slip_tolerance, s->diag, nxt, s->imp);
     AUXCHK(v, hipGetLastError());
   }
-  if (s->U) {
-    hipLaunchKernelGGL(k_ibm_commit, dim3(gu), dim3(256), 0, st, s->U, s->uc_cell, s->uc_inst, s->dom, s->spread);
-    AUXCHK(v, hipGetLastError());
-  }
-  hipLaunchKernelGGL(k_ibm_record, dim3(1), dim3(64), 0, st, s->diag, s->K, (IbmDevDiag*)ring, cap, pos, s->integ);
+  const uint32_t nb = s->U ? gu : 0u;
+  hipLaunchKernelGGL(k_ibm_commit_record, dim3(nb + 1), dim3(256), 0, st, s->U, nb, s->uc_cell, s->uc_inst, s->dom,
+                     s->spread, s->diag, s->K, (IbmDevDiag*)ring, cap, pos, s->integ);
   AUXCHK(v, hipGetLastError());
   return LBM_OK;
 }
