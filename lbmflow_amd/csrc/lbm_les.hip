@@ -44,6 +44,7 @@ struct WaleArgs {
   float* omega;       // N: omega_plus out
   float* nut;         // N or null: nu_t out (after the clip)
   uint32_t* stats;    // null or [clipped cells, bits of the largest raw nu_t]
+  uint32_t zlen;      // cells each thread of the second pass marches along z
 };
 
 // Population loads marked non-temporal: each population is read once, so
@@ -249,9 +250,11 @@ __device__ __forceinline__ float wale_nut(const WaleArgs& a, const float own[3],
 
 // Pass 2: a block is a 64 x 4 tile of (x, y) columns, each thread marches
 // its column over `zlen` cells keeping the mask and cached velocity of
-// z - 1, z, z + 1 in registers, so every velocity is read from HBM once; the
-// x and y neighbours come from the rows the tile's own threads load (L2).
-constexpr uint32_t kWaleTileX = 64, kWaleTileY = 4, kWaleZLen = 64;
+// z - 1, z, z + 1 in registers, so every velocity is read from HBM about
+// once; the x and y neighbours come from the rows the tile's own threads
+// load (L2).  Each step of the march waits for its loads, so the march is
+// kept short enough for the grid to hold many waves (wale_zlen).
+constexpr uint32_t kWaleTileX = 64, kWaleTileY = 4;
 static_assert(kWaleTileX * kWaleTileY == 256 && kWaleTileX == 64, "one wave per tile row, 256-thread blocks");
 
 template <int Q, typename S>
@@ -259,8 +262,8 @@ __global__ __launch_bounds__(256) void k_wale(WaleArgs a) {
   const uint32_t nx = a.d.nx, ny = a.d.ny, nz = a.d.nz;
   const uint32_t x = blockIdx.x * kWaleTileX + (threadIdx.x & (kWaleTileX - 1));
   const uint32_t y = blockIdx.y * kWaleTileY + threadIdx.x / kWaleTileX;
-  const uint32_t z0 = blockIdx.z * kWaleZLen;
-  const uint32_t z1 = min(z0 + kWaleZLen, nz);
+  const uint32_t z0 = blockIdx.z * a.zlen;
+  const uint32_t z1 = min(z0 + a.zlen, nz);
   const size_t N = a.d.N;
   const uint32_t plane = nx * ny;
   uint32_t cnt = 0u, mx = 0u;
@@ -341,11 +344,27 @@ __global__ __launch_bounds__(256) void k_wale(WaleArgs a) {
   }
 }
 
+// March length of the second pass: the longest power of two up to 64 that
+// still gives the grid 65,536 blocks, at least 4 (measured best: 4 cells at
+// 256^3, 8 at 512^3; 64 cost 4.6 % and 1.5 % of the whole step there);
+// LBM_WALE_ZLEN overrides it (tuning).  Any length gives the same values.
+static uint32_t wale_zlen(const Dom& d) {
+  if (const char* e = getenv("LBM_WALE_ZLEN")) {
+    const long v = strtol(e, nullptr, 10);
+    if (v >= 1 && v <= 1024) return (uint32_t)v;
+  }
+  const uint64_t cols = (uint64_t)((d.nx + kWaleTileX - 1) / kWaleTileX) * ((d.ny + kWaleTileY - 1) / kWaleTileY);
+  uint32_t z = 64;
+  while (z > 4 && cols * ((d.nz + z - 1) / z) < 65536) z >>= 1;
+  return z;
+}
+
 template <typename S>
-static hipError_t wale_s(int Q, const WaleArgs& a, hipStream_t s) {
+static hipError_t wale_s(int Q, WaleArgs a, hipStream_t s) {
+  a.zlen = wale_zlen(a.d);
   const dim3 grid((a.d.N + 255) / 256), block(256);
   const dim3 tiles((a.d.nx + kWaleTileX - 1) / kWaleTileX, (a.d.ny + kWaleTileY - 1) / kWaleTileY,
-                   (a.d.nz + kWaleZLen - 1) / kWaleZLen);
+                   (a.d.nz + a.zlen - 1) / a.zlen);
   if (Q == 19) {
     hipLaunchKernelGGL((k_wale_local<19, S>), grid, block, 0, s, a);
     hipLaunchKernelGGL((k_wale<19, S>), tiles, block, 0, s, a);

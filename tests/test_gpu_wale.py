@@ -113,11 +113,13 @@ def test_force_field_enters_the_stress():
 
 
 @pytest.mark.parametrize("periodic", [(1, 1, 1), (0, 1, 0)])
-def test_tall_column_spans_several_z_chunks(periodic):
-    """A domain taller than the gradient pass's z chunk (64 cells) with x and
-    y extents that fill no tile: the chunk seams, the wrapped z neighbours
+def test_tall_column_spans_several_z_chunks(periodic, monkeypatch):
+    """A domain taller than the gradient pass's z march (the default length
+    for the size, then 64, 7 and 1 cells through LBM_WALE_ZLEN) with x and
+    y extents that fill no tile: the march seams, the wrapped z neighbours
     of a periodic axis and the one-sided ends of a closed one, with an
-    obstacle column crossing the seams."""
+    obstacle column crossing the seams.  Every march length gives the same
+    field bit for bit."""
     dims = (70, 5, 150)
     nx, ny, nz = dims
     solid = np.zeros((nz, ny, nx), np.uint8)
@@ -136,6 +138,11 @@ def test_tall_column_spans_several_z_chunks(periodic):
     hip.run(3)
     nut, _ = _compare(orc, hip, f"tall column {periodic}")
     assert np.sum(nut > 0) > 1000
+    om = hip.omega_field()
+    for zlen in ("64", "7", "1"):
+        monkeypatch.setenv("LBM_WALE_ZLEN", zlen)
+        hip.wale_update()
+        assert np.array_equal(hip.omega_field(), om), f"march length {zlen}"
 
 
 def test_per_step_closure_follows_the_update_loop():
