@@ -418,10 +418,22 @@ __global__ __launch_bounds__(256) void k_ibm_reduce(const uint32_t* moff, const 
     if (diag->slip_max_rel <= tol) diag->converged = 1;
     conv = diag->converged;
   }
-  // an instance that goes on takes the next sweep's impulses
+  // an instance that goes on takes the next sweep's impulses: 32 loads
+  // issued before their 32 stores (the two arrays could alias as far as the
+  // compiler knows, so a plain loop waits for each load in turn)
   __syncthreads();
-  if (imp_next && !conv)
-    for (int64_t e = 3 * (int64_t)moff[k] + threadIdx.x; e < 3 * (int64_t)moff[k + 1]; e += 256) imp[e] = imp_next[e];
+  if (imp_next && !conv) {
+    constexpr int kBatch = 32;
+    const int64_t e1 = 3 * (int64_t)moff[k + 1];
+    for (int64_t e = 3 * (int64_t)moff[k] + threadIdx.x; e < e1; e += kBatch * 256) {
+      double v[kBatch];
+#pragma unroll
+      for (int j = 0; j < kBatch; ++j) v[j] = e + 256 * j < e1 ? imp_next[e + 256 * j] : 0.0;
+#pragma unroll
+      for (int j = 0; j < kBatch; ++j)
+        if (e + 256 * j < e1) imp[e + 256 * j] = v[j];
+    }
+  }
 }
 
 // force_field += T(spread)
